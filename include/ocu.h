@@ -1,0 +1,99 @@
+/*
+ * ocu.h - C ABI of libocu.so, the host drop-in classes (namespace ocu, the
+ * reference's class names and signatures) behind opaque handles, for callers
+ * that are not C++ (the Python package binds these with ctypes).
+ *
+ * Grids passed in are device grids owned by the caller (ocu_grid3d with a
+ * device base pointer and the Grid3D layout); they must outlive the handle,
+ * as the reference solvers keep raw pointers to their inputs
+ * (sol_mgpressure3ddev.cu:418, sol_divergence3ddev.cu:95-98).  Grids handed
+ * out (pressure, divergence, fields) are owned by the handle.
+ *
+ * Return values: handles are NULL on failure; ints are OCU_OK (0) or a
+ * negative ocu_status, except *_solve which return 1 (converged / true),
+ * 0 (not converged / false) or a negative status.  ocu_last_error() (from
+ * ocu_hip.h) holds the message of a negative status.
+ */
+#ifndef OCU_H
+#define OCU_H
+
+#include "ocu_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* stream every solver enqueues on (ThreadManager::get_compute_stream) */
+int ocu_set_compute_stream(void *stream);
+void *ocu_get_compute_stream(void);
+
+/* ---- Sol_MultigridPressure3DDevice<T> (ocuequation/sol_mgpressure3d.h:36-181) ----
+ * T follows rhs->dtype.  initialize_storage(nx,ny,nz,hx,hy,hz,&rhs) with the
+ * public fields taken from cfg.  use_fused_relax selects the one-launch
+ * red+black sweep where the boundary conditions allow it. */
+void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
+                    int use_fused_relax);
+int ocu_mg_pressure(void *h, ocu_grid3d *out);
+/* solve(residual, tol, max_iter) (sol_mgpressure3d.cpp:332-370) */
+int ocu_mg_solve(void *h, double tol, int max_iter, double *residual);
+int ocu_mg_run_vcycles(void *h, int count);
+int ocu_mg_run_fmg(void *h, int count);
+/* level count, V-cycles of the last call and the level-0 residual history */
+int ocu_mg_stats(void *h, int *num_levels, int *vcycles, double *history, int cap, int *len);
+void ocu_mg_destroy(void *h);
+
+/* ---- Sol_MultigridPressureMixed3DDeviceD: FP32 V-cycles in FP64 iterative
+ * refinement behind the reference's mixed-solver API
+ * (ocuequation/sol_mgpressuremixed3d.h).  rhs is FP64.  cycles /
+ * cycles_first are V-cycles per refinement iteration; fmg_first starts the
+ * first iteration with an FP32 FMG. */
+void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
+                         int cycles, int cycles_first, int fmg_first);
+int ocu_mgmixed_pressure(void *h, ocu_grid3d *out);
+/* tol is absolute in cfg->convergence's norm; max_iter bounds the
+ * refinement iterations */
+int ocu_mgmixed_solve(void *h, double tol, int max_iter, double *residual);
+int ocu_mgmixed_stats(void *h, int *ir_iterations, int *vcycles, double *rhs_rms, double *history, int cap,
+                      int *len);
+void ocu_mgmixed_destroy(void *h);
+
+/* ---- Sol_ProjectDivergence3DDevice<T> (ocuequation/sol_project3d.h:74-93) ----
+ * u, v, w: congruent MAC grids (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1).
+ * mixed = 1 (FP64 u) uses the mixed-precision pressure solver. */
+void *ocu_project_create(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc,
+                         int nx, int ny, int nz, double hx, double hy, double hz, int mixed);
+int ocu_project_solve(void *h, double tol, double *residual);
+int ocu_project_solve_divergence_only(void *h);
+int ocu_project_divergence(void *h, ocu_grid3d *out);
+int ocu_project_pressure(void *h, ocu_grid3d *out);
+void ocu_project_destroy(void *h);
+
+/* ---- Eqn_IncompressibleNS3D<T> (ocuequation/eqn_incompressns3d.h:33-175) ---- */
+typedef struct ocu_ns_params {
+  int32_t nx, ny, nz;
+  double hx, hy, hz;
+  double max_divergence;
+  ocu_bc_set temp_bc, flow_bc;
+  int32_t vertical_direction;
+  double gravity, bouyancy, thermal_diffusion, viscosity;
+  int32_t advection_scheme, time_step;
+  double cfl_factor;
+} ocu_ns_params;
+
+/* init_* are host buffers laid out by Eqn_IncompressibleNS3DParams::init_grids
+ * (eqn_incompressns3d.h:78-86), element type by dtype. */
+void *ocu_ns_create(int dtype, const ocu_ns_params *p, const void *init_u, const void *init_v,
+                    const void *init_w, const void *init_temp);
+int ocu_ns_advance_one_step(void *h, double dt);
+int ocu_ns_advance(void *h, double dt);
+int ocu_ns_max_stable_timestep(void *h, double *dt);
+/* which: 0 u, 1 v, 2 w, 3 temperature, 4 pressure (device descriptors) */
+int ocu_ns_field(void *h, int which, ocu_grid3d *out);
+int ocu_ns_num_steps(void *h);
+void ocu_ns_destroy(void *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCU_H */

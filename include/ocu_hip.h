@@ -1,0 +1,204 @@
+/*
+ * ocu_hip.h - C ABI of libocu_hip.so, the hand-written HIP kernels (gfx950)
+ * behind OpenCurrent's projection-method hot path.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (ocu_status, 0 = OK).  On error
+ *    ocu_last_error() returns a thread-local message.
+ *  - Grids are ocu_grid3d descriptors whose base points to device memory.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Kernels are enqueued asynchronously; nothing synchronises unless the
+ *    entry point says so.  Entry points never allocate device memory: any
+ *    scratch space is an explicit argument.
+ *  - Each kernel entry point names the reference routine it replaces; the
+ *    reference binds these as CUDA launches inside the named C++ methods.
+ */
+#ifndef OCU_HIP_H
+#define OCU_HIP_H
+
+#include "ocu_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- runtime: errors, memory, streams, events, graphs ------- */
+
+/* thread-local text of the last error (empty string if none) */
+const char *ocu_last_error(void);
+int ocu_get_device_count(int *count);
+int ocu_set_device(int device);
+int ocu_device_synchronize(void);
+
+/* device memory (hipMalloc / hipFree): replaces ocuutil/memory.cu:26-91 and
+ * the cudaMalloc in Grid3DDevice<T>::init (ocustorage/grid3ddev.cu:391). */
+int ocu_malloc(void **ptr, size_t bytes);
+int ocu_free(void *ptr);
+/* pinned host memory (hipHostMalloc), host_malloc(pinned=true) in memory.cu */
+int ocu_host_malloc(void **ptr, size_t bytes);
+int ocu_host_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device, 3 host->host.
+ * Asynchronous on `stream` (cudaMemcpy in grid3ddev.cu:205, :245-265). */
+int ocu_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void *stream);
+
+int ocu_stream_create(void **stream);
+int ocu_stream_destroy(void *stream);
+int ocu_stream_synchronize(void *stream);
+
+int ocu_event_create(void **event);
+int ocu_event_destroy(void *event);
+int ocu_event_record(void *event, void *stream);
+int ocu_event_synchronize(void *event);
+int ocu_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* HIP-graph capture of everything enqueued on `stream` between begin and
+ * end; end instantiates the graph and returns the executable. */
+int ocu_stream_begin_capture(void *stream);
+int ocu_stream_end_capture(void *stream, void **graph_exec);
+int ocu_graph_launch(void *graph_exec, void *stream);
+int ocu_graph_destroy(void *graph_exec);
+
+/* ---------------- whole-grid operations (ocustorage/grid3ddev.cu) -------- */
+
+/* every allocated element = value; Grid3DDevice<T>::clear / clear_zero
+ * (grid3ddev.cu:177-195).  A fill kernel, so it is safe inside a captured
+ * graph. */
+int ocu_grid_clear(const ocu_grid3d *g, double value, void *stream);
+/* dst[n] = (T_dst) src[n] over the allocation; copy_all_data<T,S>
+ * (grid3ddev.cu:25-33, 216-281).  Layouts must match. */
+int ocu_grid_copy(const ocu_grid3d *dst, const ocu_grid3d *src, void *stream);
+/* r = a1*g1 (+ a2*g2); Grid3DDevice<T>::linear_combination
+ * (grid3ddev.cu:283-349) including its 3%-waste rule for choosing between
+ * an interior-only and a whole-allocation update.  g2 may be NULL. */
+int ocu_grid_lincomb(const ocu_grid3d *r, double a1, const ocu_grid3d *g1, double a2,
+                     const ocu_grid3d *g2, void *stream);
+/* interior += value (mean removal) */
+int ocu_grid_add_scalar(const ocu_grid3d *g, double value, void *stream);
+
+/* Reductions over the interior; Grid3DDevice<T>::reduce_* (grid3ddev.cu:93-174,
+ * grid3dops.cu:26-130).  op: 0 maxabs, 1 max, 2 min, 3 sum, 4 sqrsum,
+ * 5 checknan.  Deterministic two-stage tree, accumulated in double.  The
+ * result (one double) is written to device memory `out`; `workspace` must
+ * hold ocu_reduce_workspace_bytes() bytes of device memory. */
+size_t ocu_reduce_workspace_bytes(void);
+int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, void *stream);
+
+/* ---------------- ghost-cell fills (ocustorage/grid3dboundary.cu) -------- */
+
+/* apply_3d_boundary_conditions_level1 (grid3dboundary.cu:758-820): same
+ * values as the reference's x, y, z passes, computed in one launch. */
+int ocu_bc_fill_level1(const ocu_grid3d *g, const ocu_bc_set *bc, double hx, double hy,
+                       double hz, void *stream);
+/* apply_3d_boundary_conditions_level1_nocorners, device version
+ * (grid3dboundary.cu:28-119, 719-753) */
+int ocu_bc_fill_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, double hx, double hy,
+                          double hz, void *stream);
+/* apply_3d_mac_boundary_conditions_level1 (grid3dboundary.cu:825-900) */
+int ocu_bc_fill_mac_level1(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                           const ocu_bc_set *bc, void *stream);
+
+/* ---------------- multigrid (ocuequation/sol_mgpressure3ddev*.cu) -------- */
+
+/* One colour (cells with (i+j+k)%2 == color) of red-black Gauss-Seidel,
+ * in place.  variant OCU_RELAX_DEVICE = Sol_MultigridPressure3DDeviceD_relax_DIAG
+ * (devd.cu:542-599) for FP64 grids / Sol_MultigridPressure3DDeviceF_relax
+ * (devf.cu:200-252) for FP32 grids; OCU_RELAX_HOST = do_cpu_solve
+ * (sol_mgpressure3ddev.cu:82-125) arithmetic. */
+int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color,
+                 const ocu_mg_level_params *p, int variant, void *stream);
+/* One full iteration - colour first_color then the other - with the
+ * OCU_RELAX_DEVICE arithmetic, reading u_in and writing the interior of u_out
+ * (congruent with u_in, must not alias it; the ghosts of u_out are not
+ * written).  Bit-identical to two ocu_mg_relax calls when no face is
+ * periodic (the reference refreshes ghosts between colours only for periodic
+ * faces, sol_mgpressure3ddev.cu:227-241); returns OCU_ERR_UNSUPPORTED if a
+ * face of `bc` is periodic. */
+int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                       const ocu_mg_level_params *p, const ocu_bc_set *bc, int first_color,
+                       void *stream);
+/* r = b - A u over the interior; calculate_residual (devd.cu:94-114,
+ * devf.cu:68-90; constants as in the launchers devd.cu:1372-1405). */
+int ocu_mg_residual(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r, double h,
+                    double fx, double fy, double fz, void *stream);
+/* b_coarse = .125 * sum of the 8 children of r; restrict (devd.cu:65-91) */
+int ocu_mg_restrict(const ocu_grid3d *r, const ocu_grid3d *b_coarse, void *stream);
+/* Fused residual + restriction: the fine residual of each child is computed
+ * in registers, summed in the restrict kernel's order and written to
+ * b_coarse; r is written too when r != NULL.  If norm_ws != NULL, per-block
+ * partial (sum r^2, max|r|) in double go to norm_ws (OCU_NORM_PARTS pairs)
+ * for ocu_norm_finalize.  Same values as residual followed by restrict. */
+int ocu_mg_residual_restrict(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r,
+                             const ocu_grid3d *b_coarse, double h, double fx, double fy,
+                             double fz, double *norm_ws, void *stream);
+/* r = b - A u with optional norm partials (the level-0 convergence check of
+ * restrict_residuals(0,0,...), sol_mgpressure3ddev.cu:250-279) */
+int ocu_mg_residual_norm(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r,
+                         double h, double fx, double fy, double fz, double *norm_ws,
+                         void *stream);
+/* out[0] = sum r^2, out[1] = max |r| (device doubles) from the partials */
+int ocu_norm_finalize(const double *norm_ws, double *out, void *stream);
+/* u_fine += trilinear prolongation of u_coarse; prolong (devd.cu:118-223).
+ * quirk = 1 reproduces the reference corner choice (devd.cu:181-185). */
+int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, void *stream);
+/* Coarsest-level relaxation in one workgroup (replaces relax_on_host,
+ * sol_mgpressure3ddev.cu:134-188, and its two PCIe round trips).
+ * mode 0: reference semantics - `iterations` RB-GS iterations with the
+ *   do_cpu_solve arithmetic in the grid's type, the host no-corner ghost
+ *   fill after every colour, then apply_3d_boundary_conditions_level1.
+ * mode 1: FP32 grids relaxed in double (copy in, `iterations` host-order
+ *   RB-GS iterations with a level-1 ghost fill after every colour, round
+ *   back to float) - the coarsest solve of the mixed-precision solver.
+ * order: 0 red-black, 1 black-red, 2 symmetric (as relax()).  The grid must
+ * fit in 64 KiB of LDS (u and b in the compute type, ghosts included). */
+int ocu_mg_coarse_relax(const ocu_grid3d *u, const ocu_grid3d *b, int iterations, int order,
+                        const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx, double hy,
+                        double hz, int mode, void *stream);
+
+/* ---------------- mixed-precision iterative refinement -------------------- */
+
+/* r = (b - bshift) - A x in FP64 (x, b FP64), r32 = (float)(r*scale) (FP32,
+ * congruent); partial (sum r^2, max|r|, sum r) per block to ws
+ * (OCU_NORM_PARTS triples) for ocu_norm3_finalize. */
+int ocu_ir_residual(const ocu_grid3d *x, const ocu_grid3d *b, double bshift,
+                    const ocu_grid3d *r32, double scale, double h, double fx, double fy,
+                    double fz, double *ws, void *stream);
+int ocu_norm3_finalize(const double *ws, double *out, void *stream);
+/* x += (double)e * inv_scale over the interior (x FP64, e FP32) */
+int ocu_ir_correct(const ocu_grid3d *x, const ocu_grid3d *e, double inv_scale, void *stream);
+/* out[0] = interior sum of g (double) via ws (OCU_NORM_PARTS doubles) */
+int ocu_grid_sum(const ocu_grid3d *g, double *ws, double *out, void *stream);
+
+/* number of per-block partial slots used by the norm entry points */
+int ocu_norm_parts(void);
+
+/* ---------------- MAC-grid operators (ocuequation/sol_*3ddev.cu) ----------- */
+
+/* Sol_Divergence3DDevice_calculate_divergence (sol_divergence3ddev.cu:23-45) */
+int ocu_divergence(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                   const ocu_grid3d *div, int nx, int ny, int nz, double hx, double hy,
+                   double hz, void *stream);
+/* Sol_Gradient3DDevice_subtract_grad (sol_gradient3ddev.cu:24-47) over the
+ * cells of p */
+int ocu_gradient_sub(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                     const ocu_grid3d *p, double coefficient, double hx, double hy, double hz,
+                     void *stream);
+/* Advection3DF/D_apply_upwind_TEX (sol_selfadvection3ddev.cu:136-337) */
+int ocu_self_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                    const ocu_grid3d *du, const ocu_grid3d *dv, const ocu_grid3d *dw, int nx,
+                    int ny, int nz, double hx, double hy, double hz, int scheme, void *stream);
+/* Sol_PassiveAdvection3D_apply_interp (sol_passiveadvection3ddev.cu:31-78) */
+int ocu_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                       const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
+                       double hx, double hy, double hz, int scheme, void *stream);
+/* Sol_LaplacianCentered3DDevice_stencil (sol_laplaciancent3ddev.cu:27-62) */
+int ocu_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
+                      double hx, double hy, double hz, double coefficient, void *stream);
+/* Eqn_IncompressibleNS3D_add_thermal_force (eqn_incompressns3d.cu:25-44) */
+int ocu_thermal_force(const ocu_grid3d *d, const ocu_grid3d *t, int nx, int ny, int nz,
+                      int vertical_direction, double coefficient, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCU_HIP_H */

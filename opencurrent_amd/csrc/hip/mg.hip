@@ -1,0 +1,774 @@
+// Multigrid kernels of Sol_MultigridPressure3DDevice<T>
+// (ocuequation/sol_mgpressure3ddev.cu, sol_mgpressure3ddevd.cu, ...devf.cu).
+//
+// Layout reminder: k (the reference's z) has unit stride, so a wave of 64
+// lanes always walks 64 consecutive k of one (i,j) row.  All arithmetic
+// keeps the reference's operation order; the library is built with
+// -ffp-contract=off so the results match the CPU restatement bit for bit.
+#include "ocu_hip_common.h"
+
+using namespace ocu_hip;
+
+namespace {
+
+// ------------------------------------------------------------------------
+// relaxation constants, rounded exactly as each reference launcher does
+// ------------------------------------------------------------------------
+
+// Sol_MultigridPressure3DDeviceD_relax_DIAG launcher (devd.cu:1246-1268)
+struct DevC64 {
+  double fx, fy, fz, omega, ld, neg_hsq;
+  double m[6];
+};
+// Sol_MultigridPressure3DDeviceF_relax launcher (devf.cu:367-395)
+struct DevC32 {
+  float fx, fy, fz, omega, ld, hsq;
+  float m[6];
+};
+// do_cpu_solve (sol_mgpressure3ddev.cu:82-125)
+template <typename T>
+struct HostC {
+  double fx, fy, fz;
+  T omega, ld, h;
+  T m[6];
+};
+
+DevC64 dev_c64(const ocu_mg_level_params *p) {
+  DevC64 c;
+  c.fx = p->fx; c.fy = p->fy; c.fz = p->fz; c.omega = p->omega;
+  c.ld = (double)(2 * p->fx + 2 * p->fy + 2 * p->fz);
+  c.neg_hsq = (double)-p->h * p->h;
+  for (int q = 0; q < 6; q++) c.m[q] = p->mods[q];
+  return c;
+}
+DevC32 dev_c32(const ocu_mg_level_params *p) {
+  DevC32 c;
+  c.fx = (float)p->fx; c.fy = (float)p->fy; c.fz = (float)p->fz; c.omega = (float)p->omega;
+  c.ld = (float)(2 * p->fx + 2 * p->fy + 2 * p->fz);
+  c.hsq = (float)(p->h * p->h);
+  for (int q = 0; q < 6; q++) c.m[q] = (float)p->mods[q];
+  return c;
+}
+template <typename T>
+HostC<T> host_c(const ocu_mg_level_params *p) {
+  HostC<T> c;
+  c.fx = p->fx; c.fy = p->fy; c.fz = p->fz;
+  c.omega = (T)p->omega;
+  c.ld = 2 * p->fx + 2 * p->fy + 2 * p->fz;
+  c.h = (T)p->h;
+  for (int q = 0; q < 6; q++) c.m[q] = (T)p->mods[q];
+  return c;
+}
+
+template <typename C> struct DevCFor;
+template <> struct DevCFor<double> { typedef DevC64 type; };
+template <> struct DevCFor<float> { typedef DevC32 type; };
+
+// diagonal with boundary modifications, in the order of the diag table
+// (sol_mgpressure3ddev.cu:27-43) followed by the z terms (devd.cu:585-587)
+template <typename T, typename C>
+__device__ __forceinline__ T diag_dev(const C &c, int i, int j, int k, int nx, int ny, int nz) {
+  T d = c.ld;
+  if (i == nx - 1) d += c.m[0];
+  if (i == 0) d += c.m[1];
+  if (j == ny - 1) d += c.m[2];
+  if (j == 0) d += c.m[3];
+  if (k == nz - 1) d += c.m[4];
+  if (k == 0) d += c.m[5];
+  return d;
+}
+
+// relax_DIAG point update (devd.cu:572-591)
+__device__ __forceinline__ double relax_pt(const DevC64 &c, double uc, double zm, double zp, double ym,
+                                           double yp, double xm, double xp, double b, double diag) {
+  double r = -c.ld * uc;
+  r += c.fz * (zm + zp);
+  r += c.fy * (ym + yp);
+  r += c.fx * (xm + xp);
+  r += c.neg_hsq * b;
+  return uc + c.omega * r / diag;
+}
+// Sol_MultigridPressure3DDeviceF_relax point update (devf.cu:221-243)
+__device__ __forceinline__ float relax_pt(const DevC32 &c, float uc, float zm, float zp, float ym,
+                                          float yp, float xm, float xp, float b, float diag) {
+  float r = (-c.hsq * b - c.ld * uc);
+  r += c.fz * (zm + zp);
+  r += c.fy * (ym + yp);
+  r += c.fx * (xm + xp);
+  return uc + c.omega * r / diag;
+}
+// do_cpu_solve point update (sol_mgpressure3ddev.cu:106-113): neighbour
+// products in double, each partial sum rounded back to T
+template <typename T>
+__device__ __forceinline__ T relax_pt_host(const HostC<T> &c, T uc, T zm, T zp, T ym, T yp, T xm, T xp,
+                                           T b, T x_mod, T y_mod, T z_mod) {
+  T r = (-c.h * c.h * b - c.ld * uc);
+  r += c.fz * (zm + zp);
+  r += c.fy * (yp + ym);
+  r += c.fx * (xp + xm);
+  return uc + c.omega * r / (c.ld + x_mod + y_mod + z_mod);
+}
+
+// ------------------------------------------------------------------------
+// one colour, in place
+// ------------------------------------------------------------------------
+
+// One thread per cell of the (k,j) tile; cells of the other colour idle so
+// every row is read with full 64-lane coalescing.  The device variants only
+// update k < 2*(nz/2) (the reference launches nz/2 threads along k).
+template <typename T>
+__global__ __launch_bounds__(256) void k_relax_dev(DGrid<T> u, DGrid<T> b, typename DevCFor<T>::type c,
+                                                   int color) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= u.ny || k >= 2 * (u.nz >> 1)) return;
+  if (((i + j + k) & 1) != color) return;
+  int64_t id = u.idx(i, j, k);
+  T *p = u.p;
+  T uc = p[id];
+  T d = diag_dev<T>(c, i, j, k, u.nx, u.ny, u.nz);
+  p[id] = relax_pt(c, uc, p[id - 1], p[id + 1], p[id - u.ys], p[id + u.ys], p[id - u.xs], p[id + u.xs],
+                   b.p[b.idx(i, j, k)], d);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_relax_host(DGrid<T> u, DGrid<T> b, HostC<T> c, int color) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= u.ny || k >= u.nz) return;
+  if (((i + j + k) & 1) != color) return;
+  const int nx = u.nx, ny = u.ny, nz = u.nz;
+  T xm_ = (i == 0) ? c.m[1] : (i == nx - 1) ? c.m[0] : (T)0;
+  T ym_ = (j == 0) ? c.m[3] : (j == ny - 1) ? c.m[2] : (T)0;
+  T zm_ = (k == 0) ? c.m[5] : (k == nz - 1) ? c.m[4] : (T)0;
+  int64_t id = u.idx(i, j, k);
+  T *p = u.p;
+  p[id] = relax_pt_host(c, p[id], p[id - 1], p[id + 1], p[id - u.ys], p[id + u.ys], p[id - u.xs],
+                        p[id + u.xs], b.p[b.idx(i, j, k)], xm_, ym_, zm_);
+}
+
+// ------------------------------------------------------------------------
+// fused red+black iteration (out of place)
+// ------------------------------------------------------------------------
+//
+// A block owns an output tile of OK x OJ (k,j) columns and marches along i
+// over a chunk of planes.  Its 64 x 16 threads cover the tile plus a ring
+// of two columns: plane p's first colour (U1) is recomputed on the tile and
+// ring 1 from the old values U0 (planes p-1, p, p+1), then the second
+// colour of plane p-1 is computed on the tile from U1 (planes p-2, p-1, p).
+// The old values come only from u_in, so blocks never depend on each other
+// and the halo recomputation gives exactly the two-launch result.
+// Lateral neighbours go through LDS; the i-neighbours live in registers.
+
+constexpr int FBK = 64, FBJ = 16, FOK = FBK - 4, FOJ = FBJ - 4, FXCD = 8;
+
+template <typename T>
+__global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
+                                                          typename DevCFor<T>::type c, int first_color,
+                                                          int tiles_k, int tiles_j, int ichunk, int ntiles) {
+  __shared__ T sU0[FBJ][FBK];
+  __shared__ T sU1[2][FBJ][FBK];
+
+  // blocks are dealt round-robin to the 8 XCDs: give each XCD a contiguous
+  // run of tiles so neighbouring tiles share its L2
+  int per = gridDim.x / FXCD;
+  int logical = (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD);
+  if (logical >= ntiles) return;  // uniform per block
+  int tk = logical % tiles_k;
+  int rem = logical / tiles_k;
+  int tj = rem % tiles_j;
+  int ti = rem / tiles_j;
+
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int nx = uin.nx, ny = uin.ny, nz = uin.nz;
+  const int k = tk * FOK - 2 + tx;
+  const int j = tj * FOJ - 2 + ty;
+  const int i0 = ti * ichunk;
+  const int i1 = min(nx, i0 + ichunk);
+  const bool ring1 = tx >= 1 && tx <= FBK - 2 && ty >= 1 && ty <= FBJ - 2;
+  const bool tile = tx >= 2 && tx <= FBK - 3 && ty >= 2 && ty <= FBJ - 3;
+  const bool loadable = j >= -1 && j <= ny && k >= -1 && k <= nz;
+  const bool inner_jk = j >= 0 && j < ny && k >= 0 && k < nz;
+  const bool upd_k = k < 2 * (nz >> 1);
+  const int64_t col = (int64_t)j * uin.ys + k;
+  const int64_t bcol = (int64_t)j * b.ys + k;
+
+  auto ld0 = [&](int p) -> T {
+    return (loadable && p >= -1 && p <= nx) ? uin.p[(int64_t)p * uin.xs + col] : (T)0;
+  };
+
+  T a = ld0(i0 - 2), cc = ld0(i0 - 1);
+  T u1m2 = 0, u1m1 = 0, bprev = 0;
+  for (int p = i0 - 1; p <= i1; ++p) {
+    T n = ld0(p + 1);
+    const bool inner_p = inner_jk && p >= 0 && p < nx;
+    T bcur = (ring1 && inner_p) ? b.p[(int64_t)p * b.xs + bcol] : (T)0;
+    sU0[ty][tx] = cc;
+    __syncthreads();
+    T u1 = cc;
+    if (ring1 && inner_p && upd_k && (((p + j + k) & 1) == first_color)) {
+      T d = diag_dev<T>(c, p, j, k, nx, ny, nz);
+      u1 = relax_pt(c, cc, sU0[ty][tx - 1], sU0[ty][tx + 1], sU0[ty - 1][tx], sU0[ty + 1][tx], a, n, bcur, d);
+    }
+    sU1[p & 1][ty][tx] = u1;
+    __syncthreads();
+    const int q = p - 1;
+    if (tile && q >= i0 && inner_jk && q < nx) {
+      T v = u1m1;
+      if (upd_k && (((q + j + k) & 1) != first_color)) {
+        T d = diag_dev<T>(c, q, j, k, nx, ny, nz);
+        const int s = q & 1;
+        v = relax_pt(c, u1m1, sU1[s][ty][tx - 1], sU1[s][ty][tx + 1], sU1[s][ty - 1][tx], sU1[s][ty + 1][tx],
+                     u1m2, u1, bprev, d);
+      }
+      uout.p[(int64_t)q * uout.xs + col] = v;
+    }
+    a = cc; cc = n;
+    u1m2 = u1m1; u1m1 = u1;
+    bprev = bcur;
+  }
+}
+
+// ------------------------------------------------------------------------
+// residual, restriction, fused residual->restriction, norms
+// ------------------------------------------------------------------------
+
+template <typename T>
+struct ResC {
+  T fx, fy, fz, diag;  // f/h^2 and 2(fx+fy+fz)/h^2, rounded to T (devd.cu:1396-1399)
+};
+template <typename T>
+ResC<T> res_c(double h, double fx, double fy, double fz) {
+  ResC<T> r;
+  r.fx = (T)(fx / (h * h)); r.fy = (T)(fy / (h * h)); r.fz = (T)(fz / (h * h));
+  r.diag = (T)(2 * (fx + fy + fz) / (h * h));
+  return r;
+}
+
+// calculate_residual point (devd.cu:108-112)
+template <typename T>
+__device__ __forceinline__ T res_pt(const ResC<T> &c, const T *u, int64_t id, int64_t ys, int64_t xs, T b) {
+  return b - (c.fz * (u[id - 1] + u[id + 1]) + c.fy * (u[id - ys] + u[id + ys]) +
+              c.fx * (u[id - xs] + u[id + xs]) - c.diag * u[id]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual(DGrid<T> u, DGrid<T> b, DGrid<T> r, ResC<T> c) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= u.ny || k >= u.nz) return;
+  r.p[r.idx(i, j, k)] = res_pt(c, u.p, u.idx(i, j, k), u.ys, u.xs, b.p[b.idx(i, j, k)]);
+}
+
+// restrict (devd.cu:65-91): child order (0,0,0),(0,0,1),(0,1,0),(0,1,1),
+// (1,0,0),(1,0,1),(1,1,0),(1,1,1); `.125 * sum` with a double literal
+template <typename T>
+__device__ __forceinline__ T restrict_sum(T r000, T r001, T r010, T r011, T r100, T r101, T r110, T r111) {
+  T s = r000 + r001 + r010 + r011 + r100 + r101 + r110 + r111;
+  return (T)(.125 * s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(DGrid<T> r, DGrid<T> bc) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= bc.ny || k >= bc.nz) return;
+  int64_t id = r.idx(2 * i, 2 * j, 2 * k);
+  const T *p = r.p;
+  int64_t ys = r.ys, xs = r.xs;
+  bc.p[bc.idx(i, j, k)] = restrict_sum(p[id], p[id + 1], p[id + ys], p[id + 1 + ys], p[id + xs], p[id + 1 + xs],
+                                       p[id + ys + xs], p[id + 1 + ys + xs]);
+}
+
+// Fused: each thread owns a coarse cell, computes its 8 fine residuals and
+// their restriction.  With NORM the grid is capped at NORM_PARTS blocks
+// that loop over tiles and leave (sum r^2, max|r|) partials.
+template <typename T, bool NORM>
+__global__ __launch_bounds__(256) void k_res_restrict(DGrid<T> u, DGrid<T> b, DGrid<T> r, bool write_r,
+                                                      DGrid<T> bc, ResC<T> c, int tiles_k, int tiles_j,
+                                                      int ntiles, double *parts) {
+  double s2 = 0, mx = 0;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int tk = t % tiles_k, rem = t / tiles_k, tj = rem % tiles_j, i = rem / tiles_j;
+    int k = tk * TK + threadIdx.x;
+    int j = tj * TJ + threadIdx.y;
+    if (j >= bc.ny || k >= bc.nz) continue;
+    T rr[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int di = q >> 2, dj = (q >> 1) & 1, dk = q & 1;
+      int I = 2 * i + di, J = 2 * j + dj, K = 2 * k + dk;
+      rr[q] = res_pt(c, u.p, u.idx(I, J, K), u.ys, u.xs, b.p[b.idx(I, J, K)]);
+      if (write_r) r.p[r.idx(I, J, K)] = rr[q];
+      if (NORM) {
+        double x = (double)rr[q];
+        s2 += x * x;
+        mx = fmax(mx, fabs(x));
+      }
+    }
+    bc.p[bc.idx(i, j, k)] = restrict_sum(rr[0], rr[1], rr[2], rr[3], rr[4], rr[5], rr[6], rr[7]);
+  }
+  if (NORM) {
+    __shared__ double sh[2][4];
+    int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    s2 = wave_sum(s2);
+    mx = wave_max(mx);
+    if ((tid & 63) == 0) { sh[0][tid >> 6] = s2; sh[1][tid >> 6] = mx; }
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0, m = 0;
+      for (int w = 0; w < 4; w++) { a += sh[0][w]; m = fmax(m, sh[1][w]); }
+      parts[2 * blockIdx.x] = a;
+      parts[2 * blockIdx.x + 1] = m;
+    }
+  }
+}
+
+// residual over the fine grid with norm partials (no restriction)
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_norm(DGrid<T> u, DGrid<T> b, DGrid<T> r, ResC<T> c, int tiles_k,
+                                                  int tiles_j, int ntiles, double *parts) {
+  double s2 = 0, mx = 0;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int tk = t % tiles_k, rem = t / tiles_k, tj = rem % tiles_j, i = rem / tiles_j;
+    int k = tk * TK + threadIdx.x;
+    int j = tj * TJ + threadIdx.y;
+    if (j >= u.ny || k >= u.nz) continue;
+    T x = res_pt(c, u.p, u.idx(i, j, k), u.ys, u.xs, b.p[b.idx(i, j, k)]);
+    r.p[r.idx(i, j, k)] = x;
+    double d = (double)x;
+    s2 += d * d;
+    mx = fmax(mx, fabs(d));
+  }
+  __shared__ double sh[2][4];
+  int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  s2 = wave_sum(s2);
+  mx = wave_max(mx);
+  if ((tid & 63) == 0) { sh[0][tid >> 6] = s2; sh[1][tid >> 6] = mx; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, m = 0;
+    for (int w = 0; w < 4; w++) { a += sh[0][w]; m = fmax(m, sh[1][w]); }
+    parts[2 * blockIdx.x] = a;
+    parts[2 * blockIdx.x + 1] = m;
+  }
+}
+
+__global__ void k_norm_finalize(const double *parts, int n, double *out) {
+  __shared__ double sh[2][8];
+  double s = 0, m = 0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    s += parts[2 * q];
+    m = fmax(m, parts[2 * q + 1]);
+  }
+  s = wave_sum(s);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = s; sh[1][threadIdx.x >> 6] = m; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) { a += sh[0][w]; b = fmax(b, sh[1][w]); }
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+// ------------------------------------------------------------------------
+// prolongation + correction (devd.cu:118-223, devf.cu:94-197)
+// ------------------------------------------------------------------------
+
+template <typename T> struct PW;
+template <> struct PW<double> {
+  static constexpr double c0 = 0.421875, c1 = 0.140625, c2 = 0.046875, c3 = 0.015625;
+};
+template <> struct PW<float> {
+  static constexpr float c0 = 0.421875f, c1 = 0.140625f, c2 = 0.046875f, c3 = 0.015625f;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int quirk) {
+  int kc = blockIdx.x * TK + threadIdx.x;
+  int jc = blockIdx.y * TJ + threadIdx.y;
+  int ic = blockIdx.z;
+  if (jc >= uc.ny || kc >= uc.nz) return;
+  const T *q = uc.p;
+  const int64_t xs = uc.xs, ys = uc.ys;
+  const int64_t id = uc.idx(ic, jc, kc);
+  const T C = q[id];
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    const int dx = n >> 2, dy = (n >> 1) & 1, dz = n & 1;
+    const int64_t sx = dx ? xs : -xs, sy = dy ? ys : -ys, sz = dz ? 1 : -1;
+    const int64_t cx = (quirk && dx == 0 && dy == 1 && dz == 1) ? xs : sx;
+    T X = q[id + sx], Y = q[id + sy], Z = q[id + sz];
+    T XY = q[id + sx + sy], XZ = q[id + sx + sz], YZ = q[id + sy + sz];
+    T XYZ = q[id + cx + sy + sz];
+    T *f = &uf.at(2 * ic + dx, 2 * jc + dy, 2 * kc + dz);
+    *f = *f + PW<T>::c0 * C + PW<T>::c1 * (X + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+  }
+}
+
+// ------------------------------------------------------------------------
+// coarsest level in one workgroup (LDS resident)
+// ------------------------------------------------------------------------
+
+struct CoarseGeom {
+  int nx, ny, nz;
+  int lys, lxs;   // LDS strides of the (nx+2)(ny+2)(nz+2) box
+  int lshift;     // LDS offset of cell (0,0,0)
+};
+
+template <typename T>
+__device__ void lds_relax_color(T *u, const T *b, const CoarseGeom &g, const HostC<T> &c, int color) {
+  int n = g.nx * g.ny * g.nz;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    int k = t % g.nz, j = (t / g.nz) % g.ny, i = t / (g.nz * g.ny);
+    if (((i + j + k) & 1) != color) continue;
+    int id = g.lshift + i * g.lxs + j * g.lys + k;
+    T xm_ = (i == 0) ? c.m[1] : (i == g.nx - 1) ? c.m[0] : (T)0;
+    T ym_ = (j == 0) ? c.m[3] : (j == g.ny - 1) ? c.m[2] : (T)0;
+    T zm_ = (k == 0) ? c.m[5] : (k == g.nz - 1) ? c.m[4] : (T)0;
+    u[id] = relax_pt_host(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs],
+                          u[id + g.lxs], b[id], xm_, ym_, zm_);
+  }
+}
+
+struct FaceS {
+  int type;
+  float value;
+};
+struct FacesS {
+  FaceS f[6];  // xpos, xneg, ypos, yneg, zpos, zneg
+};
+
+__device__ __forceinline__ bool wr(const FaceS &f) {
+  return f.type == OCU_BC_PERIODIC || f.type == OCU_BC_DIRICHELET || f.type == OCU_BC_NEUMANN;
+}
+// host no-corner ghost (grid3dboundary.cu:1018-1111): Dirichlet in T
+template <typename T>
+__device__ __forceinline__ T xf_host(const FaceS &f, bool neg, T h, T a) {
+  if (f.type == OCU_BC_DIRICHELET) return -a + (T)2 * (T)f.value;
+  if (f.type == OCU_BC_NEUMANN) return neg ? a - h * (T)f.value : a + h * (T)f.value;
+  return a;
+}
+// level-1 ghost (grid3dboundary.cu:237-347): Dirichlet `-phi + 2 * value`
+template <typename T>
+__device__ __forceinline__ T xf_l1(const FaceS &f, bool neg, T h, T a) {
+  if (f.type == OCU_BC_DIRICHELET) return -a + (T)(2 * f.value);
+  if (f.type == OCU_BC_NEUMANN) return neg ? a - h * (T)f.value : a + h * (T)f.value;
+  return a;
+}
+__device__ __forceinline__ int srci(const FaceS &f, bool neg, int n) {
+  if (f.type == OCU_BC_PERIODIC) return neg ? n - 1 : 0;
+  return neg ? 0 : n - 1;
+}
+
+template <typename T>
+__device__ void lds_fill_nocorners(T *u, const CoarseGeom &g, const FacesS &F, T hx, T hy, T hz) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  auto A = [&](int i, int j, int k) -> T & { return u[g.lshift + i * g.lxs + j * g.lys + k]; };
+  int nZ = nx * ny, nY = nx * nz, nX = ny * nz;
+  for (int t = threadIdx.x; t < nZ + nY + nX; t += blockDim.x) {
+    if (t < nZ) {
+      int i = t / ny, j = t % ny;
+      if (wr(F.f[5])) A(i, j, -1) = xf_host(F.f[5], true, hz, A(i, j, srci(F.f[5], true, nz)));
+      if (wr(F.f[4])) A(i, j, nz) = xf_host(F.f[4], false, hz, A(i, j, srci(F.f[4], false, nz)));
+    } else if (t < nZ + nY) {
+      int q = t - nZ, i = q / nz, k = q % nz;
+      if (wr(F.f[3])) A(i, -1, k) = xf_host(F.f[3], true, hy, A(i, srci(F.f[3], true, ny), k));
+      if (wr(F.f[2])) A(i, ny, k) = xf_host(F.f[2], false, hy, A(i, srci(F.f[2], false, ny), k));
+    } else {
+      int q = t - nZ - nY, j = q / nz, k = q % nz;
+      if (wr(F.f[1])) A(-1, j, k) = xf_host(F.f[1], true, hx, A(srci(F.f[1], true, nx), j, k));
+      if (wr(F.f[0])) A(nx, j, k) = xf_host(F.f[0], false, hx, A(srci(F.f[0], false, nx), j, k));
+    }
+  }
+}
+
+// the level-1 fill, pass by pass (block barriers between passes)
+template <typename T>
+__device__ void lds_fill_level1(T *u, const CoarseGeom &g, const FacesS &F, T hx, T hy, T hz) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  auto A = [&](int i, int j, int k) -> T & { return u[g.lshift + i * g.lxs + j * g.lys + k]; };
+  for (int t = threadIdx.x; t < ny * nz; t += blockDim.x) {
+    int j = t / nz, k = t % nz;
+    if (wr(F.f[1])) A(-1, j, k) = xf_l1(F.f[1], true, hx, A(srci(F.f[1], true, nx), j, k));
+    if (wr(F.f[0])) A(nx, j, k) = xf_l1(F.f[0], false, hx, A(srci(F.f[0], false, nx), j, k));
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < (nx + 2) * nz; t += blockDim.x) {
+    int i = t / nz - 1, k = t % nz;
+    if (wr(F.f[3])) A(i, -1, k) = xf_l1(F.f[3], true, hy, A(i, srci(F.f[3], true, ny), k));
+    if (wr(F.f[2])) A(i, ny, k) = xf_l1(F.f[2], false, hy, A(i, srci(F.f[2], false, ny), k));
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) {
+    int i = t / (ny + 2) - 1, j = t % (ny + 2) - 1;
+    if (wr(F.f[5])) A(i, j, -1) = xf_l1(F.f[5], true, hz, A(i, j, srci(F.f[5], true, nz)));
+    if (wr(F.f[4])) A(i, j, nz) = xf_l1(F.f[4], false, hz, A(i, j, srci(F.f[4], false, nz)));
+  }
+  __syncthreads();
+}
+
+// colour sequence of relax() / relax_on_host (sol_mgpressure3ddev.cu:160-171)
+__device__ __forceinline__ int color_of(int order, int it, int iterations, int rb) {
+  return (order == 1 || (order == 2 && it >= iterations / 2)) ? !rb : rb;
+}
+
+// TS = grid storage type, TC = compute type.  MODE 0: reference semantics;
+// MODE 1: double arithmetic on float storage with level-1 fills.
+template <typename TS, typename TC, int MODE>
+__global__ __launch_bounds__(256) void k_coarse(DGrid<TS> ug, DGrid<TS> bg, CoarseGeom g, HostC<TC> c, FacesS F,
+                                                TC hx, TC hy, TC hz, int iterations, int order) {
+  extern __shared__ unsigned char smem[];
+  const int box = (g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  TC *u = (TC *)smem;
+  TC *b = u + box;
+  for (int t = threadIdx.x; t < box; t += blockDim.x) {
+    int k = t % (g.nz + 2) - 1, j = (t / (g.nz + 2)) % (g.ny + 2) - 1, i = t / ((g.nz + 2) * (g.ny + 2)) - 1;
+    u[t] = (TC)ug.at(i, j, k);
+    b[t] = (TC)bg.at(i, j, k);
+  }
+  __syncthreads();
+  for (int it = 0; it < iterations; it++)
+    for (int rb = 0; rb < 2; rb++) {
+      lds_relax_color<TC>(u, b, g, c, color_of(order, it, iterations, rb));
+      __syncthreads();
+      if (MODE == 0) {
+        lds_fill_nocorners<TC>(u, g, F, hx, hy, hz);
+        __syncthreads();
+      } else {
+        lds_fill_level1<TC>(u, g, F, hx, hy, hz);
+      }
+    }
+  if (MODE == 0) lds_fill_level1<TC>(u, g, F, hx, hy, hz);
+  for (int t = threadIdx.x; t < box; t += blockDim.x) {
+    int k = t % (g.nz + 2) - 1, j = (t / (g.nz + 2)) % (g.ny + 2) - 1, i = t / ((g.nz + 2) * (g.ny + 2)) - 1;
+    ug.at(i, j, k) = (TS)u[t];
+  }
+}
+
+FacesS faces_s(const ocu_bc_set *bc) {
+  FacesS F;
+  const ocu_bc *all[6] = {&bc->xpos, &bc->xneg, &bc->ypos, &bc->yneg, &bc->zpos, &bc->zneg};
+  for (int q = 0; q < 6; q++) F.f[q] = {all[q]->type, all[q]->value};
+  return F;
+}
+
+int check_pair(const ocu_grid3d *u, const ocu_grid3d *b, const char *what) {
+  if (int e = check_grid(u, what)) return e;
+  if (int e = check_grid(b, what)) return e;
+  OCU_REQUIRE(u->dtype == b->dtype && u->dtype != OCU_I32, "%s: dtype mismatch", what);
+  OCU_REQUIRE(u->nx == b->nx && u->ny == b->ny && u->nz == b->nz, "%s: interior dimension mismatch", what);
+  OCU_REQUIRE(u->gx >= 1 && u->gy >= 1 && u->gz >= 1, "%s: needs one ghost layer", what);
+  return OCU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color, const ocu_mg_level_params *p,
+                 int variant, void *stream) {
+  if (int e = check_pair(u, b, "ocu_mg_relax")) return e;
+  OCU_REQUIRE(p, "ocu_mg_relax: null params");
+  OCU_REQUIRE(color == 0 || color == 1, "ocu_mg_relax: bad color %d", color);
+  if ((int64_t)u->nx * u->ny * u->nz == 0) return OCU_OK;
+  dim3 grid = stencil_grid(u->nx, u->ny, u->nz);
+  if (variant == OCU_RELAX_DEVICE) {
+    if (u->dtype == OCU_F64)
+      k_relax_dev<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), dev_c64(p), color);
+    else
+      k_relax_dev<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), dev_c32(p), color);
+  } else if (variant == OCU_RELAX_HOST) {
+    if (u->dtype == OCU_F64)
+      k_relax_host<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), host_c<double>(p), color);
+    else
+      k_relax_host<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), host_c<float>(p), color);
+  } else {
+    set_error("ocu_mg_relax: bad variant %d", variant);
+    return OCU_ERR_INVALID;
+  }
+  OCU_HIP_LAUNCHED("ocu_mg_relax");
+}
+
+int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                       const ocu_mg_level_params *p, const ocu_bc_set *bc, int first_color, void *stream) {
+  if (int e = check_pair(u_in, b, "ocu_mg_relax_fused")) return e;
+  if (int e = check_grid(u_out, "ocu_mg_relax_fused(out)")) return e;
+  OCU_REQUIRE(layout_match(u_in, u_out) && u_in->dtype == u_out->dtype && u_in->nx == u_out->nx &&
+                  u_in->ny == u_out->ny && u_in->nz == u_out->nz,
+              "ocu_mg_relax_fused: u_out must be congruent with u_in");
+  OCU_REQUIRE(u_in->base != u_out->base, "ocu_mg_relax_fused: u_out must not alias u_in");
+  OCU_REQUIRE(p && bc, "ocu_mg_relax_fused: null params");
+  OCU_REQUIRE(first_color == 0 || first_color == 1, "ocu_mg_relax_fused: bad first_color %d", first_color);
+  if (any_periodic(bc)) {
+    set_error("ocu_mg_relax_fused: periodic faces need a ghost refresh between colours");
+    return OCU_ERR_UNSUPPORTED;
+  }
+  const int nx = u_in->nx, ny = u_in->ny, nz = u_in->nz;
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  int tiles_k = (nz + FOK - 1) / FOK, tiles_j = (ny + FOJ - 1) / FOJ;
+  // enough i-chunks to give every CU a few blocks, but long enough marches
+  int ichunk = nx >= 256 ? 16 : nx >= 64 ? 8 : 4;
+  int tiles_i = (nx + ichunk - 1) / ichunk;
+  int ntiles = tiles_k * tiles_j * tiles_i;
+  int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+  dim3 block(FBK, FBJ);
+  if (u_in->dtype == OCU_F64)
+    k_relax_fused<double><<<nblocks, block, 0, S(stream)>>>(dgrid<double>(u_in), dgrid<double>(u_out), dgrid<double>(b),
+                                                            dev_c64(p), first_color, tiles_k, tiles_j, ichunk, ntiles);
+  else
+    k_relax_fused<float><<<nblocks, block, 0, S(stream)>>>(dgrid<float>(u_in), dgrid<float>(u_out), dgrid<float>(b),
+                                                           dev_c32(p), first_color, tiles_k, tiles_j, ichunk, ntiles);
+  OCU_HIP_LAUNCHED("ocu_mg_relax_fused");
+}
+
+int ocu_mg_residual(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r, double h, double fx,
+                    double fy, double fz, void *stream) {
+  if (int e = check_pair(u, b, "ocu_mg_residual")) return e;
+  if (int e = check_grid(r, "ocu_mg_residual(r)")) return e;
+  OCU_REQUIRE(r->dtype == u->dtype && r->nx == u->nx && r->ny == u->ny && r->nz == u->nz,
+              "ocu_mg_residual: r mismatch");
+  if ((int64_t)u->nx * u->ny * u->nz == 0) return OCU_OK;
+  dim3 grid = stencil_grid(u->nx, u->ny, u->nz);
+  if (u->dtype == OCU_F64)
+    k_residual<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), dgrid<double>(r),
+                                                                res_c<double>(h, fx, fy, fz));
+  else
+    k_residual<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), dgrid<float>(r),
+                                                               res_c<float>(h, fx, fy, fz));
+  OCU_HIP_LAUNCHED("ocu_mg_residual");
+}
+
+int ocu_mg_restrict(const ocu_grid3d *r, const ocu_grid3d *b_coarse, void *stream) {
+  if (int e = check_grid(r, "ocu_mg_restrict(r)")) return e;
+  if (int e = check_grid(b_coarse, "ocu_mg_restrict(b)")) return e;
+  OCU_REQUIRE(r->dtype == b_coarse->dtype && r->dtype != OCU_I32, "ocu_mg_restrict: dtype mismatch");
+  OCU_REQUIRE(r->nx >= 2 * b_coarse->nx && r->ny >= 2 * b_coarse->ny && r->nz >= 2 * b_coarse->nz,
+              "ocu_mg_restrict: fine grid must be twice the coarse grid");
+  if ((int64_t)b_coarse->nx * b_coarse->ny * b_coarse->nz == 0) return OCU_OK;
+  dim3 grid = stencil_grid(b_coarse->nx, b_coarse->ny, b_coarse->nz);
+  if (r->dtype == OCU_F64)
+    k_restrict<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(r), dgrid<double>(b_coarse));
+  else
+    k_restrict<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(r), dgrid<float>(b_coarse));
+  OCU_HIP_LAUNCHED("ocu_mg_restrict");
+}
+
+int ocu_mg_residual_restrict(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r,
+                             const ocu_grid3d *b_coarse, double h, double fx, double fy, double fz,
+                             double *norm_ws, void *stream) {
+  if (int e = check_pair(u, b, "ocu_mg_residual_restrict")) return e;
+  if (int e = check_grid(b_coarse, "ocu_mg_residual_restrict(b_coarse)")) return e;
+  OCU_REQUIRE(b_coarse->dtype == u->dtype, "ocu_mg_residual_restrict: dtype mismatch");
+  OCU_REQUIRE(u->nx == 2 * b_coarse->nx && u->ny == 2 * b_coarse->ny && u->nz == 2 * b_coarse->nz,
+              "ocu_mg_residual_restrict: fine grid must be exactly twice the coarse grid");
+  if (r) {
+    if (int e = check_grid(r, "ocu_mg_residual_restrict(r)")) return e;
+    OCU_REQUIRE(r->dtype == u->dtype && r->nx == u->nx && r->ny == u->ny && r->nz == u->nz,
+                "ocu_mg_residual_restrict: r mismatch");
+  }
+  const int cnx = b_coarse->nx, cny = b_coarse->ny, cnz = b_coarse->nz;
+  if ((int64_t)cnx * cny * cnz == 0) return OCU_OK;
+  int tiles_k = (cnz + TK - 1) / TK, tiles_j = (cny + TJ - 1) / TJ;
+  int ntiles = tiles_k * tiles_j * cnx;
+  dim3 block = stencil_block();
+  bool wr_r = r != nullptr;
+#define RR_LAUNCH(T, NORM, NB)                                                                          \
+  k_res_restrict<T, NORM><<<NB, block, 0, S(stream)>>>(dgrid<T>(u), dgrid<T>(b), wr_r ? dgrid<T>(r) : dgrid<T>(u), \
+                                                       wr_r, dgrid<T>(b_coarse), res_c<T>(h, fx, fy, fz), tiles_k,   \
+                                                       tiles_j, ntiles, norm_ws)
+  if (norm_ws) {
+    // always NORM_PARTS blocks: idle blocks write zero partials, so no
+    // memset is needed (keeps the call safe inside a captured graph)
+    if (u->dtype == OCU_F64) RR_LAUNCH(double, true, NORM_PARTS);
+    else RR_LAUNCH(float, true, NORM_PARTS);
+  } else {
+    if (u->dtype == OCU_F64) RR_LAUNCH(double, false, ntiles);
+    else RR_LAUNCH(float, false, ntiles);
+  }
+#undef RR_LAUNCH
+  OCU_HIP_LAUNCHED("ocu_mg_residual_restrict");
+}
+
+int ocu_mg_residual_norm(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r, double h, double fx,
+                         double fy, double fz, double *norm_ws, void *stream) {
+  if (!norm_ws) return ocu_mg_residual(u, b, r, h, fx, fy, fz, stream);
+  if (int e = check_pair(u, b, "ocu_mg_residual_norm")) return e;
+  if (int e = check_grid(r, "ocu_mg_residual_norm(r)")) return e;
+  OCU_REQUIRE(r->dtype == u->dtype && r->nx == u->nx && r->ny == u->ny && r->nz == u->nz,
+              "ocu_mg_residual_norm: r mismatch");
+  int tiles_k = (u->nz + TK - 1) / TK, tiles_j = (u->ny + TJ - 1) / TJ;
+  int ntiles = tiles_k * tiles_j * u->nx;
+  const int nb = NORM_PARTS;
+  if (u->dtype == OCU_F64)
+    k_res_norm<double><<<nb, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), dgrid<double>(r),
+                                                              res_c<double>(h, fx, fy, fz), tiles_k, tiles_j, ntiles, norm_ws);
+  else
+    k_res_norm<float><<<nb, stencil_block(), 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), dgrid<float>(r),
+                                                             res_c<float>(h, fx, fy, fz), tiles_k, tiles_j, ntiles, norm_ws);
+  OCU_HIP_LAUNCHED("ocu_mg_residual_norm");
+}
+
+int ocu_norm_finalize(const double *norm_ws, double *out, void *stream) {
+  OCU_REQUIRE(norm_ws && out, "ocu_norm_finalize: null pointer");
+  k_norm_finalize<<<1, 512, 0, S(stream)>>>(norm_ws, NORM_PARTS, out);
+  OCU_HIP_LAUNCHED("ocu_norm_finalize");
+}
+
+int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, void *stream) {
+  if (int e = check_grid(u_coarse, "ocu_mg_prolong(coarse)")) return e;
+  if (int e = check_grid(u_fine, "ocu_mg_prolong(fine)")) return e;
+  OCU_REQUIRE(u_coarse->dtype == u_fine->dtype && u_fine->dtype != OCU_I32, "ocu_mg_prolong: dtype mismatch");
+  OCU_REQUIRE(u_fine->nx >= 2 * u_coarse->nx && u_fine->ny >= 2 * u_coarse->ny && u_fine->nz >= 2 * u_coarse->nz,
+              "ocu_mg_prolong: fine grid must be twice the coarse grid");
+  OCU_REQUIRE(u_coarse->gx >= 1 && u_coarse->gy >= 1 && u_coarse->gz >= 1, "ocu_mg_prolong: coarse needs ghosts");
+  if ((int64_t)u_coarse->nx * u_coarse->ny * u_coarse->nz == 0) return OCU_OK;
+  dim3 grid = stencil_grid(u_coarse->nx, u_coarse->ny, u_coarse->nz);
+  if (u_fine->dtype == OCU_F64)
+    k_prolong<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u_coarse), dgrid<double>(u_fine), quirk);
+  else
+    k_prolong<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u_coarse), dgrid<float>(u_fine), quirk);
+  OCU_HIP_LAUNCHED("ocu_mg_prolong");
+}
+
+int ocu_mg_coarse_relax(const ocu_grid3d *u, const ocu_grid3d *b, int iterations, int order,
+                        const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx, double hy, double hz,
+                        int mode, void *stream) {
+  if (int e = check_pair(u, b, "ocu_mg_coarse_relax")) return e;
+  OCU_REQUIRE(p && bc, "ocu_mg_coarse_relax: null params");
+  OCU_REQUIRE(order >= 0 && order <= 2, "ocu_mg_coarse_relax: bad order %d", order);
+  OCU_REQUIRE(mode == 0 || (mode == 1 && u->dtype == OCU_F32), "ocu_mg_coarse_relax: bad mode %d", mode);
+  if (order == 2) {
+    if (iterations % 2 == 1) iterations++;
+    if (iterations < 4) iterations = 4;
+  }
+  CoarseGeom g;
+  g.nx = u->nx; g.ny = u->ny; g.nz = u->nz;
+  g.lys = g.nz + 2;
+  g.lxs = (g.ny + 2) * g.lys;
+  g.lshift = g.lxs + g.lys + 1;
+  size_t box = (size_t)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  size_t tsize = (mode == 1 || u->dtype == OCU_F64) ? 8 : 4;
+  size_t shmem = 2 * box * tsize;
+  OCU_REQUIRE(shmem <= 65536, "ocu_mg_coarse_relax: grid %dx%dx%d too large for one workgroup", g.nx, g.ny, g.nz);
+  FacesS F = faces_s(bc);
+  if (mode == 1) {
+    k_coarse<float, double, 1><<<1, 256, shmem, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), g, host_c<double>(p), F,
+                                                             hx, hy, hz, iterations, order);
+  } else if (u->dtype == OCU_F64) {
+    k_coarse<double, double, 0><<<1, 256, shmem, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), g, host_c<double>(p),
+                                                              F, hx, hy, hz, iterations, order);
+  } else {
+    k_coarse<float, float, 0><<<1, 256, shmem, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), g, host_c<float>(p), F,
+                                                            (float)hx, (float)hy, (float)hz, iterations, order);
+  }
+  OCU_HIP_LAUNCHED("ocu_mg_coarse_relax");
+}
+
+}  // extern "C"

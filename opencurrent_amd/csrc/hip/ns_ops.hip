@@ -1,0 +1,317 @@
+// MAC-grid operators around the pressure solve: divergence, gradient
+// subtraction, self-/passive advection, centred Laplacian, thermal force.
+// u, v, w, phi and their derivatives share one padded layout (the reference
+// requires congruent grids, sol_selfadvection3ddev.cu:622-626), so one
+// index serves every array.
+#include "ocu_hip_common.h"
+
+using namespace ocu_hip;
+
+namespace {
+
+template <typename T>
+struct Inv {
+  T x, y, z;
+};
+
+// interpolation.h:35-48
+template <typename T>
+__device__ __forceinline__ T interp(int scheme, T u, T x0, T x1) {
+  if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return ((T).5) * (x1 + x0);
+  return u < 0 ? x1 : x0;
+}
+
+// sol_divergence3ddev.cu:23-45
+template <typename T>
+__global__ __launch_bounds__(256) void k_divergence(const T *__restrict__ u, const T *__restrict__ v,
+                                                    const T *__restrict__ w, T *__restrict__ d, Inv<T> ih,
+                                                    int64_t xs, int64_t ys, int nx, int ny, int nz) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  d[id] = ih.x * (u[id + xs] - u[id]) + ih.y * (v[id + ys] - v[id]) + ih.z * (w[id + 1] - w[id]);
+}
+
+// sol_gradient3ddev.cu:24-47
+template <typename T>
+__global__ __launch_bounds__(256) void k_gradient(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+                                                  const T *__restrict__ p, T coef, Inv<T> ih, int64_t xs, int64_t ys,
+                                                  int nx, int ny, int nz) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  T pc = p[id];
+  u[id] += coef * ih.x * (pc - p[id - xs]);
+  v[id] += coef * ih.y * (pc - p[id - ys]);
+  w[id] += coef * ih.z * (pc - p[id - 1]);
+}
+
+// Advection3DF/D_apply_upwind_TEX (sol_selfadvection3ddev.cu:136-337)
+template <typename T>
+__global__ __launch_bounds__(256) void k_self_advect(const T *__restrict__ U, const T *__restrict__ V,
+                                                     const T *__restrict__ W, T *__restrict__ DU, T *__restrict__ DV,
+                                                     T *__restrict__ DW, Inv<T> ih, int64_t xs, int64_t ys, int nx,
+                                                     int ny, int nz, int scheme) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  const int64_t pi = id + xs, pj = id + ys, pk = id + 1, mi = id - xs, mj = id - ys, mk = id - 1;
+  const T half = (T).5;
+  T u0 = U[id], v0 = V[id], w0 = W[id];
+
+  T u_iph = half * (u0 + U[pi]);
+  T u_imh = half * (u0 + U[mi]);
+  T duu = (u_iph * interp(scheme, u_iph, u0, U[pi])) - (u_imh * interp(scheme, u_imh, U[mi], u0));
+  T v_jph = half * (V[pj] + V[pj - xs]);
+  T v_jmh = half * (v0 + V[mi]);
+  T dvu = (v_jph * interp(scheme, v_jph, u0, U[pj])) - (v_jmh * interp(scheme, v_jmh, U[mj], u0));
+  T w_kph = half * (W[pk] + W[pk - xs]);
+  T w_kmh = half * (w0 + W[mi]);
+  T dwu = (w_kph * interp(scheme, w_kph, u0, U[pk])) - (w_kmh * interp(scheme, w_kmh, U[mk], u0));
+  DU[id] = -duu * ih.x - dvu * ih.y - dwu * ih.z;
+
+  T u_atv_iph = half * (U[pi] + U[pi - ys]);
+  T u_atv_imh = half * (u0 + U[mj]);
+  T duv = (u_atv_iph * interp(scheme, u_atv_iph, v0, V[pi])) - (u_atv_imh * interp(scheme, u_atv_imh, V[mi], v0));
+  T v_iph = half * (v0 + V[pj]);
+  T v_imh = half * (v0 + V[mj]);
+  T dvv = (v_iph * interp(scheme, v_iph, v0, V[pj])) - (v_imh * interp(scheme, v_imh, V[mj], v0));
+  T w_atv_kph = half * (W[pk] + W[pk - ys]);
+  T w_atv_kmh = half * (w0 + W[mj]);
+  T dwv = (w_atv_kph * interp(scheme, w_atv_kph, v0, V[pk])) - (w_atv_kmh * interp(scheme, w_atv_kmh, V[mk], v0));
+  DV[id] = -duv * ih.x - dvv * ih.y - dwv * ih.z;
+
+  T u_atw_iph = half * (U[pi] + U[pi - 1]);
+  T u_atw_imh = half * (u0 + U[mk]);
+  T duw = (u_atw_iph * interp(scheme, u_atw_iph, w0, W[pi])) - (u_atw_imh * interp(scheme, u_atw_imh, W[mi], w0));
+  T v_atw_jph = half * (V[pj] + V[pj - 1]);
+  T v_atw_jmh = half * (v0 + V[mk]);
+  T dvw = (v_atw_jph * interp(scheme, v_atw_jph, w0, W[pj])) - (v_atw_jmh * interp(scheme, v_atw_jmh, W[mj], w0));
+  T w_iph = half * (w0 + W[pk]);
+  T w_imh = half * (w0 + W[mk]);
+  T dww = (w_iph * interp(scheme, w_iph, w0, W[pk])) - (w_imh * interp(scheme, w_imh, W[mk], w0));
+  DW[id] = -duw * ih.x - dvw * ih.y - dww * ih.z;
+}
+
+// Sol_PassiveAdvection3D_apply_interp (sol_passiveadvection3ddev.cu:31-78)
+template <typename T>
+__global__ __launch_bounds__(256) void k_passive_advect(const T *__restrict__ u, const T *__restrict__ v,
+                                                        const T *__restrict__ w, const T *__restrict__ phi,
+                                                        T *__restrict__ d, Inv<T> ih, int64_t xs, int64_t ys, int nx,
+                                                        int ny, int nz, int scheme) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  T p = phi[id];
+  T u_pi = u[id + xs], u0 = u[id];
+  T du = u_pi * interp(scheme, u_pi, p, phi[id + xs]) - u0 * interp(scheme, u0, phi[id - xs], p);
+  T v_pj = v[id + ys], v0 = v[id];
+  T dv = v_pj * interp(scheme, v_pj, p, phi[id + ys]) - v0 * interp(scheme, v0, phi[id - ys], p);
+  T w_pk = w[id + 1], w0 = w[id];
+  T dw = w_pk * interp(scheme, w_pk, p, phi[id + 1]) - w0 * interp(scheme, w0, phi[id - 1], p);
+  d[id] = (-du * ih.x) - (dv * ih.y) - (dw * ih.z);
+}
+
+// Sol_LaplacianCentered3DDevice_stencil (sol_laplaciancent3ddev.cu:27-62)
+template <typename T>
+__global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ phi, T *__restrict__ d, Inv<T> ih2, T coef,
+                                                   int64_t xs, int64_t ys, int nx, int ny, int nz) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  T p = phi[id];
+  T dp = d[id];
+  T lap = ih2.z * (phi[id + 1] + phi[id - 1] - ((T)2) * p) + ih2.y * (phi[id + ys] + phi[id - ys] - ((T)2) * p) +
+          ih2.x * (phi[id + xs] + phi[id - xs] - ((T)2) * p);
+  d[id] = dp + coef * lap;
+}
+
+// Eqn_IncompressibleNS3D_add_thermal_force (eqn_incompressns3d.cu:25-44)
+template <typename T>
+__global__ __launch_bounds__(256) void k_thermal(T *__restrict__ d, const T *__restrict__ t, T coef, int64_t xs,
+                                                 int64_t ys, int64_t nbr, int nx, int ny, int nz) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  d[id] += ((T).5) * coef * (t[id] + t[id - nbr]);
+}
+
+template <typename T>
+T *at0(const ocu_grid3d *g) {
+  return (T *)g->base + g->shift;
+}
+
+int check_mac(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, int nx, int ny, int nz,
+              const char *what) {
+  if (int e = check_grid(u, what)) return e;
+  if (int e = check_grid(v, what)) return e;
+  if (int e = check_grid(w, what)) return e;
+  // check_valid_mac_dimensions (ocustorage/grid3dops.cpp:23-45)
+  OCU_REQUIRE(u->nx == nx + 1 && u->ny == ny && u->nz == nz, "%s - u dimensions mismatch", what);
+  OCU_REQUIRE(v->nx == nx && v->ny == ny + 1 && v->nz == nz, "%s - v dimensions mismatch", what);
+  OCU_REQUIRE(w->nx == nx && w->ny == ny && w->nz == nz + 1, "%s - w dimensions mismatch", what);
+  OCU_REQUIRE(layout_match(u, v) && layout_match(u, w), "%s - u,v,w layout mismatch", what);
+  OCU_REQUIRE(u->dtype == v->dtype && u->dtype == w->dtype && u->dtype != OCU_I32, "%s - dtype mismatch", what);
+  OCU_REQUIRE(u->gx >= 1 && u->gy >= 1 && u->gz >= 1, "%s - must have at least one ghost cell", what);
+  return OCU_OK;
+}
+
+int check_same(const ocu_grid3d *a, const ocu_grid3d *b, const char *what) {
+  if (int e = check_grid(b, what)) return e;
+  OCU_REQUIRE(layout_match(a, b) && a->dtype == b->dtype, "%s - layout mismatch", what);
+  return OCU_OK;
+}
+
+template <typename T>
+Inv<T> inv(double hx, double hy, double hz) {
+  return Inv<T>{(T)(1 / hx), (T)(1 / hy), (T)(1 / hz)};
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocu_divergence(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *div, int nx,
+                   int ny, int nz, double hx, double hy, double hz, void *stream) {
+  const char *what = "Sol_Divergence3DDevice::solve";
+  if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
+  if (int e = check_same(u, div, what)) return e;
+  OCU_REQUIRE(div->nx == nx && div->ny == ny && div->nz == nz, "%s - invalid dimensions for divergence", what);
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
+  dim3 g = stencil_grid(nx, ny, nz);
+  if (u->dtype == OCU_F64)
+    k_divergence<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(u), at0<double>(v), at0<double>(w),
+                                                               at0<double>(div), inv<double>(hx, hy, hz), xs, ys, nx, ny, nz);
+  else
+    k_divergence<float><<<g, stencil_block(), 0, S(stream)>>>(at0<float>(u), at0<float>(v), at0<float>(w),
+                                                              at0<float>(div), inv<float>(hx, hy, hz), xs, ys, nx, ny, nz);
+  OCU_HIP_LAUNCHED("ocu_divergence");
+}
+
+int ocu_gradient_sub(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *p,
+                     double coefficient, double hx, double hy, double hz, void *stream) {
+  const char *what = "Sol_Gradient3DDevice::solve";
+  if (int e = check_grid(p, what)) return e;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
+  if (int e = check_same(u, p, what)) return e;
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
+  dim3 g = stencil_grid(nx, ny, nz);
+  if (u->dtype == OCU_F64)
+    k_gradient<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(u), at0<double>(v), at0<double>(w),
+                                                             at0<double>(p), coefficient, inv<double>(hx, hy, hz), xs,
+                                                             ys, nx, ny, nz);
+  else
+    k_gradient<float><<<g, stencil_block(), 0, S(stream)>>>(at0<float>(u), at0<float>(v), at0<float>(w), at0<float>(p),
+                                                            (float)coefficient, inv<float>(hx, hy, hz), xs, ys, nx, ny,
+                                                            nz);
+  OCU_HIP_LAUNCHED("ocu_gradient_sub");
+}
+
+int ocu_self_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *du,
+                    const ocu_grid3d *dv, const ocu_grid3d *dw, int nx, int ny, int nz, double hx, double hy,
+                    double hz, int scheme, void *stream) {
+  const char *what = "Sol_SelfAdvection3DDevice::solve";
+  if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
+  if (int e = check_same(u, du, what)) return e;
+  if (int e = check_same(u, dv, what)) return e;
+  if (int e = check_same(u, dw, what)) return e;
+  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED,
+              "%s - invalid interpolation type %d", what, scheme);
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
+  dim3 g = stencil_grid(nx, ny, nz);
+  if (u->dtype == OCU_F64)
+    k_self_advect<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(u), at0<double>(v), at0<double>(w),
+                                                                at0<double>(du), at0<double>(dv), at0<double>(dw),
+                                                                inv<double>(hx, hy, hz), xs, ys, nx, ny, nz, scheme);
+  else
+    k_self_advect<float><<<g, stencil_block(), 0, S(stream)>>>(at0<float>(u), at0<float>(v), at0<float>(w),
+                                                               at0<float>(du), at0<float>(dv), at0<float>(dw),
+                                                               inv<float>(hx, hy, hz), xs, ys, nx, ny, nz, scheme);
+  OCU_HIP_LAUNCHED("ocu_self_advect");
+}
+
+int ocu_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *phi,
+                       const ocu_grid3d *dphi, int nx, int ny, int nz, double hx, double hy, double hz, int scheme,
+                       void *stream) {
+  const char *what = "Sol_PassiveAdvection3DDevice::solve";
+  if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
+  if (int e = check_same(u, phi, what)) return e;
+  if (int e = check_same(u, dphi, what)) return e;
+  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED,
+              "%s - invalid interpolation type %d", what, scheme);
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
+  dim3 g = stencil_grid(nx, ny, nz);
+  if (u->dtype == OCU_F64)
+    k_passive_advect<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(u), at0<double>(v), at0<double>(w),
+                                                                   at0<double>(phi), at0<double>(dphi),
+                                                                   inv<double>(hx, hy, hz), xs, ys, nx, ny, nz, scheme);
+  else
+    k_passive_advect<float><<<g, stencil_block(), 0, S(stream)>>>(at0<float>(u), at0<float>(v), at0<float>(w),
+                                                                  at0<float>(phi), at0<float>(dphi),
+                                                                  inv<float>(hx, hy, hz), xs, ys, nx, ny, nz, scheme);
+  OCU_HIP_LAUNCHED("ocu_passive_advect");
+}
+
+int ocu_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz, double hx, double hy,
+                      double hz, double coefficient, void *stream) {
+  const char *what = "Sol_LaplacianCentered3DDevice::solve";
+  if (int e = check_grid(phi, what)) return e;
+  if (int e = check_same(phi, dphi, what)) return e;
+  OCU_REQUIRE(phi->nx == nx && phi->ny == ny && phi->nz == nz, "%s - phi dimensions mismatch", what);
+  OCU_REQUIRE(phi->gx >= 1 && phi->gy >= 1 && phi->gz >= 1, "%s - must have >= 1 ghost cell", what);
+  OCU_REQUIRE(phi->dtype != OCU_I32, "%s - int grid", what);
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  int64_t xs = (int64_t)phi->pny * phi->pnz, ys = phi->pnz;
+  dim3 g = stencil_grid(nx, ny, nz);
+  if (phi->dtype == OCU_F64)
+    k_laplacian<double><<<g, stencil_block(), 0, S(stream)>>>(
+        at0<double>(phi), at0<double>(dphi),
+        Inv<double>{(double)(1 / (hx * hx)), (double)(1 / (hy * hy)), (double)(1 / (hz * hz))}, coefficient, xs, ys,
+        nx, ny, nz);
+  else
+    k_laplacian<float><<<g, stencil_block(), 0, S(stream)>>>(
+        at0<float>(phi), at0<float>(dphi),
+        Inv<float>{(float)(1 / (hx * hx)), (float)(1 / (hy * hy)), (float)(1 / (hz * hz))}, (float)coefficient, xs,
+        ys, nx, ny, nz);
+  OCU_HIP_LAUNCHED("ocu_laplacian_acc");
+}
+
+int ocu_thermal_force(const ocu_grid3d *d, const ocu_grid3d *t, int nx, int ny, int nz, int vd, double coefficient,
+                      void *stream) {
+  const char *what = "Eqn_IncompressibleNS3D::add_thermal_force";
+  if (int e = check_grid(d, what)) return e;
+  if (int e = check_same(d, t, what)) return e;
+  OCU_REQUIRE(vd == OCU_DIR_XPOS || vd == OCU_DIR_XNEG || vd == OCU_DIR_YPOS || vd == OCU_DIR_YNEG ||
+                  vd == OCU_DIR_ZPOS || vd == OCU_DIR_ZNEG,
+              "%s - bad vertical_direction %d", what, vd);
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  int64_t xs = (int64_t)t->pny * t->pnz, ys = t->pnz;
+  int64_t nbr = (vd & OCU_DIR_XAXIS_FLAG) ? xs : (vd & OCU_DIR_YAXIS_FLAG) ? ys : 1;
+  dim3 g = stencil_grid(nx, ny, nz);
+  if (d->dtype == OCU_F64)
+    k_thermal<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(d), at0<double>(t), coefficient, xs, ys, nbr,
+                                                            nx, ny, nz);
+  else
+    k_thermal<float><<<g, stencil_block(), 0, S(stream)>>>(at0<float>(d), at0<float>(t), (float)coefficient, xs, ys,
+                                                           nbr, nx, ny, nz);
+  OCU_HIP_LAUNCHED("ocu_thermal_force");
+}
+
+}  // extern "C"

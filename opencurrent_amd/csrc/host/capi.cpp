@@ -1,0 +1,382 @@
+// Handle-based C ABI (include/ocu.h) over the drop-in classes.
+#include "ocu.h"
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "ocu_hip.h"
+#include "ocuequation/eqn_incompressns3d.h"
+#include "ocuequation/sol_mgpressure3d.h"
+#include "ocuequation/sol_mgpressuremixed3d.h"
+#include "ocuequation/sol_project3d.h"
+#include "ocuutil/thread.h"
+
+using namespace ocu;
+
+namespace {
+
+BoundaryCondition from_c1(const ocu_bc &b) {
+  BoundaryCondition r;
+  r.type = (BoundaryConditionType)b.type;
+  r.value = b.value;
+  r.aux_value = b.aux_value;
+  return r;
+}
+
+BoundaryConditionSet from_c(const ocu_bc_set &b) {
+  return BoundaryConditionSet(from_c1(b.xpos), from_c1(b.xneg), from_c1(b.ypos), from_c1(b.yneg), from_c1(b.zpos),
+                              from_c1(b.zneg));
+}
+
+void apply_cfg(Sol_MultigridPressure3DBase &s, const ocu_mg_config *cfg) {
+  s.bc = from_c(cfg->bc);
+  s.nu1 = cfg->nu1;
+  s.nu2 = cfg->nu2;
+  s.convergence = (ConvergenceType)cfg->convergence;
+  s.make_symmetric_operator = cfg->make_symmetric_operator != 0;
+}
+
+void copy_stats(const std::vector<double> &hist, double *history, int cap, int *len) {
+  if (len) *len = (int)hist.size();
+  if (history)
+    for (int n = 0; n < (int)hist.size() && n < cap; n++) history[n] = hist[n];
+}
+
+struct MGHandle {
+  int dtype;
+  Grid3DDevice<float> rhs_f;
+  Grid3DDevice<double> rhs_d;
+  Sol_MultigridPressure3DDevice<float> *f = nullptr;
+  Sol_MultigridPressure3DDevice<double> *d = nullptr;
+  ~MGHandle() {
+    delete f;
+    delete d;
+  }
+  Sol_MultigridPressure3DBase &base() { return d ? (Sol_MultigridPressure3DBase &)*d : *f; }
+};
+
+struct MixedHandle {
+  Grid3DDevice<double> rhs;
+  Sol_MultigridPressureMixed3DDeviceD s;
+};
+
+struct ProjHandle {
+  int dtype, mixed;
+  Grid3DDevice<float> uf, vf, wf;
+  Grid3DDevice<double> ud, vd, wd;
+  Sol_ProjectDivergence3DDevice<float> *pf = nullptr;
+  Sol_ProjectDivergence3DDevice<double> *pd = nullptr;
+  Sol_ProjectDivergenceMixed3DDeviceD *pm = nullptr;
+  ~ProjHandle() {
+    delete pf;
+    delete pd;
+    delete pm;
+  }
+};
+
+struct NSHandle {
+  int dtype;
+  Eqn_IncompressibleNS3D<float> *f = nullptr;
+  Eqn_IncompressibleNS3D<double> *d = nullptr;
+  ~NSHandle() {
+    delete f;
+    delete d;
+  }
+};
+
+template <typename T>
+bool fill_params(Eqn_IncompressibleNS3DParams<T> &P, const ocu_ns_params *p, const void *iu, const void *iv,
+                 const void *iw, const void *it) {
+  P.init_grids(p->nx, p->ny, p->nz, true);
+  P.hx = p->hx; P.hy = p->hy; P.hz = p->hz;
+  P.max_divergence = p->max_divergence;
+  P.temp_bc = from_c(p->temp_bc);
+  P.flow_bc = from_c(p->flow_bc);
+  P.vertical_direction = (DirectionType)p->vertical_direction;
+  P.gravity = (T)p->gravity;
+  P.bouyancy = (T)p->bouyancy;
+  P.thermal_diffusion = (T)p->thermal_diffusion;
+  P.viscosity = (T)p->viscosity;
+  P.advection_scheme = (InterpolationType)p->advection_scheme;
+  P.time_step = (TimeStepType)p->time_step;
+  P.cfl_factor = p->cfl_factor;
+  size_t n = (size_t)P.init_u.num_allocated_elements() * sizeof(T);
+  memcpy(P.init_u.buffer(), iu, n);
+  memcpy(P.init_v.buffer(), iv, n);
+  memcpy(P.init_w.buffer(), iw, n);
+  memcpy(P.init_temp.buffer(), it, n);
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocu_set_compute_stream(void *stream) {
+  ThreadManager::set_compute_stream(stream);
+  return OCU_OK;
+}
+
+void *ocu_get_compute_stream(void) { return ThreadManager::get_compute_stream(); }
+
+// ---------------------------------------------------------------- multigrid
+
+void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
+                    int use_fused_relax) {
+  if (!rhs || !cfg || (rhs->dtype != OCU_F32 && rhs->dtype != OCU_F64)) return nullptr;
+  MGHandle *h = new (std::nothrow) MGHandle();
+  if (!h) return nullptr;
+  h->dtype = rhs->dtype;
+  bool ok;
+  if (rhs->dtype == OCU_F64) {
+    h->d = new Sol_MultigridPressure3DDevice<double>();
+    apply_cfg(*h->d, cfg);
+    h->d->disable_relax_on_host = cfg->disable_relax_on_host != 0;
+    h->d->use_fused_relax = use_fused_relax != 0;
+    ok = h->rhs_d.init_external(*rhs) &&
+         h->d->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_d);
+  } else {
+    h->f = new Sol_MultigridPressure3DDevice<float>();
+    apply_cfg(*h->f, cfg);
+    h->f->disable_relax_on_host = cfg->disable_relax_on_host != 0;
+    h->f->use_fused_relax = use_fused_relax != 0;
+    ok = h->rhs_f.init_external(*rhs) &&
+         h->f->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_f);
+  }
+  if (!ok) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int ocu_mg_pressure(void *hh, ocu_grid3d *out) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h || !out) return OCU_ERR_INVALID;
+  *out = h->d ? h->d->pressure().desc() : h->f->pressure().desc();
+  return OCU_OK;
+}
+
+int ocu_mg_solve(void *hh, double tol, int max_iter, double *residual) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  double r = 0;
+  bool ok = h->base().solve(r, tol, max_iter);
+  if (residual) *residual = r;
+  return ok ? 1 : 0;
+}
+
+int ocu_mg_run_vcycles(void *hh, int count) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return h->base().run_vcycles(count) ? 1 : 0;
+}
+
+int ocu_mg_run_fmg(void *hh, int count) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return h->base().run_fmg(count) ? 1 : 0;
+}
+
+int ocu_mg_stats(void *hh, int *num_levels, int *vcycles, double *history, int cap, int *len) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  Sol_MultigridPressure3DBase &b = h->base();
+  if (num_levels) *num_levels = b.num_levels();
+  if (vcycles) *vcycles = b.num_vcycles;
+  copy_stats(b.residual_history, history, cap, len);
+  return OCU_OK;
+}
+
+void ocu_mg_destroy(void *hh) { delete (MGHandle *)hh; }
+
+// ------------------------------------------------------------------ mixed IR
+
+void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
+                         int cycles, int cycles_first, int fmg_first) {
+  if (!rhs || !cfg || rhs->dtype != OCU_F64) return nullptr;
+  MixedHandle *h = new (std::nothrow) MixedHandle();
+  if (!h) return nullptr;
+  apply_cfg(h->s, cfg);
+  h->s.ir_cycles = cycles;
+  h->s.ir_cycles_first = cycles_first;
+  h->s.ir_fmg_first = fmg_first != 0;
+  if (!h->rhs.init_external(*rhs) || !h->s.initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs)) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int ocu_mgmixed_pressure(void *hh, ocu_grid3d *out) {
+  MixedHandle *h = (MixedHandle *)hh;
+  if (!h || !out) return OCU_ERR_INVALID;
+  *out = h->s.pressure().desc();
+  return OCU_OK;
+}
+
+int ocu_mgmixed_solve(void *hh, double tol, int max_iter, double *residual) {
+  MixedHandle *h = (MixedHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  double r = 0;
+  bool ok = h->s.solve(r, tol, max_iter);
+  if (residual) *residual = r;
+  return ok ? 1 : 0;
+}
+
+int ocu_mgmixed_stats(void *hh, int *ir_iterations, int *vcycles, double *rhs_rms, double *history, int cap,
+                      int *len) {
+  MixedHandle *h = (MixedHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  if (ir_iterations) *ir_iterations = h->s.ir_iterations;
+  if (vcycles) *vcycles = h->s.total_vcycles;
+  if (rhs_rms) *rhs_rms = h->s.rhs_l2;
+  copy_stats(h->s.residual_history, history, cap, len);
+  return OCU_OK;
+}
+
+void ocu_mgmixed_destroy(void *hh) { delete (MixedHandle *)hh; }
+
+// ---------------------------------------------------------------- projection
+
+void *ocu_project_create(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc,
+                         int nx, int ny, int nz, double hx, double hy, double hz, int mixed) {
+  if (!u || !v || !w || !bc) return nullptr;
+  ProjHandle *h = new (std::nothrow) ProjHandle();
+  if (!h) return nullptr;
+  h->dtype = u->dtype;
+  h->mixed = mixed;
+  bool ok = false;
+  if (u->dtype == OCU_F64) {
+    ok = h->ud.init_external(*u) && h->vd.init_external(*v) && h->wd.init_external(*w);
+    if (ok && mixed) {
+      h->pm = new Sol_ProjectDivergenceMixed3DDeviceD();
+      h->pm->bc = from_c(*bc);
+      ok = h->pm->initialize_storage(nx, ny, nz, hx, hy, hz, &h->ud, &h->vd, &h->wd);
+    } else if (ok) {
+      h->pd = new Sol_ProjectDivergence3DDevice<double>();
+      h->pd->bc = from_c(*bc);
+      ok = h->pd->initialize_storage(nx, ny, nz, hx, hy, hz, &h->ud, &h->vd, &h->wd);
+    }
+  } else if (u->dtype == OCU_F32 && !mixed) {
+    ok = h->uf.init_external(*u) && h->vf.init_external(*v) && h->wf.init_external(*w);
+    if (ok) {
+      h->pf = new Sol_ProjectDivergence3DDevice<float>();
+      h->pf->bc = from_c(*bc);
+      ok = h->pf->initialize_storage(nx, ny, nz, hx, hy, hz, &h->uf, &h->vf, &h->wf);
+    }
+  }
+  if (!ok) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int ocu_project_solve(void *hh, double tol, double *residual) {
+  ProjHandle *h = (ProjHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  bool ok;
+  double r;
+  if (h->pm) { ok = h->pm->solve(tol); r = h->pm->last_residual; }
+  else if (h->pd) { ok = h->pd->solve(tol); r = h->pd->last_residual; }
+  else { ok = h->pf->solve(tol); r = h->pf->last_residual; }
+  if (residual) *residual = r;
+  return ok ? 1 : 0;
+}
+
+int ocu_project_solve_divergence_only(void *hh) {
+  ProjHandle *h = (ProjHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  bool ok = h->pm ? h->pm->solve_divergence_only() : h->pd ? h->pd->solve_divergence_only()
+                                                           : h->pf->solve_divergence_only();
+  return ok ? 1 : 0;
+}
+
+int ocu_project_divergence(void *hh, ocu_grid3d *out) {
+  ProjHandle *h = (ProjHandle *)hh;
+  if (!h || !out) return OCU_ERR_INVALID;
+  *out = h->pm ? h->pm->divergence.desc() : h->pd ? h->pd->divergence.desc() : h->pf->divergence.desc();
+  return OCU_OK;
+}
+
+int ocu_project_pressure(void *hh, ocu_grid3d *out) {
+  ProjHandle *h = (ProjHandle *)hh;
+  if (!h || !out) return OCU_ERR_INVALID;
+  *out = h->pm ? h->pm->pressure_solver.pressure().desc()
+         : h->pd ? h->pd->pressure_solver.pressure().desc()
+                 : h->pf->pressure_solver.pressure().desc();
+  return OCU_OK;
+}
+
+void ocu_project_destroy(void *hh) { delete (ProjHandle *)hh; }
+
+// ------------------------------------------------------------------- NS eqn
+
+void *ocu_ns_create(int dtype, const ocu_ns_params *p, const void *iu, const void *iv, const void *iw,
+                    const void *it) {
+  if (!p || !iu || !iv || !iw || !it) return nullptr;
+  NSHandle *h = new (std::nothrow) NSHandle();
+  if (!h) return nullptr;
+  h->dtype = dtype;
+  bool ok = false;
+  if (dtype == OCU_F64) {
+    Eqn_IncompressibleNS3DParams<double> P;
+    fill_params(P, p, iu, iv, iw, it);
+    h->d = new Eqn_IncompressibleNS3D<double>();
+    ok = h->d->set_parameters(P);
+  } else if (dtype == OCU_F32) {
+    Eqn_IncompressibleNS3DParams<float> P;
+    fill_params(P, p, iu, iv, iw, it);
+    h->f = new Eqn_IncompressibleNS3D<float>();
+    ok = h->f->set_parameters(P);
+  }
+  if (!ok) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int ocu_ns_advance_one_step(void *hh, double dt) {
+  NSHandle *h = (NSHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return (h->d ? h->d->advance_one_step(dt) : h->f->advance_one_step(dt)) ? 1 : 0;
+}
+
+int ocu_ns_advance(void *hh, double dt) {
+  NSHandle *h = (NSHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return (h->d ? h->d->advance(dt) : h->f->advance(dt)) ? 1 : 0;
+}
+
+int ocu_ns_max_stable_timestep(void *hh, double *dt) {
+  NSHandle *h = (NSHandle *)hh;
+  if (!h || !dt) return OCU_ERR_INVALID;
+  *dt = h->d ? h->d->get_max_stable_timestep() : h->f->get_max_stable_timestep();
+  return OCU_OK;
+}
+
+int ocu_ns_field(void *hh, int which, ocu_grid3d *out) {
+  NSHandle *h = (NSHandle *)hh;
+  if (!h || !out || which < 0 || which > 4) return OCU_ERR_INVALID;
+  if (h->d) {
+    *out = which == 0 ? h->d->get_u().desc() : which == 1 ? h->d->get_v().desc() : which == 2 ? h->d->get_w().desc()
+           : which == 3 ? h->d->get_temperature().desc() : h->d->get_pressure().desc();
+  } else {
+    *out = which == 0 ? h->f->get_u().desc() : which == 1 ? h->f->get_v().desc() : which == 2 ? h->f->get_w().desc()
+           : which == 3 ? h->f->get_temperature().desc() : h->f->get_pressure().desc();
+  }
+  return OCU_OK;
+}
+
+int ocu_ns_num_steps(void *hh) {
+  NSHandle *h = (NSHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return h->d ? h->d->num_steps : h->f->num_steps;
+}
+
+void ocu_ns_destroy(void *hh) { delete (NSHandle *)hh; }
+
+}  // extern "C"

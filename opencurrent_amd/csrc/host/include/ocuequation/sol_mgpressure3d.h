@@ -1,0 +1,152 @@
+// Multigrid pressure solver, the drop-in for ocuequation/sol_mgpressure3d.h.
+//
+// Sol_MultigridPressure3DBase keeps the reference's FMG / V-cycle driver
+// (sol_mgpressure3d.cpp:113-398) and public fields; Sol_MultigridPressure3DDevice<T>
+// implements the per-level operations with the kernels of libocu_hip.so:
+//  - relaxation: fused red+black sweeps (one launch per iteration, out of
+//    place into a second buffer) when no face is periodic, otherwise one
+//    launch per colour with the ghost refresh between colours;
+//  - restrict_residuals: residual and restriction fused in one launch, the
+//    level-0 norms reduced on the device (deterministic, in double) and read
+//    back through pinned memory - the only synchronisation per V-cycle;
+//  - the coarsest level: relaxed by one LDS-resident workgroup on the GPU
+//    with the same arithmetic as the reference's host relaxation
+//    (sol_mgpressure3ddev.cu:82-188), without the two PCIe copies.
+#ifndef __OCU_EQUATION_MULTIGRID_PRESSURE_3D_DEV_H__
+#define __OCU_EQUATION_MULTIGRID_PRESSURE_3D_DEV_H__
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ocuequation/solver.h"
+#include "ocustorage/grid3d.h"
+#include "ocuutil/boundary_condition.h"
+#include "ocuutil/convergence.h"
+
+namespace ocu {
+
+struct int3_ {
+  int x, y, z;
+};
+
+class Sol_MultigridPressure3DBase : public Solver {
+ protected:
+  enum RelaxOrder { RO_RED_BLACK, RO_BLACK_RED, RO_SYMMETRIC };
+
+  int _num_levels;
+  double *_h;
+  double _omega;
+  double _fx, _fy, _fz;
+  int3_ *_dim;
+
+  int nx(int level) const { return _dim[level].x; }
+  int ny(int level) const { return _dim[level].y; }
+  int nz(int level) const { return _dim[level].z; }
+  double get_h(int level) const { return _h[level]; }
+  double hx(int level) const { return get_h(level) / sqrt(_fx); }
+  double hy(int level) const { return get_h(level) / sqrt(_fy); }
+  double hz(int level) const { return get_h(level) / sqrt(_fz); }
+
+  int num_levels(int nx_val, int ny_val, int nz_val);
+  double optimal_omega(double fx, double fy, double fz);
+  double bc_diag_mod(const BoundaryCondition &bc, double factor) const;
+
+  virtual bool do_fmg(double tolerance, int max_iter, double &result_l2, double &result_linf);
+  virtual bool do_vcycle(double tolerance, int max_iter, double &result_l2, double &result_linf);
+  bool initialize_base_storage(int nx_val, int ny_val, int nz_val, double hx_val, double hy_val, double hz_val);
+
+  virtual void clear_zero(int level) = 0;
+  virtual void apply_boundary_conditions(int level) = 0;
+  virtual void relax(int level, int iterations, RelaxOrder order) = 0;
+  virtual void restrict_residuals(int fine_level, int coarse_level, double *l2, double *linf) = 0;
+  virtual void prolong(int coarse_level, int fine_level) = 0;
+
+ public:
+  BoundaryConditionSet bc;
+  int nu1;
+  int nu2;
+  ConvergenceType convergence;
+  bool make_symmetric_operator;
+  // print the per-cycle residual like the reference (sol_mgpressure3d.cpp:201-202)
+  bool verbose;
+
+  // statistics of the last solve / run
+  int num_vcycles;
+  std::vector<double> residual_history;
+
+  bool solve(double &residual, double tolerance = 1e-6, int max_iter = 4);
+  bool run_vcycles(int count);
+  bool run_fmg(int count);
+
+  int num_levels() const { return _num_levels; }
+  double omega() const { return _omega; }
+
+  Sol_MultigridPressure3DBase();
+  virtual ~Sol_MultigridPressure3DBase();
+};
+
+class Sol_MultigridPressureMixed3DDeviceD;
+
+template <typename T>
+class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
+ protected:
+  friend class Sol_MultigridPressureMixed3DDeviceD;
+
+  Grid3DDevice<T> **_r_grid;
+  Grid3DDevice<T> **_u_grid;
+  Grid3DDevice<T> **_b_grid;
+  Grid3DDevice<T> **_u_alt;  // second buffer per level for out-of-place sweeps
+  Grid3DDevice<T> _pressure;
+  bool _update_bc_between_colors;
+  bool _fused_ok;  // every face Neumann or Dirichlet
+  // 0: the coarsest level is relaxed with the reference's host arithmetic in T;
+  // 1: in double on float grids (the coarsest solve of the mixed solver)
+  int _coarse_mode;
+  double *_norm_ws;    // device: NORM_PARTS (sum, max) pairs + 2 results
+  double *_norm_host;  // pinned host: 2 results
+
+  Grid3DDevice<T> &get_u(int level) { return *_u_grid[level]; }
+  Grid3DDevice<T> &get_r(int level) { return *_r_grid[level]; }
+  Grid3DDevice<T> &get_b(int level) { return *_b_grid[level]; }
+
+  virtual BoundaryConditionSet get_bc_at_level(int level) const;
+  void level_params(int level, void *params_out) const;
+  void free_storage();
+
+  virtual void apply_boundary_conditions(int level);
+  virtual void relax(int level, int iterations, RelaxOrder order);
+  virtual void restrict_residuals(int fine_level, int coarse_level, double *l2, double *linf);
+  virtual void prolong(int coarse_level, int fine_level);
+  virtual void clear_zero(int level);
+
+  void relax_colour(int level, int color);
+  void relax_coarsest(int level, int iterations, RelaxOrder order);
+
+ public:
+  bool disable_relax_on_host;
+  // one launch per red+black iteration when no face is periodic
+  bool use_fused_relax;
+  // also store the residual grids in restrict_residuals (read_r())
+  bool store_residuals;
+
+  Sol_MultigridPressure3DDevice();
+  ~Sol_MultigridPressure3DDevice();
+
+  virtual bool initialize_storage(int nx_val, int ny_val, int nz_val, double hx_val, double hy_val, double hz_val,
+                                  Grid3DDevice<T> *rhs);
+
+  Grid3DDevice<T> &pressure() { return _pressure; }
+  const Grid3DDevice<T> &pressure() const { return _pressure; }
+
+  const Grid3DDevice<T> &read_u(int level) const { return *_u_grid[level]; }
+  const Grid3DDevice<T> &read_r(int level) const { return *_r_grid[level]; }
+  const Grid3DDevice<T> &read_b(int level) const { return *_b_grid[level]; }
+};
+
+typedef Sol_MultigridPressure3DDevice<float> Sol_MultigridPressure3DDeviceF;
+typedef Sol_MultigridPressure3DDevice<double> Sol_MultigridPressure3DDeviceD;
+
+}  // namespace ocu
+
+#endif

@@ -1,0 +1,37 @@
+// ocuequation/sol_selfadvection3d.h: flux-form self-advection of a MAC
+// velocity (sol_selfadvection3ddev.cu)
+#ifndef __OCU_EQUATION_SELFADVECTION_3D_H__
+#define __OCU_EQUATION_SELFADVECTION_3D_H__
+
+#include "ocuequation/solver.h"
+#include "ocustorage/grid3d.h"
+#include "ocuutil/interpolation.h"
+
+namespace ocu {
+
+template <typename T>
+class Sol_SelfAdvection3DDevice : public Solver {
+  int _nx, _ny, _nz;
+  double _hx, _hy, _hz;
+
+ public:
+  Sol_SelfAdvection3DDevice();
+  ~Sol_SelfAdvection3DDevice() {}
+
+  Grid3DDevice<T> *u, *v, *w;
+  Grid3DDevice<T> *deriv_udt, *deriv_vdt, *deriv_wdt;
+  InterpolationType interp_type;
+
+  bool solve();
+  bool initialize_storage(int nx, int ny, int nz, double hx, double hy, double hz, Grid3DDevice<T> *u_val,
+                          Grid3DDevice<T> *v_val, Grid3DDevice<T> *w_val, Grid3DDevice<T> *deriv_udt_val,
+                          Grid3DDevice<T> *deriv_vdt_val, Grid3DDevice<T> *deriv_wdt_val);
+
+  int nx() const { return _nx; }
+  int ny() const { return _ny; }
+  int nz() const { return _nz; }
+};
+
+}  // namespace ocu
+
+#endif

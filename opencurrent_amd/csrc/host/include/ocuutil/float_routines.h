@@ -1,0 +1,23 @@
+// ocuutil/float_routines.h: check_float, min3, max3
+#ifndef __OCU_UTIL_FLOAT_ROUTINES_H__
+#define __OCU_UTIL_FLOAT_ROUTINES_H__
+
+#include <cmath>
+
+namespace ocu {
+
+inline bool check_float(double f) { return std::isfinite(f); }
+
+template <typename T>
+T min3(T a, T b, T c) {
+  return a < b ? (a < c ? a : c) : (b < c ? b : c);
+}
+
+template <typename T>
+T max3(T a, T b, T c) {
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+}  // namespace ocu
+
+#endif

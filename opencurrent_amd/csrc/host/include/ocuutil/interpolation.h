@@ -1,0 +1,16 @@
+// ocuutil/interpolation.h:26-31 (the functors themselves live in the kernels)
+#ifndef __OCU_UTIL_INTERPOLATION_H__
+#define __OCU_UTIL_INTERPOLATION_H__
+
+namespace ocu {
+
+enum InterpolationType {
+  IT_ERROR = 0,
+  IT_FIRST_ORDER_UPWIND,
+  IT_SECOND_ORDER_CENTERED,
+  IT_THIRD_ORDER_VAN_LEER_FLUX_LIMITED
+};
+
+}  // namespace ocu
+
+#endif

@@ -1,0 +1,366 @@
+/*
+ * ORACLE - test infrastructure only (see oracle_grid.h).
+ *
+ * CPU restatement of the MAC-grid operators around the pressure solve:
+ * divergence, gradient subtraction, self- and passive advection, the
+ * centred Laplacian, the Boussinesq thermal force, the projection solver and
+ * the Eqn_IncompressibleNS3D / Eqn_ScalarAdvection3D time steps.
+ */
+#ifndef OCU_ORACLE_NS_H
+#define OCU_ORACLE_NS_H
+
+#include "oracle_mg.h"
+
+namespace orc {
+
+/* interpolation.h:35-48 */
+template <typename T>
+inline T interp(int scheme, T u, T x0, T x1) {
+  if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return ((T).5) * (x1 + x0);
+  return u < 0 ? x1 : x0;
+}
+
+/* Sol_Divergence3DDevice_calculate_divergence, sol_divergence3ddev.cu:23-45;
+ * 1/h rounded to T by the launcher (:80-84). */
+template <typename T>
+void divergence(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
+                const ocu_grid3d &dg, int nx, int ny, int nz, double hx, double hy, double hz) {
+  View<T> u(ug), v(vg), w(wg), d(dg);
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++)
+        d.at(i, j, k) = ix * (u.at(i + 1, j, k) - u.at(i, j, k)) +
+                        iy * (v.at(i, j + 1, k) - v.at(i, j, k)) +
+                        iz * (w.at(i, j, k + 1) - w.at(i, j, k));
+}
+
+/* Sol_Gradient3DDevice_subtract_grad, sol_gradient3ddev.cu:24-47: over the
+ * cells of phi, u += c*(1/hx)*(p - p_{i-1}) etc. */
+template <typename T>
+void gradient_sub(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
+                  const ocu_grid3d &pg, T coefficient, double hx, double hy, double hz) {
+  View<T> u(ug), v(vg), w(wg), p(pg);
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < pg.nx; i++)
+    for (int j = 0; j < pg.ny; j++)
+      for (int k = 0; k < pg.nz; k++) {
+        T pc = p.at(i, j, k);
+        u.at(i, j, k) += coefficient * ix * (pc - p.at(i - 1, j, k));
+        v.at(i, j, k) += coefficient * iy * (pc - p.at(i, j - 1, k));
+        w.at(i, j, k) += coefficient * iz * (pc - p.at(i, j, k - 1));
+      }
+}
+
+/* Self-advection of a MAC velocity in flux form,
+ * Advection3DF/D_apply_upwind_TEX (sol_selfadvection3ddev.cu:136-337; the
+ * naive kernel :29-114 has the same arithmetic).  Writes du,dv,dw on
+ * i<nx, j<ny, k<nz only. */
+template <typename T>
+void self_advect(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
+                 const ocu_grid3d &dug, const ocu_grid3d &dvg, const ocu_grid3d &dwg,
+                 int nx, int ny, int nz, double hx, double hy, double hz, int scheme) {
+  View<T> U(ug), V(vg), W(wg), DU(dug), DV(dvg), DW(dwg);
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+  const T half = (T).5;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        auto u = [&](int a, int b, int c) { return U.at(i + a, j + b, k + c); };
+        auto v = [&](int a, int b, int c) { return V.at(i + a, j + b, k + c); };
+        auto w = [&](int a, int b, int c) { return W.at(i + a, j + b, k + c); };
+        T u0 = u(0, 0, 0), v0 = v(0, 0, 0), w0 = w(0, 0, 0);
+        // dudt
+        T u_iph = half * (u0 + u(1, 0, 0));
+        T u_imh = half * (u0 + u(-1, 0, 0));
+        T duu = (u_iph * interp<T>(scheme, u_iph, u0, u(1, 0, 0))) -
+                (u_imh * interp<T>(scheme, u_imh, u(-1, 0, 0), u0));
+        T v_jph = half * (v(0, 1, 0) + v(-1, 1, 0));
+        T v_jmh = half * (v0 + v(-1, 0, 0));
+        T dvu = (v_jph * interp<T>(scheme, v_jph, u0, u(0, 1, 0))) -
+                (v_jmh * interp<T>(scheme, v_jmh, u(0, -1, 0), u0));
+        T w_kph = half * (w(0, 0, 1) + w(-1, 0, 1));
+        T w_kmh = half * (w0 + w(-1, 0, 0));
+        T dwu = (w_kph * interp<T>(scheme, w_kph, u0, u(0, 0, 1))) -
+                (w_kmh * interp<T>(scheme, w_kmh, u(0, 0, -1), u0));
+        DU.at(i, j, k) = -duu * ix - dvu * iy - dwu * iz;
+        // dvdt
+        T u_atv_iph = half * (u(1, 0, 0) + u(1, -1, 0));
+        T u_atv_imh = half * (u0 + u(0, -1, 0));
+        T duv = (u_atv_iph * interp<T>(scheme, u_atv_iph, v0, v(1, 0, 0))) -
+                (u_atv_imh * interp<T>(scheme, u_atv_imh, v(-1, 0, 0), v0));
+        T v_iph = half * (v0 + v(0, 1, 0));
+        T v_imh = half * (v0 + v(0, -1, 0));
+        T dvv = (v_iph * interp<T>(scheme, v_iph, v0, v(0, 1, 0))) -
+                (v_imh * interp<T>(scheme, v_imh, v(0, -1, 0), v0));
+        T w_atv_kph = half * (w(0, 0, 1) + w(0, -1, 1));
+        T w_atv_kmh = half * (w0 + w(0, -1, 0));
+        T dwv = (w_atv_kph * interp<T>(scheme, w_atv_kph, v0, v(0, 0, 1))) -
+                (w_atv_kmh * interp<T>(scheme, w_atv_kmh, v(0, 0, -1), v0));
+        DV.at(i, j, k) = -duv * ix - dvv * iy - dwv * iz;
+        // dwdt
+        T u_atw_iph = half * (u(1, 0, 0) + u(1, 0, -1));
+        T u_atw_imh = half * (u0 + u(0, 0, -1));
+        T duw = (u_atw_iph * interp<T>(scheme, u_atw_iph, w0, w(1, 0, 0))) -
+                (u_atw_imh * interp<T>(scheme, u_atw_imh, w(-1, 0, 0), w0));
+        T v_atw_jph = half * (v(0, 1, 0) + v(0, 1, -1));
+        T v_atw_jmh = half * (v0 + v(0, 0, -1));
+        T dvw = (v_atw_jph * interp<T>(scheme, v_atw_jph, w0, w(0, 1, 0))) -
+                (v_atw_jmh * interp<T>(scheme, v_atw_jmh, w(0, -1, 0), w0));
+        T w_iph = half * (w0 + w(0, 0, 1));
+        T w_imh = half * (w0 + w(0, 0, -1));
+        T dww = (w_iph * interp<T>(scheme, w_iph, w0, w(0, 0, 1))) -
+                (w_imh * interp<T>(scheme, w_imh, w(0, 0, -1), w0));
+        DW.at(i, j, k) = -duw * ix - dvw * iy - dww * iz;
+      }
+}
+
+/* Sol_PassiveAdvection3D_apply_interp, sol_passiveadvection3ddev.cu:31-78 */
+template <typename T>
+void passive_advect(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
+                    const ocu_grid3d &phig, const ocu_grid3d &dg, int nx, int ny, int nz,
+                    double hx, double hy, double hz, int scheme) {
+  View<T> u(ug), v(vg), w(wg), phi(phig), d(dg);
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        T p = phi.at(i, j, k);
+        T u_pi = u.at(i + 1, j, k), u0 = u.at(i, j, k);
+        T du = u_pi * interp<T>(scheme, u_pi, p, phi.at(i + 1, j, k)) -
+               u0 * interp<T>(scheme, u0, phi.at(i - 1, j, k), p);
+        T v_pj = v.at(i, j + 1, k), v0 = v.at(i, j, k);
+        T dv = v_pj * interp<T>(scheme, v_pj, p, phi.at(i, j + 1, k)) -
+               v0 * interp<T>(scheme, v0, phi.at(i, j - 1, k), p);
+        T w_pk = w.at(i, j, k + 1), w0 = w.at(i, j, k);
+        T dw = w_pk * interp<T>(scheme, w_pk, p, phi.at(i, j, k + 1)) -
+               w0 * interp<T>(scheme, w0, phi.at(i, j, k - 1), p);
+        d.at(i, j, k) = (-du * ix) - (dv * iy) - (dw * iz);
+      }
+}
+
+/* Sol_LaplacianCentered3DDevice_stencil, sol_laplaciancent3ddev.cu:27-62:
+ * dphi += c * (z, y, x second differences), 1/h^2 rounded to T (:100-102). */
+template <typename T>
+void laplacian_acc(const ocu_grid3d &pg, const ocu_grid3d &dg, int nx, int ny, int nz,
+                   double hx, double hy, double hz, T coefficient) {
+  View<T> phi(pg), d(dg);
+  const T ix2 = (T)(1 / (hx * hx)), iy2 = (T)(1 / (hy * hy)), iz2 = (T)(1 / (hz * hz));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        T p = phi.at(i, j, k);
+        T dp = d.at(i, j, k);
+        T lap = iz2 * (phi.at(i, j, k + 1) + phi.at(i, j, k - 1) - ((T)2) * p) +
+                iy2 * (phi.at(i, j + 1, k) + phi.at(i, j - 1, k) - ((T)2) * p) +
+                ix2 * (phi.at(i + 1, j, k) + phi.at(i - 1, j, k) - ((T)2) * p);
+        d.at(i, j, k) = dp + coefficient * lap;
+      }
+}
+
+/* Eqn_IncompressibleNS3D_add_thermal_force, eqn_incompressns3d.cu:25-44,
+ * launched at :119-155: d += (T).5 * c * (t + t_{-vertical}). */
+template <typename T>
+void thermal_force(const ocu_grid3d &dg, const ocu_grid3d &tg, int nx, int ny, int nz,
+                   int64_t nbr_stride, T coefficient) {
+  View<T> d(dg), t(tg);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        T *tp = &t.at(i, j, k);
+        d.at(i, j, k) += ((T).5) * coefficient * (tp[0] + tp[-nbr_stride]);
+      }
+}
+
+/* convert_bc_to_poisson_eqn, sol_project3d.cpp:26-43 */
+inline ocu_bc poisson_bc(const ocu_bc &b) {
+  ocu_bc r{OCU_BC_INVALID, 0, 0};
+  if (b.type == OCU_BC_PERIODIC) r.type = OCU_BC_PERIODIC;
+  else if (b.type == OCU_BC_FORCED_INFLOW_VARIABLE_SLIP) { r.type = OCU_BC_NEUMANN; r.value = 0; }
+  return r;
+}
+
+/* Sol_ProjectDivergence3DDevice<T> (sol_project3ddev.cu:28-135): MAC BC ->
+ * divergence -> MG solve (LINF, 15 iterations) -> gradient (c = -1) -> MAC
+ * BC.  pressure and divergence are congruent with u. */
+template <typename T>
+struct Projection {
+  ocu_bc_set bc;
+  int nx = 0, ny = 0, nz = 0;
+  double hx = 1, hy = 1, hz = 1;
+  ocu_grid3d u{}, v{}, w{};
+  HostGrid<T> divg, pressure;
+  MGSolver<T> mg;
+  double last_residual = 0;
+
+  bool initialize_storage(int nx_, int ny_, int nz_, double hx_, double hy_, double hz_,
+                          const ocu_grid3d &u_, const ocu_grid3d &v_, const ocu_grid3d &w_) {
+    nx = nx_; ny = ny_; nz = nz_; hx = hx_; hy = hy_; hz = hz_;
+    u = u_; v = v_; w = w_;
+    divg.init(nx, ny, nz, 1, 1, 1, u.pnx - (nx + 2), u.pny - (ny + 2), u.pnz - (nz + 2));
+    if (!layout_match(divg.desc, u)) return false;
+    pressure.init_congruent(divg.desc);
+    mg.bc.xneg = poisson_bc(bc.xneg); mg.bc.xpos = poisson_bc(bc.xpos);
+    mg.bc.yneg = poisson_bc(bc.yneg); mg.bc.ypos = poisson_bc(bc.ypos);
+    mg.bc.zneg = poisson_bc(bc.zneg); mg.bc.zpos = poisson_bc(bc.zpos);
+    if (!mg.initialize_storage(nx, ny, nz, hx, hy, hz, divg.desc, pressure.desc)) return false;
+    mg.convergence = OCU_CONVERGENCE_LINF;
+    return true;
+  }
+
+  bool solve(double tolerance) {
+    bool ok = bc_fill_mac_level1<T>(u, v, w, bc);
+    divergence<T>(u, v, w, divg.desc, nx, ny, nz, hx, hy, hz);
+    double res = 0;
+    ok = mg.solve(res, tolerance, 15) && ok;
+    last_residual = res;
+    gradient_sub<T>(u, v, w, pressure.desc, (T)-1, hx, hy, hz);
+    ok = bc_fill_mac_level1<T>(u, v, w, bc) && ok;
+    return ok;
+  }
+};
+
+struct NSParams {
+  int nx, ny, nz;
+  double hx, hy, hz;
+  double max_divergence;
+  ocu_bc_set temp_bc, flow_bc;
+  int vertical_direction;
+  double gravity, bouyancy;
+  double thermal_diffusion, viscosity;
+  int advection_scheme, time_step;  // time_step: 1 FE, 2 AB2
+  double cfl_factor;
+};
+
+/* Eqn_IncompressibleNS3D<T> (eqn_incompressns3d.cu:119-399) and
+ * Equation::advance (equation.cpp:31-56). */
+template <typename T>
+struct NS {
+  NSParams p;
+  HostGrid<T> u, v, w, du, dv, dw, ldu, ldv, ldw, temp, dtemp, ldtemp;
+  Projection<T> proj;
+  double lastdt = 0;
+  int num_steps = 0;
+  T gravity, bouyancy, viscosity, thermal_diffusion;
+
+  /* set_parameters (:164-299); init_* are host grids laid out like
+   * Eqn_IncompressibleNS3DParams::init_grids (eqn_incompressns3d.h:78-86). */
+  bool set_parameters(const NSParams &prm, const ocu_grid3d &iu, const ocu_grid3d &iv,
+                      const ocu_grid3d &iw, const ocu_grid3d &it) {
+    p = prm;
+    gravity = (T)p.gravity; bouyancy = (T)p.bouyancy;
+    viscosity = (T)p.viscosity; thermal_diffusion = (T)p.thermal_diffusion;
+    auto cp = [](HostGrid<T> &dst, const ocu_grid3d &src) {
+      dst.init_congruent(src);
+      std::memcpy(dst.buf.data(), src.base, sizeof(T) * (size_t)src.allocated);
+    };
+    cp(u, iu); du.init_congruent(iu); ldu.init_congruent(iu);
+    cp(v, iv); dv.init_congruent(iv); ldv.init_congruent(iv);
+    cp(w, iw); dw.init_congruent(iw); ldw.init_congruent(iw);
+    cp(temp, it); dtemp.init_congruent(it); ldtemp.init_congruent(it);
+    proj.bc = p.flow_bc;
+    if (!proj.initialize_storage(p.nx, p.ny, p.nz, p.hx, p.hy, p.hz, u.desc, v.desc, w.desc)) return false;
+    if (!bc_fill_mac_level1<T>(u.desc, v.desc, w.desc, p.flow_bc)) return false;
+    if (!bc_fill_level1<T>(temp.desc, p.temp_bc, p.hx, p.hy, p.hz)) return false;
+    return true;
+  }
+
+  void add_thermal_force() {
+    int vd = p.vertical_direction;
+    T direction_mult = (vd & OCU_DIR_NEGATIVE_FLAG) ? 1 : -1;
+    const ocu_grid3d &tgt = (vd & OCU_DIR_XAXIS_FLAG) ? du.desc : (vd & OCU_DIR_YAXIS_FLAG) ? dv.desc : dw.desc;
+    int64_t stride = (vd & OCU_DIR_XAXIS_FLAG) ? (int64_t)temp.desc.pny * temp.desc.pnz
+                   : (vd & OCU_DIR_YAXIS_FLAG) ? temp.desc.pnz : 1;
+    thermal_force<T>(tgt, temp.desc, p.nx, p.ny, p.nz, stride, direction_mult * gravity * bouyancy);
+  }
+
+  /* get_max_stable_timestep (:301-328) */
+  double get_max_stable_timestep() const {
+    T mu = (T)reduce<T>(u.desc, 0), mv = (T)reduce<T>(v.desc, 0), mw = (T)reduce<T>(w.desc, 0);
+    double ut = p.hx / mu, vt = p.hy / mv, wt = p.hz / mw;
+    if (!std::isfinite(ut)) ut = 1e10;
+    if (!std::isfinite(vt)) vt = 1e10;
+    if (!std::isfinite(wt)) wt = 1e10;
+    double step = p.cfl_factor * min3(ut, vt, wt);
+    double minh = min3(p.hx, p.hy, p.hz);
+    if (thermal_diffusion > 0) step = std::min(step, (minh * minh) / (6 * thermal_diffusion));
+    if (viscosity > 0) step = std::min(step, (minh * minh) / (6 * viscosity));
+    return step;
+  }
+
+  /* advance_one_step (:331-399) */
+  bool advance_one_step(double dt) {
+    bool ok = true;
+    num_steps++;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    self_advect<T>(u.desc, v.desc, w.desc, du.desc, dv.desc, dw.desc, nx, ny, nz, p.hx, p.hy, p.hz, p.advection_scheme);
+    if (viscosity > 0) {
+      laplacian_acc<T>(u.desc, du.desc, nx + 1, ny, nz, p.hx, p.hy, p.hz, viscosity);
+      laplacian_acc<T>(v.desc, dv.desc, nx, ny + 1, nz, p.hx, p.hy, p.hz, viscosity);
+      laplacian_acc<T>(w.desc, dw.desc, nx, ny, nz + 1, p.hx, p.hy, p.hz, viscosity);
+    }
+    add_thermal_force();
+    passive_advect<T>(u.desc, v.desc, w.desc, temp.desc, dtemp.desc, nx, ny, nz, p.hx, p.hy, p.hz, p.advection_scheme);
+    if (thermal_diffusion > 0)
+      laplacian_acc<T>(temp.desc, dtemp.desc, nx, ny, nz, p.hx, p.hy, p.hz, thermal_diffusion);
+    T ab_coeff = -dt * dt / (2 * lastdt);
+    bool ab2 = p.time_step == 2 && lastdt > 0;
+    auto adv = [&](HostGrid<T> &x, HostGrid<T> &d, HostGrid<T> &ld) {
+      if (ab2) {
+        ok = lincomb2<T>(x.desc, (T)1.0, x.desc, (T)(dt - ab_coeff), d.desc) && ok;
+        ok = lincomb2<T>(x.desc, (T)1.0, x.desc, (T)ab_coeff, ld.desc) && ok;
+      } else {
+        ok = lincomb2<T>(x.desc, (T)1.0, x.desc, (T)dt, d.desc) && ok;
+      }
+    };
+    adv(temp, dtemp, ldtemp);
+    ok = bc_fill_nocorners_device<T>(temp.desc, p.temp_bc, p.hx, p.hy, p.hz) && ok;
+    adv(u, du, ldu);
+    adv(v, dv, ldv);
+    adv(w, dw, ldw);
+    if (p.time_step == 2) {
+      lastdt = dt;
+      ldtemp.buf = dtemp.buf; ldtemp.desc.base = ldtemp.buf.data();
+      ldu.buf = du.buf; ldu.desc.base = ldu.buf.data();
+      ldv.buf = dv.buf; ldv.desc.base = ldv.buf.data();
+      ldw.buf = dw.buf; ldw.desc.base = ldw.buf.data();
+    }
+    ok = proj.solve(p.max_divergence) && ok;
+    return ok;
+  }
+
+  bool advance(double dt) {
+    if (!std::isfinite(dt)) return false;
+    double dtleft = dt;
+    while (dtleft > 0) {
+      int subds = (int)std::ceil(dtleft / get_max_stable_timestep());
+      double subdt = dtleft / subds;
+      if (!advance_one_step(subdt)) return false;
+      dtleft -= subdt;
+    }
+    return true;
+  }
+};
+
+/* Eqn_ScalarAdvection3D<T>::advance_one_step (eqn_scalaradvection3d.cpp:
+ * 88-105): phi = phi + dt * dphi (forward Euler) then the device
+ * no-corner ghost fill. */
+template <typename T>
+bool scalar_advect_step(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
+                        const ocu_grid3d &phig, const ocu_grid3d &dphig, const ocu_bc_set &bc,
+                        int nx, int ny, int nz, double hx, double hy, double hz, int scheme, double dt) {
+  passive_advect<T>(ug, vg, wg, phig, dphig, nx, ny, nz, hx, hy, hz, scheme);
+  bool ok = lincomb2<T>(phig, (T)1, phig, (T)dt, dphig);
+  ok = bc_fill_nocorners_device<T>(phig, bc, hx, hy, hz) && ok;
+  return ok;
+}
+
+}  // namespace orc
+
+#endif

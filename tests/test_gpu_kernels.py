@@ -1,0 +1,365 @@
+"""Kernel-level parity: every HIP entry point against the CPU oracle on the
+same seeded inputs.  The bar is bit-exact: both sides perform the same IEEE
+operations in the same order (the library and the oracle are compiled
+without FMA contraction), so whole buffers - interior, ghosts and padding -
+must agree bit for bit."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from opencurrent_amd import _types as T
+from opencurrent_amd._types import BC, BCSet, HostGrid
+from fields import uniform, mac_grids
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def D():
+    from opencurrent_amd import device
+    return device
+
+
+def assert_bitexact(got, want, what=""):
+    a, b = got.buf, want.buf
+    assert a.dtype == b.dtype and a.shape == b.shape, what
+    ia = a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+    ib = b.view(np.uint64 if b.dtype == np.float64 else np.uint32)
+    bad = np.nonzero(ia != ib)[0]
+    if bad.size:
+        n = bad[0]
+        raise AssertionError(f"{what}: {bad.size} elements differ, first at {n}: got {a[n]!r} want {b[n]!r}")
+
+
+def random_grid(nx, ny, nz, dtype, seed, g=1, pad=(0, 0, 0)):
+    h = HostGrid(nx, ny, nz, g, g, g, *pad, dtype=dtype)
+    h.buf[:] = (2 * uniform(seed, h.buf.shape) - 1).astype(dtype)
+    return h
+
+
+def bcset(xp, xn, yp, yn, zp, zn):
+    s = BCSet()
+    for f, spec in zip(BCSet.FACES, (xp, xn, yp, yn, zp, zn)):
+        typ, val = spec if isinstance(spec, tuple) else (spec, 0.0)
+        setattr(s, f, BC(typ, val, 0.0))
+    return s
+
+
+BC_CASES = {
+    "neumann0": bcset(*[T.BC_NEUMANN] * 6),
+    "neumann_v": bcset((T.BC_NEUMANN, .37), (T.BC_NEUMANN, -.25), (T.BC_NEUMANN, 1.5), (T.BC_NEUMANN, .1),
+                       (T.BC_NEUMANN, -2.0), (T.BC_NEUMANN, .321)),
+    "dirichlet": bcset(*[(T.BC_DIRICHELET, .5)] * 6),
+    "mixed": bcset(T.BC_PERIODIC, T.BC_PERIODIC, (T.BC_DIRICHELET, .321), (T.BC_DIRICHELET, -1.0),
+                   (T.BC_NEUMANN, .7), (T.BC_NEUMANN, 0.0)),
+    "periodic": bcset(*[T.BC_PERIODIC] * 6),
+    "none_faces": bcset(T.BC_NONE, (T.BC_DIRICHELET, 1.0), T.BC_NONE, (T.BC_NEUMANN, .2), T.BC_PERIODIC,
+                        T.BC_PERIODIC),
+}
+SIZES = [(16, 12, 20), (7, 9, 5), (33, 8, 64)]
+DTYPES = [np.float64, np.float32]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", SIZES)
+@pytest.mark.parametrize("case", list(BC_CASES))
+def test_bc_fill_level1(gpu, dtype, size, case):
+    h = random_grid(*size, dtype, 11)
+    bc = BC_CASES[case]
+    want = h.copy()
+    assert O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), .3, .25, .125) == 1
+    g = D().DeviceGrid.from_host(h)
+    D().bc_fill_level1(g, bc, .3, .25, .125)
+    assert_bitexact(g.to_host(), want, f"level1 {case} {size}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", list(BC_CASES))
+def test_bc_fill_nocorners(gpu, dtype, case):
+    h = random_grid(13, 10, 17, dtype, 12)
+    bc = BC_CASES[case]
+    want = h.copy()
+    assert O.lib().orc_bc_fill(1, want.ptr(), C.byref(bc), .5, .125, .25) == 1
+    g = D().DeviceGrid.from_host(h)
+    D().bc_fill_nocorners(g, bc, .5, .125, .25)
+    assert_bitexact(g.to_host(), want, f"nocorners {case}")
+
+
+MAC_CASES = {
+    "noslip": bcset(*[(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0)] * 6),
+    "rb": None,  # periodic x/y, no-slip z (aux 1)
+    "inflow": None,
+    "periodic": bcset(*[T.BC_PERIODIC] * 6),
+}
+
+
+def mac_bc(case):
+    if case == "rb":
+        s = bcset(T.BC_PERIODIC, T.BC_PERIODIC, T.BC_PERIODIC, T.BC_PERIODIC, T.BC_FORCED_INFLOW_VARIABLE_SLIP,
+                  T.BC_FORCED_INFLOW_VARIABLE_SLIP)
+        s.zpos.aux_value = 1.0
+        s.zneg.aux_value = 1.0
+        return s
+    if case == "inflow":
+        s = bcset((T.BC_FORCED_INFLOW_VARIABLE_SLIP, .75), (T.BC_FORCED_INFLOW_VARIABLE_SLIP, .75),
+                  T.BC_PERIODIC, T.BC_PERIODIC, (T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0),
+                  (T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0))
+        s.xpos.aux_value = .3
+        s.zneg.aux_value = 1.0
+        return s
+    return MAC_CASES[case]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", list(MAC_CASES))
+def test_bc_fill_mac(gpu, dtype, case):
+    u, v, w, _ = mac_grids(12, 9, 14, dtype)
+    for q, g in enumerate((u, v, w)):
+        g.buf[:] = (2 * uniform(20 + q, g.buf.shape) - 1).astype(dtype)
+    bc = mac_bc(case)
+    wu, wv, ww = u.copy(), v.copy(), w.copy()
+    assert O.lib().orc_bc_fill_mac(wu.ptr(), wv.ptr(), ww.ptr(), C.byref(bc)) == 1
+    du, dv, dw = (D().DeviceGrid.from_host(x) for x in (u, v, w))
+    D().bc_fill_mac(du, dv, dw, bc)
+    for got, want, n in ((du, wu, "u"), (dv, wv, "v"), (dw, ww, "w")):
+        assert_bitexact(got.to_host(), want, f"mac {case} {n}")
+
+
+def filled(nx, ny, nz, dtype, seed, bc, h):
+    g = random_grid(nx, ny, nz, dtype, seed)
+    O.lib().orc_bc_fill(0, g.ptr(), C.byref(bc), h, h, h)
+    return g
+
+
+RELAX_BCS = ["neumann0", "dirichlet", "mixed", "neumann_v"]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("variant", [T.RELAX_DEVICE, T.RELAX_HOST])
+@pytest.mark.parametrize("size", [(16, 16, 16), (12, 10, 7), (32, 8, 24)])
+@pytest.mark.parametrize("case", RELAX_BCS)
+def test_relax_colour(gpu, dtype, variant, size, case):
+    bc = BC_CASES[case]
+    u = filled(*size, dtype, 31, bc, 1 / 16)
+    b = random_grid(*size, dtype, 32)
+    for level in (0, 1):
+        p = O.level_params(bc, level, 1 / 16, 1 / 12, 1 / 20)
+        for color in (0, 1):
+            want = u.copy()
+            O.lib().orc_mg_relax(want.ptr(), b.ptr(), color, C.byref(p), variant)
+            gu, gb = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(b)
+            D().mg_relax(gu, gb, color, p, variant)
+            assert_bitexact(gu.to_host(), want, f"relax {case} l{level} c{color}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", [(16, 16, 16), (40, 30, 70), (9, 20, 13), (64, 64, 64)])
+@pytest.mark.parametrize("case", ["neumann0", "dirichlet", "neumann_v"])
+@pytest.mark.parametrize("first", [0, 1])
+def test_relax_fused_matches_two_colours(gpu, dtype, size, case, first):
+    bc = BC_CASES[case]
+    u = filled(*size, dtype, 41, bc, 1 / 32)
+    b = random_grid(*size, dtype, 42)
+    p = O.level_params(bc, 0, 1 / 32, 1 / 32, 1 / 16)
+    want = u.copy()
+    O.lib().orc_mg_relax(want.ptr(), b.ptr(), first, C.byref(p), T.RELAX_DEVICE)
+    O.lib().orc_mg_relax(want.ptr(), b.ptr(), 1 - first, C.byref(p), T.RELAX_DEVICE)
+    gin, gb = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(b)
+    gout = D().DeviceGrid.from_host(u)  # ghosts of the output are not written: start from the same ghosts
+    D().mg_relax_fused(gin, gout, gb, p, bc, first)
+    assert_bitexact(gout.to_host(), want, f"fused {case} {size}")
+
+
+def test_relax_fused_rejects_periodic(gpu):
+    from opencurrent_amd._lib import OcuError
+    bc = BC_CASES["mixed"]
+    u = D().DeviceGrid(8, 8, 8)
+    o = D().DeviceGrid(8, 8, 8)
+    b = D().DeviceGrid(8, 8, 8)
+    p = O.level_params(bc, 0, .1, .1, .1)
+    with pytest.raises(OcuError):
+        D().mg_relax_fused(u, o, b, p, bc, 0)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", [(16, 16, 16), (32, 8, 24), (64, 64, 64)])
+def test_residual_restrict(gpu, dtype, size):
+    bc = BC_CASES["mixed"]
+    h = 1 / 32
+    u = filled(*size, dtype, 51, bc, h)
+    b = random_grid(*size, dtype, 52)
+    r_want = HostGrid.congruent(u)
+    O.lib().orc_mg_residual(u.ptr(), b.ptr(), r_want.ptr(), h, 1.0, .64, .25)
+    cn = tuple(s // 2 for s in size)
+    bc_want = HostGrid(*cn, dtype=dtype)
+    O.lib().orc_mg_restrict(r_want.ptr(), bc_want.ptr())
+    gu, gb = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(b)
+    # residual alone
+    gr = D().DeviceGrid.congruent(gu)
+    D().mg_residual(gu, gb, gr, h, 1.0, .64, .25)
+    assert_bitexact(gr.to_host(), r_want, "residual")
+    # restriction alone
+    gbc = D().DeviceGrid(*cn, dtype=dtype)
+    D().mg_restrict(D().DeviceGrid.from_host(r_want), gbc)
+    assert_bitexact(gbc.to_host(), bc_want, "restrict")
+    # fused, with and without r, with norms
+    gr2 = D().DeviceGrid.congruent(gu)
+    gbc2 = D().DeviceGrid(*cn, dtype=dtype)
+    s2, mx = D().mg_residual_restrict(gu, gb, gr2, gbc2, h, 1.0, .64, .25, with_norm=True)
+    assert_bitexact(gbc2.to_host(), bc_want, "fused restrict")
+    assert_bitexact(gr2.to_host(), r_want, "fused residual")
+    ri = r_want.interior().astype(np.float64)
+    assert mx == np.abs(ri).max()
+    assert abs(s2 - (ri * ri).sum()) <= 1e-12 * (ri * ri).sum()
+    gbc3 = D().DeviceGrid(*cn, dtype=dtype)
+    D().mg_residual_restrict(gu, gb, None, gbc3, h, 1.0, .64, .25)
+    assert_bitexact(gbc3.to_host(), bc_want, "fused restrict (no r)")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_prolong(gpu, dtype, quirk):
+    cn = (8, 6, 10)
+    bc = BC_CASES["mixed"]
+    uc = filled(*cn, dtype, 61, bc, .1)
+    uf = random_grid(*(2 * s for s in cn), dtype, 62)
+    want = uf.copy()
+    O.lib().orc_mg_prolong(uc.ptr(), want.ptr(), quirk)
+    guc, guf = D().DeviceGrid.from_host(uc), D().DeviceGrid.from_host(uf)
+    D().mg_prolong(guc, guf, quirk)
+    assert_bitexact(guf.to_host(), want, f"prolong quirk={quirk}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", ["neumann0", "dirichlet", "mixed", "neumann_v"])
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_coarse_relax_reference_semantics(gpu, dtype, case, order):
+    """relax_on_host (sol_mgpressure3ddev.cu:134-188): host arithmetic, host
+    no-corner fill after each colour, level-1 fill at the end."""
+    bc = BC_CASES[case]
+    cn = (4, 6, 4)
+    h = .5
+    u = filled(*cn, dtype, 71, bc, h)
+    b = random_grid(*cn, dtype, 72)
+    p = O.level_params(bc, 0, h, h, h)
+    iters = 8
+    want = u.copy()
+    it_n = iters
+    if order == 2:
+        it_n = max(4, iters + (iters % 2))
+    for it in range(it_n):
+        for rb in (0, 1):
+            color = rb
+            if order == 1 or (order == 2 and it >= it_n // 2):
+                color = 1 - rb
+            O.lib().orc_mg_relax(want.ptr(), b.ptr(), color, C.byref(p), T.RELAX_HOST)
+            O.lib().orc_bc_fill(2, want.ptr(), C.byref(bc), h, h, h)
+    O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), h, h, h)
+    gu, gb = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(b)
+    D().mg_coarse_relax(gu, gb, iters, order, p, bc, h, h, h, mode=0)
+    assert_bitexact(gu.to_host(), want, f"coarse {case} order {order}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_grid_ops(gpu, dtype):
+    # 64^3 with padding (66,66,80): > 3% waste -> interior-only update
+    a = random_grid(64, 64, 64, dtype, 81)
+    b = random_grid(64, 64, 64, dtype, 82)
+    r = random_grid(64, 64, 64, dtype, 83)
+    # a grid whose waste is under 3% (256*256*272 allocated for 254*254*270)
+    # -> whole-allocation update
+    a2 = random_grid(254, 254, 270, dtype, 84)
+    b2 = random_grid(254, 254, 270, dtype, 85)
+    r2 = random_grid(254, 254, 270, dtype, 86)
+    for (ra, aa, bb) in ((r, a, b), (r2, a2, b2)):
+        want = ra.copy()
+        O.lib().orc_lincomb2(want.ptr(), 1.0, aa.ptr(), .125, bb.ptr())
+        gr, ga, gb = (D().DeviceGrid.from_host(x) for x in (ra, aa, bb))
+        D().grid_lincomb(gr, 1.0, ga, .125, gb)
+        assert_bitexact(gr.to_host(), want, "lincomb2")
+        want1 = ra.copy()
+        O.lib().orc_lincomb1(want1.ptr(), -.75, aa.ptr())
+        D().grid_lincomb(gr, -.75, ga)
+        assert_bitexact(gr.to_host(), want1, "lincomb1")
+    ga = D().DeviceGrid.from_host(a)
+    ai = a.interior().astype(np.float64)
+    assert D().grid_reduce(ga, 0) == np.abs(ai).max()
+    assert D().grid_reduce(ga, 1) == ai.max()
+    assert D().grid_reduce(ga, 2) == ai.min()
+    assert abs(D().grid_reduce(ga, 3) - ai.sum()) <= 1e-12 * np.abs(ai).sum()
+    assert abs(D().grid_reduce(ga, 4) - (ai * ai).sum()) <= 1e-12 * (ai * ai).sum()
+    # conversion copy
+    other = np.float32 if dtype == np.float64 else np.float64
+    dst = HostGrid.congruent(a, dtype=other)
+    want = dst.copy()
+    O.lib().orc_copy_convert(want.ptr(), a.ptr())
+    gd = D().DeviceGrid.from_host(dst)
+    D().grid_copy(gd, ga)
+    assert_bitexact(gd.to_host(), want, "copy_convert")
+
+
+NS_SIZES = [(16, 12, 20), (9, 7, 5)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", NS_SIZES)
+def test_divergence_gradient(gpu, dtype, size):
+    nx, ny, nz = size
+    u, v, w, t = mac_grids(nx, ny, nz, dtype)
+    for q, g in enumerate((u, v, w, t)):
+        g.buf[:] = (2 * uniform(90 + q, g.buf.shape) - 1).astype(dtype)
+    hx, hy, hz = .5, .25, .125
+    want = t.copy()
+    O.lib().orc_divergence(u.ptr(), v.ptr(), w.ptr(), want.ptr(), nx, ny, nz, hx, hy, hz)
+    du, dv, dw, dt = (D().DeviceGrid.from_host(x) for x in (u, v, w, t))
+    D().divergence(du, dv, dw, dt, nx, ny, nz, hx, hy, hz)
+    assert_bitexact(dt.to_host(), want, "divergence")
+    wu, wv, ww = u.copy(), v.copy(), w.copy()
+    O.lib().orc_gradient_sub(wu.ptr(), wv.ptr(), ww.ptr(), t.ptr(), -1.0, hx, hy, hz)
+    dp = D().DeviceGrid.from_host(t)
+    D().gradient_sub(du, dv, dw, dp, -1.0, hx, hy, hz)
+    for got, want_, n in ((du, wu, "u"), (dv, wv, "v"), (dw, ww, "w")):
+        assert_bitexact(got.to_host(), want_, f"gradient {n}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("scheme", [T.IT_FIRST_ORDER_UPWIND, T.IT_SECOND_ORDER_CENTERED])
+@pytest.mark.parametrize("size", NS_SIZES)
+def test_advection(gpu, dtype, scheme, size):
+    nx, ny, nz = size
+    u, v, w, t = mac_grids(nx, ny, nz, dtype)
+    for q, g in enumerate((u, v, w, t)):
+        g.buf[:] = (2 * uniform(100 + q, g.buf.shape) - 1).astype(dtype)
+    hx, hy, hz = .5, .25, .125
+    d = [HostGrid.congruent(t) for _ in range(4)]
+    O.lib().orc_self_advect(u.ptr(), v.ptr(), w.ptr(), d[0].ptr(), d[1].ptr(), d[2].ptr(), nx, ny, nz, hx, hy, hz,
+                            scheme)
+    O.lib().orc_passive_advect(u.ptr(), v.ptr(), w.ptr(), t.ptr(), d[3].ptr(), nx, ny, nz, hx, hy, hz, scheme)
+    du, dv, dw, dt = (D().DeviceGrid.from_host(x) for x in (u, v, w, t))
+    g = [D().DeviceGrid.congruent(dt) for _ in range(4)]
+    D().self_advect(du, dv, dw, g[0], g[1], g[2], nx, ny, nz, hx, hy, hz, scheme)
+    D().passive_advect(du, dv, dw, dt, g[3], nx, ny, nz, hx, hy, hz, scheme)
+    for q, n in enumerate(("du", "dv", "dw", "dphi")):
+        assert_bitexact(g[q].to_host(), d[q], f"advection {n}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_laplacian_thermal(gpu, dtype):
+    nx, ny, nz = 16, 12, 20
+    u, v, w, t = mac_grids(nx, ny, nz, dtype)
+    for q, g in enumerate((u, t)):
+        g.buf[:] = (2 * uniform(110 + q, g.buf.shape) - 1).astype(dtype)
+    d = HostGrid.congruent(t)
+    d.buf[:] = (2 * uniform(115, d.buf.shape) - 1).astype(dtype)
+    want = d.copy()
+    O.lib().orc_laplacian_acc(u.ptr(), want.ptr(), nx + 1, ny, nz, .5, .25, .125, .71)
+    gu, gd = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(d)
+    D().laplacian_acc(gu, gd, nx + 1, ny, nz, .5, .25, .125, .71)
+    assert_bitexact(gd.to_host(), want, "laplacian")
+    for vd in (T.DIR_ZPOS, T.DIR_XNEG, T.DIR_YPOS):
+        want2 = d.copy()
+        O.lib().orc_thermal_force(want2.ptr(), t.ptr(), nx, ny, nz, vd, 7.1e4)
+        gd2, gt = D().DeviceGrid.from_host(d), D().DeviceGrid.from_host(t)
+        D().thermal_force(gd2, gt, nx, ny, nz, vd, 7.1e4)
+        assert_bitexact(gd2.to_host(), want2, f"thermal {vd}")

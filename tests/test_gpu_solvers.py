@@ -1,0 +1,318 @@
+"""Solver-level parity: the drop-in classes (libocu.so over libocu_hip.so)
+against the oracle's restatement of the same driver.
+
+The FP64 and FP32 multigrid solves follow the reference's FMG schedule step
+for step, so the GPU and the oracle must take the same number of V-cycles
+and their solutions agree bit for bit (every kernel is bit-exact; only the
+convergence norms are summed in a different order, which can only change a
+decision at a tie).  The mixed-precision solver is compared against the
+FP64 multigrid solution."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from opencurrent_amd import _types as T
+from opencurrent_amd._types import BC, BCSet, HostGrid, MGConfig
+from fields import divergence_rhs, trig_rhs, uniform, random_mac_velocity, mac_grids
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def D():
+    from opencurrent_amd import device
+    return device
+
+
+def S():
+    from opencurrent_amd import solvers
+    return solvers
+
+
+def bc_isotropic(axis, value):
+    """run_isotropic_test's BCs (mgdoubletest.cpp:138-160): periodic except a
+    Dirichlet pair on `axis`."""
+    s = BCSet.uniform(T.BC_PERIODIC)
+    pair = (("xpos", "xneg"), ("ypos", "yneg"), ("zpos", "zneg"))[axis]
+    for f in pair:
+        setattr(s, f, BC(T.BC_DIRICHELET, value, 0.0))
+    return s
+
+
+CASES = {
+    "neumann": (BCSet.uniform(T.BC_NEUMANN), -1, False),
+    "dirichlet_x0": (bc_isotropic(0, 0.0), 0, True),
+    "dirichlet_y1": (bc_isotropic(1, -1.0), 1, True),
+    "dirichlet_z321": (bc_isotropic(2, .321), 2, True),
+    "periodic": (BCSet.uniform(T.BC_PERIODIC), -1, False),
+    "all_dirichlet": (BCSet.uniform(T.BC_DIRICHELET, .5), None, True),
+}
+
+
+def rhs_for(case, n, h, dtype):
+    bc, axis, rand_init = CASES[case]
+    if axis is None:
+        return bc, HostGrid(*n, dtype=dtype), rand_init
+    return bc, trig_rhs(*n, *h, axis=axis, dtype=dtype), rand_init
+
+
+def init_guess(b, rand_init):
+    p = HostGrid.congruent(b)
+    if rand_init:
+        p.interior()[:] = (.5 - uniform(7, p.interior().shape)).astype(p.buf.dtype)
+    return p
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("fused", [True, False])
+def test_mg_solve_matches_oracle(gpu, dtype, case, fused):
+    n = (32, 32, 32)
+    h = (4.0 / 32, 4.0 / 32, 4.0 / 32)
+    tol = 1e-8 if dtype == np.float64 else 1e-3
+    bc, b, rand_init = rhs_for(case, n, h, dtype)
+    p0 = init_guess(b, rand_init)
+    cfg = MGConfig.make(bc)
+    want = p0.copy()
+    ok_o, res_o, vc_o, hist_o = O.mg_solve(want, b, h, cfg, tol, 16)
+    assert ok_o == 1
+    gb = D().DeviceGrid.from_host(b)
+    mg = S().MultigridPressure3D(gb, *h, bc, use_fused_relax=fused)
+    mg.set_pressure(p0)
+    ok, res = mg.solve(tol, 16)
+    st = mg.stats()
+    assert ok
+    assert st["vcycles"] == vc_o
+    assert np.allclose(st["history"], hist_o, rtol=1e-9, atol=0)
+    got = mg.pressure()
+    assert np.array_equal(got.buf, want.buf), f"max diff {np.abs(got.buf - want.buf).max()}"
+    assert res <= tol
+
+
+@pytest.mark.parametrize("n,h", [((64, 32, 48), (1 / 64, 1 / 32, 1 / 48)), ((32, 32, 32), (.25, .25, .5))])
+def test_mg_anisotropic_and_sym(gpu, n, h):
+    """anisotropic spacing (non power-of-two coefficients) and
+    make_symmetric_operator, as in run_all_bcs (mgdoubletest.cpp:290-305)"""
+    bc = BCSet.uniform(T.BC_NEUMANN)
+    b = trig_rhs(*n, *h)
+    for sym in (False, True):
+        cfg = MGConfig.make(bc, sym=sym)
+        want = HostGrid.congruent(b)
+        ok_o, _, vc_o, _ = O.mg_solve(want, b, h, cfg, 1e-8, 16)
+        gb = D().DeviceGrid.from_host(b)
+        mg = S().MultigridPressure3D(gb, *h, bc, sym=sym)
+        ok, res = mg.solve(1e-8, 16)
+        assert ok and ok_o == 1
+        assert mg.stats()["vcycles"] == vc_o
+        assert np.array_equal(mg.pressure().buf, want.buf)
+
+
+def test_mg_all_dirichlet_constant(gpu):
+    """run_all_dirichelet_test (mgdoubletest.cpp:186-213): zero RHS, value v on
+    every face -> p == v everywhere"""
+    for v in (0.0, .5, 1.0):
+        bc = BCSet.uniform(T.BC_DIRICHELET, v)
+        b = HostGrid(32, 32, 32)
+        gb = D().DeviceGrid.from_host(b)
+        mg = S().MultigridPressure3D(gb, 1 / 32, 1 / 32, 1 / 32, bc)
+        mg.set_pressure(init_guess(b, True))
+        ok, res = mg.solve(1e-8, 15)
+        assert ok
+        p = mg.pressure().interior()
+        assert np.abs(p - v).max() <= 1e-8
+
+
+def test_mg_run_vcycles_and_fmg(gpu):
+    bc = BCSet.uniform(T.BC_NEUMANN)
+    n, h = (32, 32, 32), (1 / 32,) * 3
+    b = divergence_rhs(*n, h[0])
+    for mode, fn in ((1, "run_vcycles"), (2, "run_fmg")):
+        cfg = MGConfig.make(bc, convergence=T.CONVERGENCE_NONE)
+        want = HostGrid.congruent(b)
+        O.mg_solve(want, b, h, cfg, 0.0, 3, mode=mode)
+        gb = D().DeviceGrid.from_host(b)
+        mg = S().MultigridPressure3D(gb, *h, bc, convergence=T.CONVERGENCE_NONE)
+        assert getattr(mg, fn)(3)
+        assert np.array_equal(mg.pressure().buf, want.buf), fn
+
+
+def test_mg_nonconvergence_retries_and_fails(gpu):
+    """solve() retries from a zero guess and then returns false
+    (sol_mgpressure3d.cpp:351-365)"""
+    bc = BCSet.uniform(T.BC_NEUMANN)
+    b = divergence_rhs(16, 16, 16, 1 / 16)
+    gb = D().DeviceGrid.from_host(b)
+    mg = S().MultigridPressure3D(gb, 1 / 16, 1 / 16, 1 / 16, bc)
+    ok, res = mg.solve(0.0, 1)
+    assert not ok and res > 0
+
+
+def test_mg_bad_arguments(gpu):
+    from opencurrent_amd._lib import OcuError
+    bc = BCSet.uniform(T.BC_NEUMANN)
+    b = D().DeviceGrid(16, 16, 16, 0, 0, 0)  # no ghost cells
+    with pytest.raises(OcuError):
+        S().MultigridPressure3D(b, .1, .1, .1, bc)
+    b = D().DeviceGrid(16, 16, 16)
+    mg = S().MultigridPressure3D(b, .1, .1, .1, bc)
+    assert mg.solve(float("nan"), 4)[0] is False
+    assert mg.solve(1e-6, 0)[0] is False
+
+
+@pytest.mark.parametrize("n", [(32, 32, 32), (64, 64, 64)])
+@pytest.mark.parametrize("fmg_first", [False, True])
+def test_mixed_ir_neumann(gpu, n, fmg_first):
+    """FP32 V-cycles in FP64 refinement reach 1e-10 relative residual on the
+    all-Neumann divergence problem and agree with the FP64 multigrid solution
+    (both mean-free) to the discretisation's round-off."""
+    h = (1.0 / n[0],) * 3
+    bc = BCSet.uniform(T.BC_NEUMANN)
+    b = divergence_rhs(*n, h[0])
+    rms_b = np.sqrt((b.interior() ** 2).mean())
+    tol = 1e-10 * rms_b
+    # oracle FP64 multigrid solution
+    ref = HostGrid.congruent(b)
+    ok_o, _, _, _ = O.mg_solve(ref, b, h, MGConfig.make(bc), 1e-3 * tol, 30)
+    assert ok_o == 1
+    xr = ref.interior() - ref.interior().mean()
+    # oracle restatement of the refinement
+    xo = HostGrid.congruent(b)
+    ok_i, res_i, it_i, vc_i = O.ir_solve(xo, b, h, MGConfig.make(bc), tol, 10, 4, 4, fmg_first)
+    assert ok_i == 1
+    gb = D().DeviceGrid.from_host(b)
+    mx = S().MultigridPressureMixed3D(gb, *h, bc, cycles=4, cycles_first=4, fmg_first=fmg_first)
+    ok, res = mx.solve(tol, 10)
+    st = mx.stats()
+    assert ok and res <= tol
+    assert st["ir_iterations"] == it_i
+    x = mx.pressure().interior()
+    scale = np.abs(xr).max()
+    assert np.abs(x - xr).max() <= 1e-9 * scale
+    assert np.abs(x - xo.interior()).max() <= 1e-9 * scale
+    assert abs(x.mean()) <= 1e-12 * scale
+
+
+def test_mixed_ir_dirichlet(gpu):
+    n, h = (32, 32, 32), (1 / 32,) * 3
+    bc = bc_isotropic(2, .321)
+    b = trig_rhs(*n, *h, axis=2)
+    ref = HostGrid.congruent(b)
+    O.mg_solve(ref, b, h, MGConfig.make(bc), 1e-13, 30)
+    gb = D().DeviceGrid.from_host(b)
+    mx = S().MultigridPressureMixed3D(gb, *h, bc)
+    ok, res = mx.solve(1e-11, 10)
+    assert ok
+    x = mx.pressure().interior()
+    assert np.abs(x - ref.interior()).max() <= 1e-9 * max(1.0, np.abs(ref.interior()).max())
+
+
+def rb_bc():
+    s = BCSet.uniform(T.BC_PERIODIC)
+    for f in ("zpos", "zneg"):
+        setattr(s, f, BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0, 1.0))
+    return s
+
+
+@pytest.mark.parametrize("dtype,tol,divtol", [(np.float64, 1e-10, 1e-10), (np.float32, 1e-4, 2e-4)])
+@pytest.mark.parametrize("case", ["periodic", "rb", "inflow"])
+def test_projection(gpu, dtype, tol, divtol, case):
+    """ProjectDoubleTest / ProjectTest (projectdoubletest.cpp:146-308,
+    projecttest.cpp:114-136): max |div| after the projection, plus parity
+    with the oracle's projection."""
+    nx, ny, nz = 32, 16, 16
+    hx, hy, hz = 1 / 32, 1 / 16, 1 / 16
+    if case == "periodic":
+        bc = BCSet.uniform(T.BC_PERIODIC)
+    elif case == "rb":
+        bc = rb_bc()
+    else:
+        bc = BCSet.uniform(T.BC_PERIODIC)
+        bc.xneg = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, .5, 0.0)
+        bc.xpos = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, .5, 0.0)
+    u, v, w, _ = random_mac_velocity(nx, ny, nz, dtype=dtype)
+    wu, wv, ww = u.copy(), v.copy(), w.copy()
+    pw = HostGrid.congruent(wu)
+    r = C.c_double()
+    ok_o = O.lib().orc_project(wu.ptr(), wv.ptr(), ww.ptr(), C.byref(bc), nx, ny, nz, hx, hy, hz, tol, pw.ptr(),
+                               C.byref(r))
+    du, dv, dw = (D().DeviceGrid.from_host(x) for x in (u, v, w))
+    pr = S().ProjectDivergence3D(du, dv, dw, bc, nx, ny, nz, hx, hy, hz)
+    ok, res = pr.solve(tol)
+    assert ok and ok_o == 1
+    for got, want, name in ((du, wu, "u"), (dv, wv, "v"), (dw, ww, "w")):
+        assert np.array_equal(got.to_host().buf, want.buf), name
+    pr.solve_divergence_only()
+    div = pr.divergence().interior()
+    assert np.abs(div).max() <= divtol
+
+
+def test_projection_mixed(gpu):
+    nx, ny, nz = 32, 32, 32
+    h = 1 / 32
+    bc = rb_bc()
+    u, v, w, _ = random_mac_velocity(nx, ny, nz)
+    du, dv, dw = (D().DeviceGrid.from_host(x) for x in (u, v, w))
+    pr = S().ProjectDivergence3D(du, dv, dw, bc, nx, ny, nz, h, h, h, mixed=True)
+    ok, res = pr.solve(1e-10)
+    assert ok
+    pr.solve_divergence_only()
+    assert np.abs(pr.divergence().interior()).max() <= 1e-10
+
+
+def ns_params(n, dtype, scheme=T.IT_FIRST_ORDER_UPWIND, time_step=T.TS_FORWARD_EULER):
+    from opencurrent_amd._lib import NSParams
+    p = NSParams()
+    p.nx, p.ny, p.nz = n
+    p.hx = p.hy = p.hz = 1.0 / n[2]
+    p.max_divergence = 1e-8 if dtype == np.float64 else 1e-4
+    tb = BCSet.uniform(T.BC_PERIODIC)
+    tb.zneg = BC(T.BC_DIRICHELET, 1.0, 0.0)
+    tb.zpos = BC(T.BC_DIRICHELET, 0.0, 0.0)
+    p.temp_bc = tb
+    p.flow_bc = rb_bc()
+    p.vertical_direction = T.DIR_ZPOS
+    p.gravity = -1.0
+    p.bouyancy = 1e5 * .71
+    p.viscosity = .71
+    p.thermal_diffusion = 1.0
+    p.advection_scheme = scheme
+    p.time_step = time_step
+    p.cfl_factor = .5
+    return p
+
+
+def rb_initial(n, dtype):
+    u, v, w, t = mac_grids(*n, dtype=dtype)
+    z = (np.arange(n[2]) + .5) / n[2]
+    t.interior()[:] = (1 - z)[None, None, :] + 1e-3 * (2 * uniform(2001, t.interior().shape) - 1)
+    return u, v, w, t
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("scheme", [T.IT_FIRST_ORDER_UPWIND, T.IT_SECOND_ORDER_CENTERED])
+@pytest.mark.parametrize("ts", [T.TS_FORWARD_EULER, T.TS_ADAMS_BASHFORD2])
+def test_ns_steps_match_oracle(gpu, dtype, scheme, ts):
+    """Rayleigh-Benard steps (rayleightiming.cpp set-up at 16^3): the GPU
+    Eqn_IncompressibleNS3D and the oracle's restatement stay bit-identical."""
+    n = (16, 16, 16)
+    p = ns_params(n, dtype, scheme, ts)
+    u, v, w, t = rb_initial(n, dtype)
+    from opencurrent_amd._types import dtype_code
+    ho = O.lib().orc_ns_create(dtype_code(dtype), C.byref(p), u.ptr(), v.ptr(), w.ptr(), t.ptr())
+    assert ho
+    ns = S().IncompressibleNS3D(p, u, v, w, t)
+    try:
+        for step in range(5):
+            dt_o = O.lib().orc_ns_max_stable_timestep(ho)
+            dt_g = ns.get_max_stable_timestep()
+            assert dt_g == dt_o
+            assert O.lib().orc_ns_advance_one_step(ho, dt_o) == 1
+            assert ns.advance_one_step(dt_g)
+        for which in range(5):
+            got = ns.field(which)
+            want = HostGrid.congruent(got)
+            assert O.lib().orc_ns_get(ho, which, want.ptr()) == 1
+            assert np.array_equal(got.buf, want.buf), f"field {which}: {np.abs(got.buf - want.buf).max()}"
+    finally:
+        O.lib().orc_ns_destroy(ho)

@@ -1,0 +1,77 @@
+"""CPU checks of the native boundary: both libraries load without a GPU and
+export every entry point their headers declare; the Python layout and the
+C ABI structs agree with the headers."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from opencurrent_amd import _lib
+from opencurrent_amd import _types as T
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(ocu_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+@pytest.mark.parametrize("header,loader", [("ocu_hip.h", _lib.hip), ("ocu.h", _lib.host)])
+def test_every_declared_symbol_is_exported(header, loader):
+    lib = loader()
+    names = declared(header)
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_the_headers():
+    assert set(declared("ocu_hip.h")) == set(_lib.HIP_SIGS)
+    assert set(declared("ocu.h")) == set(_lib.HOST_SIGS)
+
+
+def test_struct_sizes_match_the_c_layout():
+    # ocu_bc: int32 + 2 floats; ocu_bc_set: 6 of them (boundary_condition.h:38-112)
+    assert C.sizeof(T.BC) == 12
+    assert C.sizeof(T.BCSet) == 72
+    assert C.sizeof(T.Grid3DDesc) == 8 + 10 * 4 + 8 + 8
+    assert C.sizeof(T.MGLevelParams) == 11 * 8
+
+
+def test_host_only_entry_points_without_gpu():
+    hip = _lib.hip()
+    assert hip.ocu_norm_parts() == 2048
+    assert hip.ocu_reduce_workspace_bytes() == 8 * 2048
+
+
+def test_invalid_arguments_are_rejected_before_any_launch():
+    """Argument checks run on the host: a null or inconsistent descriptor is
+    an OCU_ERR_INVALID with a message, and nothing is launched."""
+    hip = _lib.hip()
+    assert hip.ocu_bc_fill_level1(None, None, 1.0, 1.0, 1.0, None) == -1
+    assert "null" in _lib.last_error()
+    g = T.layout(8, 8, 8)
+    g.base = 16  # bogus, but never dereferenced: the ghost check fails first
+    g0 = T.layout(8, 8, 8, 0, 0, 0)
+    g0.base = 16
+    bc = T.BCSet.uniform(T.BC_NEUMANN)
+    assert hip.ocu_bc_fill_level1(C.byref(g0), C.byref(bc), 1.0, 1.0, 1.0, None) == -1
+    assert "ghost" in _lib.last_error()
+    bad = T.BCSet.uniform(T.BC_FORCED_INFLOW_VARIABLE_SLIP)
+    assert hip.ocu_bc_fill_level1(C.byref(g), C.byref(bad), 1.0, 1.0, 1.0, None) == -1
+    assert "invalid boundary condition" in _lib.last_error()
+    c = T.layout(4, 4, 4)
+    c.base = 16
+    assert hip.ocu_mg_residual_restrict(C.byref(g), C.byref(g), None, C.byref(T.layout(3, 4, 4)), 1.0, 1.0, 1.0,
+                                        1.0, None, None) == -1
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    monkeypatch.setattr(_lib, "LIBDIR", tmp_path)
+    monkeypatch.setattr(_lib, "_hip", None)
+    with pytest.raises(_lib.NativeLibraryMissing):
+        _lib.hip()
