@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the projection-method hot path on one MI355X.
+
+Workload (one "step"): the 256^3 all-Neumann Poisson problem on the unit cube
+(h = 1/256) whose right-hand side is the divergence of a seeded U[-1,1] MAC
+velocity with its mean removed, solved from a zero guess to a relative
+residual ||b - Ax||_2 / ||b||_2 <= 1e-10 by FP32 V-cycles inside FP64
+iterative refinement (Sol_MultigridPressureMixed3DDeviceD).  Inputs are
+resident in HBM before the timed region.
+
+Printed (one JSON line): the solve throughput in Mcells/s (cells solved to
+1e-10 per second), the FP64 multigrid solve of the same problem to the same
+target for comparison, the error of the mixed solution against the FP64
+one, the roofline of the dominant kernel (the level-0 FP32 fused red+black
+sweep, timed with HIP events on the solver's stream) and a CPU baseline (the
+oracle's OpenMP FP64 multigrid on the host cores).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "MG time to 1e-10 rel. residual + NS step Mcells/s, FP32-refined vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--rel-tol", type=float, default=1e-10)
+    ap.add_argument("--cycles", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def rhs(n):
+    import fields
+    return fields.divergence_rhs(n, n, n, 1.0 / n)
+
+
+class Events:
+    def __init__(self, hip, stream):
+        self.hip, self.stream = hip, stream
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        hip.ocu_event_create(C.byref(self.a))
+        hip.ocu_event_create(C.byref(self.b))
+
+    def start(self):
+        self.hip.ocu_event_record(self.a, self.stream)
+
+    def stop_ms(self):
+        self.hip.ocu_event_record(self.b, self.stream)
+        self.hip.ocu_event_synchronize(self.b)
+        ms = C.c_float()
+        self.hip.ocu_event_elapsed_ms(C.byref(ms), self.a, self.b)
+        return ms.value
+
+
+def time_fused_sweep(n, reps, dtype=np.float32):
+    """Average duration of the level-0 fused red+black sweep at n^3 (the
+    dominant kernel of the V-cycle), launched back to back on the solver's
+    stream and timed with HIP events."""
+    from opencurrent_amd import _lib
+    from opencurrent_amd import _types as T
+    from opencurrent_amd.device import DeviceGrid
+    import oracle_lib as O
+    hip = _lib.hip()
+    stream = _lib.host().ocu_get_compute_stream()
+    bc = T.BCSet.uniform(T.BC_NEUMANN)
+    u = DeviceGrid(n, n, n, dtype=dtype)
+    o = DeviceGrid(n, n, n, dtype=dtype)
+    b = DeviceGrid(n, n, n, dtype=dtype)
+    u.tensor.uniform_(-1, 1)
+    b.tensor.uniform_(-1, 1)
+    p = O.level_params(bc, 0, 1.0 / n, 1.0 / n, 1.0 / n)
+    import torch
+    torch.cuda.synchronize()
+    args = (u.ptr(), o.ptr(), b.ptr(), C.byref(p), C.byref(bc), 0, stream)
+    for _ in range(5):
+        _lib.check(hip.ocu_mg_relax_fused(*args), "fused sweep")
+    ev = Events(hip, stream)
+    ev.start()
+    for _ in range(reps):
+        hip.ocu_mg_relax_fused(*args)
+    ms = ev.stop_ms() / reps
+    w = np.dtype(dtype).itemsize
+    alg_bytes = 3 * w * n ** 3  # read u and b, write u, once per cell
+    return ms, alg_bytes
+
+
+def cpu_baseline(n, rel_tol, budget_s=25.0):
+    """The oracle's FP64 multigrid (reference FMG schedule, OpenMP over
+    i-planes) on the host cores, same problem and target, repeated while the
+    budget allows."""
+    import oracle_lib as O
+    from opencurrent_amd import _types as T
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    b = rhs(n)
+    rms = float(np.sqrt((b.interior() ** 2).mean()))
+    cfg = T.MGConfig.make(T.BCSet.uniform(T.BC_NEUMANN))
+    times = []
+    t_all = time.perf_counter()
+    while True:
+        u = T.HostGrid.congruent(b)
+        t0 = time.perf_counter()
+        ok, res, vc, _ = O.mg_solve(u, b, (1.0 / n,) * 3, cfg, rel_tol * rms, 40)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_all > budget_s * 0.5 or len(times) >= 5:
+            break
+    t = float(np.median(times))
+    return {"value": round(n ** 3 / t / 1e6, 3), "unit": "Mcells/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "port",
+            "sample": f"oracle FP64 FMG (reference schedule, OpenMP) on the same {n}^3 problem to "
+                      f"{rel_tol:g} rel. residual, {len(times)} solve(s), median {t:.2f} s, {vc} V-cycles"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the multi-GPU sharding is a "
+                                   "design note, not built)"}))
+        sys.exit(2)
+    import torch
+    from opencurrent_amd import _lib
+    from opencurrent_amd import _types as T
+    from opencurrent_amd import device, solvers
+    device.require_gpu()
+    hip = _lib.hip()
+    stream = _lib.host().ocu_get_compute_stream()
+
+    n = args.n
+    cells = n ** 3
+    h = 1.0 / n
+    bc = T.BCSet.uniform(T.BC_NEUMANN)
+    b_host = rhs(n)
+    rms_b = float(np.sqrt((b_host.interior() ** 2).mean()))
+    tol = args.rel_tol * rms_b
+    gb = device.DeviceGrid.from_host(b_host)
+    torch.cuda.synchronize()
+
+    # ---- mixed precision (the measured workload)
+    mx = solvers.MultigridPressureMixed3D(gb, h, h, h, bc, cycles=args.cycles, cycles_first=args.cycles)
+    zero = T.HostGrid.congruent(b_host)
+    xdesc = mx.pressure_desc()
+
+    def solve_mixed():
+        hip.ocu_grid_clear(C.byref(xdesc), 0.0, stream)
+        ok, res = mx.solve(tol, 20)
+        if not ok:
+            raise RuntimeError(f"mixed solve did not reach {args.rel_tol:g}: {res / rms_b:.3e}")
+        return res
+
+    for _ in range(args.warmup):
+        solve_mixed()
+    hip.ocu_stream_synchronize(stream)
+    torch.cuda.synchronize()
+    ev = Events(hip, stream)
+    t0 = time.perf_counter()
+    ev.start()
+    for _ in range(args.steps):
+        res = solve_mixed()
+    gpu_ms = ev.stop_ms()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = gpu_ms / args.steps
+    st = mx.stats()
+    x_mixed = mx.pressure().interior().copy()
+
+    out = {
+        "metric": METRIC,
+        "value": round(cells * args.steps / (gpu_ms / 1e3) / 1e6, 2),
+        "unit": "Mcells/s (256^3 cells solved to 1e-10 rel. residual per second)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (FP32 V-cycles, FP64 refinement)",
+        "data": "synthetic (seeded divergence of a U[-1,1] MAC velocity, mean removed)",
+        "config": {"workload": f"mixed-precision MG Poisson {n}^3, unit cube, Neumann faces, to "
+                               f"{args.rel_tol:g} relative residual (FP32 V-cycles in FP64 IR)",
+                   "grid": [n, n, n], "levels": 7 if n == 256 else None, "nu1": 2, "nu2": 2,
+                   "v_cycles_per_refinement": args.cycles},
+        "rel_residual": res / rms_b,
+        "ir_iterations": st["ir_iterations"],
+        "vcycles": st["vcycles"],
+    }
+
+    # ---- FP64 multigrid on the same problem (reference algorithm, FMG)
+    if not args.no_fp64:
+        mg = solvers.MultigridPressure3D(gb, h, h, h, bc)
+        pdesc = mg.pressure_desc()
+
+        def solve_fp64():
+            hip.ocu_grid_clear(C.byref(pdesc), 0.0, stream)
+            ok, r = mg.solve(tol, 40)
+            if not ok:
+                raise RuntimeError("FP64 solve did not converge")
+            return r
+
+        for _ in range(args.warmup):
+            solve_fp64()
+        ev.start()
+        reps64 = max(1, min(args.steps, 5))
+        for _ in range(reps64):
+            solve_fp64()
+        ms64 = ev.stop_ms() / reps64
+        x64 = mg.pressure().interior()
+        x64 = x64 - x64.mean()
+        err = float(np.abs(x_mixed - x64).max() / np.abs(x64).max())
+        out["fp64_ms_per_step"] = round(ms64, 4)
+        out["fp64_mcells_s"] = round(cells / (ms64 / 1e3) / 1e6, 2)
+        out["speedup_vs_fp64"] = round(ms64 / ms_step, 3)
+        out["fp64_vcycles"] = mg.stats()["vcycles"]
+        out["max_rel_err_vs_fp64"] = err
+        del mg
+
+    # ---- roofline of the dominant kernel
+    kms, kbytes = time_fused_sweep(n, args.kernel_reps)
+    achieved = kbytes / (kms / 1e3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": "k_relax_fused<float> (level-0 red+black sweep)",
+                       "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes}
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, args.rel_tol)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
