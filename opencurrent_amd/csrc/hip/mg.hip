@@ -232,6 +232,205 @@ __global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T>
 }
 
 // ------------------------------------------------------------------------
+// fused red+black iteration, whole-row variant
+// ------------------------------------------------------------------------
+//
+// Used when one wave holds a whole k-row: lane l owns the V consecutive
+// cells k = V*l .. V*l+V-1 (V = 1, 2 or 4, the smallest with 64*V >= nz),
+// so rows move as dwordx(V) accesses and no k-halo is recomputed.  The
+// k-neighbours across lanes come from DPP wave shifts; the ghost cells
+// k = -1 and k = nz travel in a per-lane register g (lane 0 holds k = -1;
+// the lane just past the last cell, or lane 63 when the row fills the wave,
+// holds k = nz).  A block of NW waves owns NW rows j (NW-4 output rows and a
+// ring of two rows each side) and marches along i like the tiled kernel:
+// first colour of plane p on the ring-1 rows from U0, second colour of plane
+// p-1 on the output rows from U1, j-neighbours through LDS.
+
+template <typename T, int V>
+struct alignas(sizeof(T) * V) Vec {
+  T v[V];
+};
+
+// DPP wave_shr:1 / wave_shl:1; a lane whose source is off the wave keeps old
+__device__ __forceinline__ int dpp_lower_i(int src, int old) {
+  return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_upper_i(int src, int old) {
+  return __builtin_amdgcn_update_dpp(old, src, 0x130, 0xf, 0xf, false);
+}
+// value of lane l-1 (lane 0: old)
+__device__ __forceinline__ float from_lower(float src, float old) {
+  return __int_as_float(dpp_lower_i(__float_as_int(src), __float_as_int(old)));
+}
+__device__ __forceinline__ double from_lower(double src, double old) {
+  long long s = __double_as_longlong(src), o = __double_as_longlong(old);
+  int lo = dpp_lower_i((int)s, (int)o);
+  int hi = dpp_lower_i((int)(s >> 32), (int)(o >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// value of lane l+1 (lane 63: old)
+__device__ __forceinline__ float from_upper(float src, float old) {
+  return __int_as_float(dpp_upper_i(__float_as_int(src), __float_as_int(old)));
+}
+__device__ __forceinline__ double from_upper(double src, double old) {
+  long long s = __double_as_longlong(src), o = __double_as_longlong(old);
+  int lo = dpp_upper_i((int)s, (int)o);
+  int hi = dpp_upper_i((int)(s >> 32), (int)(o >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Rows of a block: NW waves of R rows each; the outer two rows on each side
+// are the halo, so a block writes R*NW-4 rows.
+template <typename T, int V, int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
+                                                               typename DevCFor<T>::type c, int first_color,
+                                                               int tiles_j, int ichunk, int ntiles) {
+  typedef Vec<T, V> VT;
+  constexpr int NR = R * NW;
+  __shared__ VT sU0[NR][64];
+  __shared__ VT sU1[2][NR][64];
+
+  int per = gridDim.x / FXCD;
+  int logical = (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD);
+  if (logical >= ntiles) return;  // uniform per block
+  const int tj = logical % tiles_j, ti = logical / tiles_j;
+
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int t0 = w * R;  // first block row of this wave
+  const int nx = uin.nx, ny = uin.ny, nz = uin.nz;
+  const int jb = tj * (NR - 4) - 2 + t0;  // j of the wave's first row
+  const int k0 = lane * V;
+  const int i0 = ti * ichunk, i1 = min(nx, i0 + ichunk);
+  const bool active = k0 < nz;
+  const int hi_lane = (nz == 64 * V) ? 63 : nz / V;
+  const int gk = lane == 0 ? -1 : nz;
+  const bool g_in_data = lane == hi_lane && nz < 64 * V;
+  const int kupd = 2 * (nz >> 1);
+  // Loads are unconditional so that the prefetches stay in flight across
+  // iterations: addresses are clamped into the allocated ghost box (rows
+  // -1..ny, planes -1..nx, cells 0..nz-1) and the values read for rows,
+  // planes or lanes outside it only feed updates that are dropped.
+  const int kc = active ? k0 : 0;
+  const T *urow[R];
+  const T *brow[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    urow[r] = uin.p + (int64_t)min(max(jb + r, -1), ny) * uin.ys;
+    brow[r] = b.p + (int64_t)min(max(jb + r, 0), ny - 1) * b.ys + kc;
+  }
+  auto inner_j = [&](int r) { return jb + r >= 0 && jb + r < ny; };
+  auto ring1 = [&](int r) { return t0 + r >= 1 && t0 + r <= NR - 2; };
+  auto tile = [&](int r) { return t0 + r >= 2 && t0 + r <= NR - 3; };
+
+  auto load = [&](int r, int p, VT &d, T &g) {
+    const T *row = urow[r] + (int64_t)min(max(p, -1), nx) * uin.xs;
+    d = *reinterpret_cast<const VT *>(row + kc);
+    g = row[gk];
+    if (g_in_data) d.v[0] = g;
+  };
+  auto loadb = [&](int r, int p) -> VT {
+    return *reinterpret_cast<const VT *>(brow[r] + (int64_t)min(max(p, 0), nx - 1) * b.xs);
+  };
+
+  // per row: U0 planes p-1 (a), p (cc), p+1 (n) and the prefetched p+2 (nn);
+  // U1 planes p-2, p-1, p; b planes p-1, p and the prefetched p+1
+  VT a[R], cc[R], n[R], nn[R], u1m2[R], u1m1[R], u1[R], bprev[R], bcur[R], bn[R];
+  T ga[R], gc[R], gn[R], gnn[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    load(r, i0 - 2, a[r], ga[r]);
+    load(r, i0 - 1, cc[r], gc[r]);
+    load(r, i0, n[r], gn[r]);
+    bn[r] = loadb(r, i0 - 1);
+#pragma unroll
+    for (int e = 0; e < V; e++) u1m2[r].v[e] = u1m1[r].v[e] = bprev[r].v[e] = (T)0;
+  }
+
+  for (int p = i0 - 1; p <= i1; ++p) {
+    const bool p_in = p >= 0 && p < nx;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      load(r, p + 2, nn[r], gnn[r]);
+      bcur[r] = bn[r];
+      bn[r] = loadb(r, p + 1);
+      sU0[t0 + r][lane] = cc[r];
+    }
+    __syncthreads();
+
+    // first colour on plane p (rows of ring 1); j-neighbours inside the
+    // wave come from registers, across waves from LDS
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const T lo = from_lower(cc[r].v[V - 1], gc[r]), hi = from_upper(cc[r].v[0], gc[r]);
+      u1[r] = cc[r];
+      if (ring1(r) && inner_j(r) && p_in) {
+        const int j = jb + r;
+        const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[t0 - 1][lane];
+        const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[t0 + R][lane];
+#pragma unroll
+        for (int e = 0; e < V; e++) {
+          const int k = k0 + e;
+          if (k < kupd && ((p + j + k) & 1) == first_color) {
+            const T d = diag_dev<T>(c, p, j, k, nx, ny, nz);
+            u1[r].v[e] = relax_pt(c, cc[r].v[e], e > 0 ? cc[r].v[e - 1] : lo, e < V - 1 ? cc[r].v[e + 1] : hi,
+                                  ym.v[e], yp.v[e], a[r].v[e], n[r].v[e], bcur[r].v[e], d);
+          }
+        }
+      }
+      sU1[p & 1][t0 + r][lane] = u1[r];
+    }
+    __syncthreads();
+
+    // second colour on plane q = p-1 (output rows); U1 == U0 on ghosts
+    const int q = p - 1;
+    const bool q_out = q >= i0 && q < nx;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const T lo = from_lower(u1m1[r].v[V - 1], ga[r]), hi = from_upper(u1m1[r].v[0], ga[r]);
+      if (tile(r) && inner_j(r) && q_out) {
+        const int j = jb + r;
+        const VT ym = r > 0 ? u1m1[r > 0 ? r - 1 : 0] : sU1[q & 1][t0 - 1][lane];
+        const VT yp = r < R - 1 ? u1m1[r < R - 1 ? r + 1 : 0] : sU1[q & 1][t0 + R][lane];
+        VT v = u1m1[r];
+#pragma unroll
+        for (int e = 0; e < V; e++) {
+          const int k = k0 + e;
+          if (k < kupd && ((q + j + k) & 1) != first_color) {
+            const T d = diag_dev<T>(c, q, j, k, nx, ny, nz);
+            v.v[e] = relax_pt(c, u1m1[r].v[e], e > 0 ? u1m1[r].v[e - 1] : lo, e < V - 1 ? u1m1[r].v[e + 1] : hi,
+                              ym.v[e], yp.v[e], u1m2[r].v[e], u1[r].v[e], bprev[r].v[e], d);
+          }
+        }
+        if (active) *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      a[r] = cc[r]; ga[r] = gc[r];
+      cc[r] = n[r]; gc[r] = gn[r];
+      n[r] = nn[r]; gn[r] = gnn[r];
+      u1m2[r] = u1m1[r]; u1m1[r] = u1[r];
+      bprev[r] = bcur[r];
+    }
+  }
+}
+
+// launch shape (rows per wave R, waves per block NW) of the whole-row kernel
+// for FP32 rows and FP64 rows of four cells per lane; the macros exist for
+// tuning builds
+#ifndef OCU_ROWS_F32
+#define OCU_ROWS_F32 2, 10
+#endif
+#ifndef OCU_ROWS_F64
+#define OCU_ROWS_F64 1, 12
+#endif
+template <int R_, int NW_> struct Shape { static constexpr int R = R_, NW = NW_; };
+template <typename T, int V> struct RowsShape : Shape<OCU_ROWS_F32> {};
+template <int V> struct RowsShape<double, V> : Shape<OCU_ROWS_F32> {};
+template <> struct RowsShape<double, 4> : Shape<OCU_ROWS_F64> {};
+
+
+// ------------------------------------------------------------------------
 // residual, restriction, fused residual->restriction, norms
 // ------------------------------------------------------------------------
 
@@ -568,6 +767,74 @@ int check_pair(const ocu_grid3d *u, const ocu_grid3d *b, const char *what) {
   return OCU_OK;
 }
 
+// Lanes per cell of the whole-row fused kernel for this problem, or 0 when
+// the tiled kernel must run: the row must fit one wave (nz <= 256), split
+// evenly over the lanes, and every V-vector access must be aligned.
+int row_vec_width(const ocu_grid3d *u, const ocu_grid3d *o, const ocu_grid3d *b) {
+  const int nz = u->nz;
+  const int V = nz <= 64 ? 1 : nz <= 128 ? 2 : nz <= 256 ? 4 : 0;
+  if (!V || nz % V) return 0;
+  const size_t w = u->dtype == OCU_F64 ? 8 : 4;
+  for (const ocu_grid3d *g : {u, o, b}) {
+    if (g->pnz % V || ((int64_t)g->pny * g->pnz) % V) return 0;
+    if (((uintptr_t)g->base + (uintptr_t)g->shift * w) % (w * V)) return 0;
+  }
+  return V;
+}
+
+// Concurrent blocks of a kernel on the current device (CUs x blocks per CU),
+// cached per device and kernel.
+template <typename K>
+int device_slots(K kern, int threads) {
+  static int cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cache[dev]) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per <= 0) per = 1;
+    cache[dev] = cus * per;
+  }
+  return cache[dev];
+}
+
+// i-planes per block of the whole-row kernel.  A block reads ichunk+3
+// planes to write ichunk (the two-colour halo), and blocks run in rounds of
+// `slots`: pick the chunk that minimises rounds x planes per block.  The
+// OCU_FUSED_ICHUNK environment variable overrides the choice (tuning).
+int fused_ichunk(int nx, int tiles_j, int slots) {
+  if (const char *s = getenv("OCU_FUSED_ICHUNK")) {
+    int v = atoi(s);
+    if (v > 0) return v;
+  }
+  static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
+  int best = 4;
+  int64_t best_cost = INT64_MAX;
+  for (int ic : cand) {
+    const int64_t nb = (int64_t)tiles_j * ((nx + ic - 1) / ic);
+    const int64_t cost = ((nb + slots - 1) / slots) * (min(ic, nx) + 3);
+    if (cost < best_cost) best_cost = cost, best = ic;
+    if (ic >= nx) break;
+  }
+  return best;
+}
+
+template <typename T, int V>
+int launch_fused_rows(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                      const typename DevCFor<T>::type &c, int first_color, hipStream_t stream) {
+  typedef RowsShape<T, V> RS;
+  constexpr int NR = RS::R * RS::NW;
+  auto kern = k_relax_fused_rows<T, V, RS::R, RS::NW>;
+  const int nx = u_in->nx, ny = u_in->ny;
+  const int tiles_j = (ny + NR - 5) / (NR - 4);
+  const int ichunk = fused_ichunk(nx, tiles_j, device_slots(kern, 64 * RS::NW));
+  const int ntiles = tiles_j * ((nx + ichunk - 1) / ichunk);
+  const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+  kern<<<nblocks, dim3(64, RS::NW), 0, stream>>>(dgrid<T>(u_in), dgrid<T>(u_out), dgrid<T>(b), c, first_color,
+                                                 tiles_j, ichunk, ntiles);
+  OCU_HIP_LAUNCHED("ocu_mg_relax_fused");
+}
+
 }  // namespace
 
 extern "C" {
@@ -612,6 +879,18 @@ int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const oc
   }
   const int nx = u_in->nx, ny = u_in->ny, nz = u_in->nz;
   if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  if (int V = row_vec_width(u_in, u_out, b)) {
+    if (u_in->dtype == OCU_F64) {
+      const DevC64 c = dev_c64(p);
+      return V == 1   ? launch_fused_rows<double, 1>(u_in, u_out, b, c, first_color, S(stream))
+             : V == 2 ? launch_fused_rows<double, 2>(u_in, u_out, b, c, first_color, S(stream))
+                      : launch_fused_rows<double, 4>(u_in, u_out, b, c, first_color, S(stream));
+    }
+    const DevC32 c = dev_c32(p);
+    return V == 1   ? launch_fused_rows<float, 1>(u_in, u_out, b, c, first_color, S(stream))
+           : V == 2 ? launch_fused_rows<float, 2>(u_in, u_out, b, c, first_color, S(stream))
+                    : launch_fused_rows<float, 4>(u_in, u_out, b, c, first_color, S(stream));
+  }
   int tiles_k = (nz + FOK - 1) / FOK, tiles_j = (ny + FOJ - 1) / FOJ;
   // enough i-chunks to give every CU a few blocks, but long enough marches
   int ichunk = nx >= 256 ? 16 : nx >= 64 ? 8 : 4;

@@ -154,7 +154,13 @@ def test_relax_colour(gpu, dtype, variant, size, case):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("size", [(16, 16, 16), (40, 30, 70), (9, 20, 13), (64, 64, 64)])
+@pytest.mark.parametrize("size", [
+    # whole-row kernel: one, two or four cells per lane, rows that fill the
+    # wave or stop short of it
+    (16, 16, 16), (9, 20, 13), (64, 64, 64), (40, 30, 70), (20, 40, 66), (10, 18, 128), (12, 16, 200),
+    (8, 26, 256),
+    # tiled kernel: rows that do not split evenly over the lanes or exceed a wave
+    (6, 14, 130), (7, 15, 101), (5, 13, 300)])
 @pytest.mark.parametrize("case", ["neumann0", "dirichlet", "neumann_v"])
 @pytest.mark.parametrize("first", [0, 1])
 def test_relax_fused_matches_two_colours(gpu, dtype, size, case, first):

@@ -222,14 +222,21 @@ def test_projection(gpu, dtype, tol, divtol, case):
     with the oracle's projection."""
     nx, ny, nz = 32, 16, 16
     hx, hy, hz = 1 / 32, 1 / 16, 1 / 16
+    inflow = .5
+    if dtype == np.float32:
+        # the FP32 solver stalls near 3e-4 on a 1/32 grid (the oracle's
+        # restatement of the reference driver does too): use the reference
+        # test's spacing and inflow speed (projecttest.cpp:145-167)
+        hx, hy, hz = .5, .5, 1.0
+        inflow = .2
     if case == "periodic":
         bc = BCSet.uniform(T.BC_PERIODIC)
     elif case == "rb":
         bc = rb_bc()
     else:
         bc = BCSet.uniform(T.BC_PERIODIC)
-        bc.xneg = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, .5, 0.0)
-        bc.xpos = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, .5, 0.0)
+        bc.xneg = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, inflow, 0.0)
+        bc.xpos = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, inflow, 0.0)
     if dtype == np.float64:
         u, v, w, _ = random_mac_velocity(nx, ny, nz, dtype=dtype)
     else:
