@@ -116,6 +116,18 @@ int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color,
 int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                        const ocu_mg_level_params *p, const ocu_bc_set *bc, int first_color,
                        void *stream);
+/* ocu_mg_relax_fused with two options (flags, OR-ed):
+ *   OCU_FUSED_WRITE_GHOSTS: also write the first ghost layer of u_out exactly
+ *     as apply_3d_boundary_conditions_level1(u_out, bc, hx, hy, hz) would
+ *     (grid3dboundary.cu:237-347), so no separate fill is needed;
+ *   OCU_FUSED_ZERO_INPUT: u_in is taken to be the zero grid with its ghosts
+ *     filled by that same fill (the state after clear_zero + level-1 fill,
+ *     sol_mgpressure3d.cpp:168-170) and is not read; u_in may equal u_out.
+ * Either flag needs every face Neumann or Dirichlet (else
+ * OCU_ERR_UNSUPPORTED). */
+int ocu_mg_relax_fused_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                          const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx,
+                          double hy, double hz, int first_color, int flags, void *stream);
 /* r = b - A u over the interior; calculate_residual (devd.cu:94-114,
  * devf.cu:68-90; constants as in the launchers devd.cu:1372-1405). */
 int ocu_mg_residual(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r, double h,
@@ -140,6 +152,12 @@ int ocu_norm_finalize(const double *norm_ws, double *out, void *stream);
 /* u_fine += trilinear prolongation of u_coarse; prolong (devd.cu:118-223).
  * quirk = 1 reproduces the reference corner choice (devd.cu:181-185). */
 int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, void *stream);
+/* ocu_mg_prolong followed by apply_3d_boundary_conditions_level1(u_fine,
+ * bc, hx, hy, hz) in the same launch (the pair in prolong,
+ * sol_mgpressure3ddev.cu:281-300).  Needs u_fine exactly twice u_coarse
+ * and every face Neumann or Dirichlet. */
+int ocu_mg_prolong_bc(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk,
+                      const ocu_bc_set *bc, double hx, double hy, double hz, void *stream);
 /* Coarsest-level relaxation in one workgroup (replaces relax_on_host,
  * sol_mgpressure3ddev.cu:134-188, and its two PCIe round trips).
  * mode 0: reference semantics - `iterations` RB-GS iterations with the

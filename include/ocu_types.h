@@ -112,6 +112,8 @@ typedef struct ocu_grid3d {
  *     (sol_mgpressure3ddev.cu:82-125), the coarsest-level relaxation, whose
  *     coefficient products are formed in double even for T=float. */
 enum ocu_relax_variant { OCU_RELAX_DEVICE = 0, OCU_RELAX_HOST = 1 };
+/* options of ocu_mg_relax_fused_bc */
+enum ocu_fused_flags { OCU_FUSED_WRITE_GHOSTS = 1, OCU_FUSED_ZERO_INPUT = 2 };
 
 /* Per-level constants of the multigrid operator
  *   f_x (u_{i+1}+u_{i-1}) + f_y (..) + f_z (..) - 2(f_x+f_y+f_z) u = h^2 b

@@ -71,12 +71,14 @@ HIP_SIGS = {
     "ocu_bc_fill_mac_level1": (I, [GP, GP, GP, BP, P]),
     "ocu_mg_relax": (I, [GP, GP, I, MP, I, P]),
     "ocu_mg_relax_fused": (I, [GP, GP, GP, MP, BP, I, P]),
+    "ocu_mg_relax_fused_bc": (I, [GP, GP, GP, MP, BP, D, D, D, I, I, P]),
     "ocu_mg_residual": (I, [GP, GP, GP, D, D, D, D, P]),
     "ocu_mg_restrict": (I, [GP, GP, P]),
     "ocu_mg_residual_restrict": (I, [GP, GP, GP, GP, D, D, D, D, DP, P]),
     "ocu_mg_residual_norm": (I, [GP, GP, GP, D, D, D, D, DP, P]),
     "ocu_norm_finalize": (I, [DP, DP, P]),
     "ocu_mg_prolong": (I, [GP, GP, I, P]),
+    "ocu_mg_prolong_bc": (I, [GP, GP, I, BP, D, D, D, P]),
     "ocu_mg_coarse_relax": (I, [GP, GP, I, I, MP, BP, D, D, D, I, P]),
     "ocu_ir_residual": (I, [GP, GP, D, GP, D, D, D, D, D, DP, P]),
     "ocu_norm3_finalize": (I, [DP, DP, P]),

@@ -30,6 +30,7 @@ IT_FIRST_ORDER_UPWIND, IT_SECOND_ORDER_CENTERED = 1, 2
 DIR_XPOS, DIR_XNEG, DIR_YPOS, DIR_YNEG, DIR_ZPOS, DIR_ZNEG = 0x05, 0x06, 0x09, 0x0A, 0x11, 0x12
 
 RELAX_DEVICE, RELAX_HOST = 0, 1
+FUSED_WRITE_GHOSTS, FUSED_ZERO_INPUT = 1, 2
 
 TS_FORWARD_EULER, TS_ADAMS_BASHFORD2 = 1, 2
 

@@ -5,6 +5,7 @@
 // lanes always walks 64 consecutive k of one (i,j) row.  All arithmetic
 // keeps the reference's operation order; the library is built with
 // -ffp-contract=off so the results match the CPU restatement bit for bit.
+#include "ghost_l1.h"
 #include "ocu_hip_common.h"
 
 using namespace ocu_hip;
@@ -164,10 +165,11 @@ __global__ __launch_bounds__(256) void k_relax_host(DGrid<T> u, DGrid<T> b, Host
 
 constexpr int FBK = 64, FBJ = 16, FOK = FBK - 4, FOJ = FBJ - 4, FXCD = 8;
 
-template <typename T>
+template <typename T, bool ZERO>
 __global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
                                                           typename DevCFor<T>::type c, int first_color,
-                                                          int tiles_k, int tiles_j, int ichunk, int ntiles) {
+                                                          int tiles_k, int tiles_j, int ichunk, int ntiles,
+                                                          GhostL1<T> gw, int wghost) {
   __shared__ T sU0[FBJ][FBK];
   __shared__ T sU1[2][FBJ][FBK];
 
@@ -196,7 +198,9 @@ __global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T>
   const int64_t bcol = (int64_t)j * b.ys + k;
 
   auto ld0 = [&](int p) -> T {
-    return (loadable && p >= -1 && p <= nx) ? uin.p[(int64_t)p * uin.xs + col] : (T)0;
+    if (!(loadable && p >= -1 && p <= nx)) return (T)0;
+    if (ZERO) return (T)0;  // homogeneous faces: the zero grid's ghosts are +0.0
+    return uin.p[(int64_t)p * uin.xs + col];
   };
 
   T a = ld0(i0 - 2), cc = ld0(i0 - 1);
@@ -224,6 +228,7 @@ __global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T>
                      u1m2, u1, bprev, d);
       }
       uout.p[(int64_t)q * uout.xs + col] = v;
+      if (wghost) gw.write(uout, q, j, k, v);
     }
     a = cc; cc = n;
     u1m2 = u1m1; u1m1 = u1;
@@ -279,12 +284,33 @@ __device__ __forceinline__ double from_upper(double src, double old) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// One ghost row of o derived from the row (q,j) of the cells this lane wrote
+// (k0 .. k0+V-1, values v): the row at offset (sx,sy) gets the x/y face
+// transforms (skipped for sx = sy = 0, the row itself) and its two z-end
+// ghosts the z transforms on top.
+template <typename T, int V>
+__device__ __forceinline__ void write_ghost_row(const DGrid<T> &o, const GhostL1<T> &gw, int q, int j, int k0,
+                                                const Vec<T, V> &v, int sx, int sy) {
+  Vec<T, V> w;
+#pragma unroll
+  for (int e = 0; e < V; e++) w.v[e] = gw.chain(sx, sy, 0, v.v[e]);
+  T *row = o.p + (int64_t)(q + sx) * o.xs + (int64_t)(j + sy) * o.ys;
+  if (sx || sy) *reinterpret_cast<Vec<T, V> *>(row + k0) = w;
+  if (k0 == 0) row[-1] = gw.xf(5, w.v[0]);
+  if (k0 + V == o.nz) row[o.nz] = gw.xf(4, w.v[V - 1]);
+}
+
 // Rows of a block: NW waves of R rows each; the outer two rows on each side
 // are the halo, so a block writes R*NW-4 rows.
-template <typename T, int V, int R, int NW>
+//
+// ZERO: u_in is the zero grid with its level-1 ghosts (not read; the values
+// are the ghost transforms of 0).  wghost: also write the level-1 ghost layer
+// of u_out from each boundary cell (Neumann/Dirichlet faces only).
+template <typename T, int V, int R, int NW, bool ZERO>
 __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
                                                                typename DevCFor<T>::type c, int first_color,
-                                                               int tiles_j, int ichunk, int ntiles) {
+                                                               int tiles_j, int ichunk, int ntiles,
+                                                               GhostL1<T> gw, int wghost) {
   typedef Vec<T, V> VT;
   constexpr int NR = R * NW;
   __shared__ VT sU0[NR][64];
@@ -323,10 +349,16 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
   auto tile = [&](int r) { return t0 + r >= 2 && t0 + r <= NR - 3; };
 
   auto load = [&](int r, int p, VT &d, T &g) {
-    const T *row = urow[r] + (int64_t)min(max(p, -1), nx) * uin.xs;
-    d = *reinterpret_cast<const VT *>(row + kc);
-    g = row[gk];
-    if (g_in_data) d.v[0] = g;
+    if (ZERO) {  // homogeneous faces: every ghost of the zero grid is +0.0
+#pragma unroll
+      for (int e = 0; e < V; e++) d.v[e] = (T)0;
+      g = (T)0;
+    } else {
+      const T *row = urow[r] + (int64_t)min(max(p, -1), nx) * uin.xs;
+      d = *reinterpret_cast<const VT *>(row + kc);
+      g = row[gk];
+      if (g_in_data) d.v[0] = g;
+    }
   };
   auto loadb = [&](int r, int p) -> VT {
     return *reinterpret_cast<const VT *>(brow[r] + (int64_t)min(max(p, 0), nx - 1) * b.xs);
@@ -401,7 +433,16 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
                               ym.v[e], yp.v[e], u1m2[r].v[e], u1[r].v[e], bprev[r].v[e], d);
           }
         }
-        if (active) *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
+        if (active) {
+          *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
+          if (wghost) {  // ghosts of this row: z ends, x and y face rows
+            write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 0);
+            if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, -1);
+            if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 1);
+            if (q == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, -1, 0);
+            if (q == nx - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 1, 0);
+          }
+        }
       }
     }
 #pragma unroll
@@ -411,6 +452,22 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
       n[r] = nn[r]; gn[r] = gnn[r];
       u1m2[r] = u1m1[r]; u1m1[r] = u1[r];
       bprev[r] = bcur[r];
+    }
+  }
+  if (!wghost || !active || (i0 != 0 && i1 != nx)) return;
+  // the x-y edge ghost rows (boundary plane and boundary row) from this
+  // wave's own stores: same lane and addresses, so program order makes the
+  // values visible
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int j = jb + r;
+    if (!(tile(r) && inner_j(r)) || (j != 0 && j != ny - 1)) continue;
+    for (int sx = -1; sx <= 1; sx += 2) {
+      const int q = sx < 0 ? 0 : nx - 1;
+      if (q < i0 || q >= i1) continue;
+      const VT v = *reinterpret_cast<const VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0);
+      if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, -1);
+      if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, 1);
     }
   }
 }
@@ -589,7 +646,7 @@ template <> struct PW<float> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int quirk) {
+__global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int quirk, GhostL1<T> gw, int wghost) {
   int kc = blockIdx.x * TK + threadIdx.x;
   int jc = blockIdx.y * TJ + threadIdx.y;
   int ic = blockIdx.z;
@@ -607,7 +664,9 @@ __global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int q
     T XY = q[id + sx + sy], XZ = q[id + sx + sz], YZ = q[id + sy + sz];
     T XYZ = q[id + cx + sy + sz];
     T *f = &uf.at(2 * ic + dx, 2 * jc + dy, 2 * kc + dz);
-    *f = *f + PW<T>::c0 * C + PW<T>::c1 * (X + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+    const T v = *f + PW<T>::c0 * C + PW<T>::c1 * (X + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+    *f = v;
+    if (wghost) gw.write(uf, 2 * ic + dx, 2 * jc + dy, 2 * kc + dz, v);
   }
 }
 
@@ -819,20 +878,121 @@ int fused_ichunk(int nx, int tiles_j, int slots) {
   return best;
 }
 
-template <typename T, int V>
+template <typename T, int V, bool ZERO>
 int launch_fused_rows(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
-                      const typename DevCFor<T>::type &c, int first_color, hipStream_t stream) {
+                      const typename DevCFor<T>::type &c, int first_color, const GhostL1<T> &gw, int wghost,
+                      hipStream_t stream) {
   typedef RowsShape<T, V> RS;
   constexpr int NR = RS::R * RS::NW;
-  auto kern = k_relax_fused_rows<T, V, RS::R, RS::NW>;
+  auto kern = k_relax_fused_rows<T, V, RS::R, RS::NW, ZERO>;
   const int nx = u_in->nx, ny = u_in->ny;
   const int tiles_j = (ny + NR - 5) / (NR - 4);
   const int ichunk = fused_ichunk(nx, tiles_j, device_slots(kern, 64 * RS::NW));
   const int ntiles = tiles_j * ((nx + ichunk - 1) / ichunk);
   const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
   kern<<<nblocks, dim3(64, RS::NW), 0, stream>>>(dgrid<T>(u_in), dgrid<T>(u_out), dgrid<T>(b), c, first_color,
-                                                 tiles_j, ichunk, ntiles);
+                                                 tiles_j, ichunk, ntiles, gw, wghost);
   OCU_HIP_LAUNCHED("ocu_mg_relax_fused");
+}
+
+template <typename T, bool ZERO>
+int launch_fused_tiled(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                       const typename DevCFor<T>::type &c, int first_color, const GhostL1<T> &gw, int wghost,
+                       hipStream_t stream) {
+  const int nx = u_in->nx, ny = u_in->ny, nz = u_in->nz;
+  const int tiles_k = (nz + FOK - 1) / FOK, tiles_j = (ny + FOJ - 1) / FOJ;
+  // enough i-chunks to give every CU a few blocks, but long enough marches
+  const int ichunk = nx >= 256 ? 16 : nx >= 64 ? 8 : 4;
+  const int ntiles = tiles_k * tiles_j * ((nx + ichunk - 1) / ichunk);
+  const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+  k_relax_fused<T, ZERO><<<nblocks, dim3(FBK, FBJ), 0, stream>>>(dgrid<T>(u_in), dgrid<T>(u_out), dgrid<T>(b), c,
+                                                                 first_color, tiles_k, tiles_j, ichunk, ntiles, gw,
+                                                                 wghost);
+  OCU_HIP_LAUNCHED("ocu_mg_relax_fused");
+}
+
+template <typename T, bool ZERO>
+int launch_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                 const typename DevCFor<T>::type &c, int first_color, const GhostL1<T> &gw, int wghost,
+                 hipStream_t stream) {
+  switch (row_vec_width(u_in, u_out, b)) {
+    case 1: return launch_fused_rows<T, 1, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
+    case 2: return launch_fused_rows<T, 2, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
+    case 4: return launch_fused_rows<T, 4, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
+    default: return launch_fused_tiled<T, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
+  }
+}
+
+int relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b, const ocu_mg_level_params *p,
+                const ocu_bc_set *bc, double hx, double hy, double hz, int first_color, int flags, void *stream) {
+  const char *what = "ocu_mg_relax_fused";
+  if (int e = check_pair(u_in, b, what)) return e;
+  if (int e = check_grid(u_out, "ocu_mg_relax_fused(out)")) return e;
+  OCU_REQUIRE(layout_match(u_in, u_out) && u_in->dtype == u_out->dtype && u_in->nx == u_out->nx &&
+                  u_in->ny == u_out->ny && u_in->nz == u_out->nz,
+              "%s: u_out must be congruent with u_in", what);
+  OCU_REQUIRE(u_out->gx >= 1 && u_out->gy >= 1 && u_out->gz >= 1, "%s: u_out needs one ghost layer", what);
+  OCU_REQUIRE((flags & ~(OCU_FUSED_WRITE_GHOSTS | OCU_FUSED_ZERO_INPUT)) == 0, "%s: bad flags %d", what, flags);
+  const bool zero = flags & OCU_FUSED_ZERO_INPUT;
+  const int wghost = (flags & OCU_FUSED_WRITE_GHOSTS) ? 1 : 0;
+  OCU_REQUIRE(zero || u_in->base != u_out->base, "%s: u_out must not alias u_in", what);
+  OCU_REQUIRE(p && bc, "%s: null params", what);
+  OCU_REQUIRE(first_color == 0 || first_color == 1, "%s: bad first_color %d", what, first_color);
+  if (any_periodic(bc)) {
+    set_error("%s: periodic faces need a ghost refresh between colours", what);
+    return OCU_ERR_UNSUPPORTED;
+  }
+  if ((wghost || zero) && !gfaces_local(bc)) {
+    set_error("%s: ghost writes and zero input need Neumann or Dirichlet faces", what);
+    return OCU_ERR_UNSUPPORTED;
+  }
+  if (zero && !gfaces_homogeneous(bc)) {
+    set_error("%s: zero input needs homogeneous faces (value 0)", what);
+    return OCU_ERR_UNSUPPORTED;
+  }
+  if ((int64_t)u_in->nx * u_in->ny * u_in->nz == 0) return OCU_OK;
+  if (u_in->dtype == OCU_F64) {
+    const DevC64 c = dev_c64(p);
+    const GhostL1<double> gw = ghost_l1<double>(bc, hx, hy, hz);
+    return zero ? launch_fused<double, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream))
+                : launch_fused<double, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream));
+  }
+  const DevC32 c = dev_c32(p);
+  const GhostL1<float> gw = ghost_l1<float>(bc, hx, hy, hz);
+  return zero ? launch_fused<float, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream))
+              : launch_fused<float, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream));
+}
+
+int prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, const ocu_bc_set *bc, double hx,
+            double hy, double hz, void *stream) {
+  if (int e = check_grid(u_coarse, "ocu_mg_prolong(coarse)")) return e;
+  if (int e = check_grid(u_fine, "ocu_mg_prolong(fine)")) return e;
+  OCU_REQUIRE(u_coarse->dtype == u_fine->dtype && u_fine->dtype != OCU_I32, "ocu_mg_prolong: dtype mismatch");
+  OCU_REQUIRE(u_fine->nx >= 2 * u_coarse->nx && u_fine->ny >= 2 * u_coarse->ny && u_fine->nz >= 2 * u_coarse->nz,
+              "ocu_mg_prolong: fine grid must be twice the coarse grid");
+  OCU_REQUIRE(u_coarse->gx >= 1 && u_coarse->gy >= 1 && u_coarse->gz >= 1, "ocu_mg_prolong: coarse needs ghosts");
+  const int wghost = bc ? 1 : 0;
+  if (wghost) {
+    // every fine boundary cell must be produced by this launch
+    OCU_REQUIRE(u_fine->nx == 2 * u_coarse->nx && u_fine->ny == 2 * u_coarse->ny && u_fine->nz == 2 * u_coarse->nz,
+                "ocu_mg_prolong_bc: the fine grid must be exactly twice the coarse grid");
+    OCU_REQUIRE(u_fine->gx >= 1 && u_fine->gy >= 1 && u_fine->gz >= 1, "ocu_mg_prolong_bc: fine needs ghosts");
+    if (!gfaces_local(bc)) {
+      set_error("ocu_mg_prolong_bc: ghost writes need Neumann or Dirichlet faces");
+      return OCU_ERR_UNSUPPORTED;
+    }
+  }
+  if ((int64_t)u_coarse->nx * u_coarse->ny * u_coarse->nz == 0) return OCU_OK;
+  static const ocu_bc_set none = {};
+  const ocu_bc_set *fb = bc ? bc : &none;
+  dim3 grid = stencil_grid(u_coarse->nx, u_coarse->ny, u_coarse->nz);
+  if (u_fine->dtype == OCU_F64)
+    k_prolong<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u_coarse), dgrid<double>(u_fine), quirk,
+                                                               ghost_l1<double>(fb, hx, hy, hz), wghost);
+  else
+    k_prolong<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u_coarse), dgrid<float>(u_fine), quirk,
+                                                              ghost_l1<float>(fb, hx, hy, hz), wghost);
+  OCU_HIP_LAUNCHED("ocu_mg_prolong");
 }
 
 }  // namespace
@@ -865,46 +1025,13 @@ int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color, const ocu_
 
 int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                        const ocu_mg_level_params *p, const ocu_bc_set *bc, int first_color, void *stream) {
-  if (int e = check_pair(u_in, b, "ocu_mg_relax_fused")) return e;
-  if (int e = check_grid(u_out, "ocu_mg_relax_fused(out)")) return e;
-  OCU_REQUIRE(layout_match(u_in, u_out) && u_in->dtype == u_out->dtype && u_in->nx == u_out->nx &&
-                  u_in->ny == u_out->ny && u_in->nz == u_out->nz,
-              "ocu_mg_relax_fused: u_out must be congruent with u_in");
-  OCU_REQUIRE(u_in->base != u_out->base, "ocu_mg_relax_fused: u_out must not alias u_in");
-  OCU_REQUIRE(p && bc, "ocu_mg_relax_fused: null params");
-  OCU_REQUIRE(first_color == 0 || first_color == 1, "ocu_mg_relax_fused: bad first_color %d", first_color);
-  if (any_periodic(bc)) {
-    set_error("ocu_mg_relax_fused: periodic faces need a ghost refresh between colours");
-    return OCU_ERR_UNSUPPORTED;
-  }
-  const int nx = u_in->nx, ny = u_in->ny, nz = u_in->nz;
-  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
-  if (int V = row_vec_width(u_in, u_out, b)) {
-    if (u_in->dtype == OCU_F64) {
-      const DevC64 c = dev_c64(p);
-      return V == 1   ? launch_fused_rows<double, 1>(u_in, u_out, b, c, first_color, S(stream))
-             : V == 2 ? launch_fused_rows<double, 2>(u_in, u_out, b, c, first_color, S(stream))
-                      : launch_fused_rows<double, 4>(u_in, u_out, b, c, first_color, S(stream));
-    }
-    const DevC32 c = dev_c32(p);
-    return V == 1   ? launch_fused_rows<float, 1>(u_in, u_out, b, c, first_color, S(stream))
-           : V == 2 ? launch_fused_rows<float, 2>(u_in, u_out, b, c, first_color, S(stream))
-                    : launch_fused_rows<float, 4>(u_in, u_out, b, c, first_color, S(stream));
-  }
-  int tiles_k = (nz + FOK - 1) / FOK, tiles_j = (ny + FOJ - 1) / FOJ;
-  // enough i-chunks to give every CU a few blocks, but long enough marches
-  int ichunk = nx >= 256 ? 16 : nx >= 64 ? 8 : 4;
-  int tiles_i = (nx + ichunk - 1) / ichunk;
-  int ntiles = tiles_k * tiles_j * tiles_i;
-  int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
-  dim3 block(FBK, FBJ);
-  if (u_in->dtype == OCU_F64)
-    k_relax_fused<double><<<nblocks, block, 0, S(stream)>>>(dgrid<double>(u_in), dgrid<double>(u_out), dgrid<double>(b),
-                                                            dev_c64(p), first_color, tiles_k, tiles_j, ichunk, ntiles);
-  else
-    k_relax_fused<float><<<nblocks, block, 0, S(stream)>>>(dgrid<float>(u_in), dgrid<float>(u_out), dgrid<float>(b),
-                                                           dev_c32(p), first_color, tiles_k, tiles_j, ichunk, ntiles);
-  OCU_HIP_LAUNCHED("ocu_mg_relax_fused");
+  return relax_fused(u_in, u_out, b, p, bc, 1.0, 1.0, 1.0, first_color, 0, stream);
+}
+
+int ocu_mg_relax_fused_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                          const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx, double hy, double hz,
+                          int first_color, int flags, void *stream) {
+  return relax_fused(u_in, u_out, b, p, bc, hx, hy, hz, first_color, flags, stream);
 }
 
 int ocu_mg_residual(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r, double h, double fx,
@@ -1001,19 +1128,13 @@ int ocu_norm_finalize(const double *norm_ws, double *out, void *stream) {
 }
 
 int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, void *stream) {
-  if (int e = check_grid(u_coarse, "ocu_mg_prolong(coarse)")) return e;
-  if (int e = check_grid(u_fine, "ocu_mg_prolong(fine)")) return e;
-  OCU_REQUIRE(u_coarse->dtype == u_fine->dtype && u_fine->dtype != OCU_I32, "ocu_mg_prolong: dtype mismatch");
-  OCU_REQUIRE(u_fine->nx >= 2 * u_coarse->nx && u_fine->ny >= 2 * u_coarse->ny && u_fine->nz >= 2 * u_coarse->nz,
-              "ocu_mg_prolong: fine grid must be twice the coarse grid");
-  OCU_REQUIRE(u_coarse->gx >= 1 && u_coarse->gy >= 1 && u_coarse->gz >= 1, "ocu_mg_prolong: coarse needs ghosts");
-  if ((int64_t)u_coarse->nx * u_coarse->ny * u_coarse->nz == 0) return OCU_OK;
-  dim3 grid = stencil_grid(u_coarse->nx, u_coarse->ny, u_coarse->nz);
-  if (u_fine->dtype == OCU_F64)
-    k_prolong<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u_coarse), dgrid<double>(u_fine), quirk);
-  else
-    k_prolong<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u_coarse), dgrid<float>(u_fine), quirk);
-  OCU_HIP_LAUNCHED("ocu_mg_prolong");
+  return prolong(u_coarse, u_fine, quirk, nullptr, 1.0, 1.0, 1.0, stream);
+}
+
+int ocu_mg_prolong_bc(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, const ocu_bc_set *bc,
+                      double hx, double hy, double hz, void *stream) {
+  OCU_REQUIRE(bc, "ocu_mg_prolong_bc: null bc");
+  return prolong(u_coarse, u_fine, quirk, bc, hx, hy, hz, stream);
 }
 
 int ocu_mg_coarse_relax(const ocu_grid3d *u, const ocu_grid3d *b, int iterations, int order,

@@ -61,6 +61,9 @@ class Sol_MultigridPressure3DBase : public Solver {
   virtual void relax(int level, int iterations, RelaxOrder order) = 0;
   virtual void restrict_residuals(int fine_level, int coarse_level, double *l2, double *linf) = 0;
   virtual void prolong(int coarse_level, int fine_level) = 0;
+  // bring every level's storage to the state the reference would hold
+  // (implementations may defer work between the steps above)
+  virtual void settle() {}
 
  public:
   BoundaryConditionSet bc;
@@ -105,6 +108,11 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   int _coarse_mode;
   double *_norm_ws;    // device: NORM_PARTS (sum, max) pairs + 2 results
   double *_norm_host;  // pinned host: 2 results
+  // _u_zero[l]: u at level l is logically the zero grid with its level-1
+  // ghosts (clear_zero + apply_boundary_conditions) but memory was not
+  // written; the next fused relaxation starts from zero without reading u
+  static const int kMaxLevels = 40;
+  bool _u_zero[kMaxLevels];
 
   Grid3DDevice<T> &get_u(int level) { return *_u_grid[level]; }
   Grid3DDevice<T> &get_r(int level) { return *_r_grid[level]; }
@@ -122,6 +130,9 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
 
   void relax_colour(int level, int color);
   void relax_coarsest(int level, int iterations, RelaxOrder order);
+  bool defer_ok(int level) const;
+  void materialize(int level);
+  virtual void settle();
 
  public:
   bool disable_relax_on_host;

@@ -238,10 +238,12 @@ bool Sol_MultigridPressure3DBase::solve(double &residual, double tolerance, int 
     if (!do_fmg(tolerance, max_iter, l2, linf)) {
       printf("[WARNING] Sol_MultigridPressure3DBase::solve - do_fmg failed\n");
       residual = (convergence == CONVERGENCE_L2) ? l2 : linf;
+      settle();
       return false;
     }
   }
   residual = (convergence == CONVERGENCE_L2) ? l2 : linf;
+  settle();
   return true;
 }
 
@@ -254,6 +256,7 @@ bool Sol_MultigridPressure3DBase::run_vcycles(int count) {
   residual_history.clear();
   double l2, linf;
   do_vcycle(0, count, l2, linf);
+  settle();
   return !any_error();
 }
 
@@ -266,6 +269,7 @@ bool Sol_MultigridPressure3DBase::run_fmg(int count) {
   residual_history.clear();
   double l2, linf;
   do_fmg(0, count, l2, linf);
+  settle();
   return !any_error();
 }
 
@@ -284,6 +288,7 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   disable_relax_on_host = false;
   use_fused_relax = true;
   store_residuals = false;
+  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = false;
 }
 
 template <typename T>
@@ -337,13 +342,43 @@ void Sol_MultigridPressure3DDevice<T>::level_params(int level, void *out) const 
   p.mods[5] = bc_diag_mod(lbc.zneg, _fz);
 }
 
+// A cleared level may stay unwritten when its next use is the fused
+// relaxation, which can start from the zero grid without reading it.
+template <typename T>
+bool Sol_MultigridPressure3DDevice<T>::defer_ok(int level) const {
+  if (!(use_fused_relax && _fused_ok && level < _num_levels - 1 && nu1 >= 2)) return false;
+  // the zero grid's ghosts are all zero only for homogeneous faces
+  BoundaryConditionSet lbc = get_bc_at_level(level);
+  return lbc.xpos.value == 0 && lbc.xneg.value == 0 && lbc.ypos.value == 0 && lbc.yneg.value == 0 &&
+         lbc.zpos.value == 0 && lbc.zneg.value == 0;
+}
+
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::materialize(int level) {
+  if (!_u_zero[level]) return;
+  _u_zero[level] = false;
+  check_ok(get_u(level).clear_zero());
+  apply_boundary_conditions(level);
+}
+
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::settle() {
+  for (int l = 0; l < _num_levels; l++) materialize(l);
+}
+
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::clear_zero(int level) {
+  if (defer_ok(level)) {
+    _u_zero[level] = true;
+    return;
+  }
+  _u_zero[level] = false;
   check_ok(get_u(level).clear_zero());
 }
 
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::apply_boundary_conditions(int level) {
+  if (_u_zero[level]) return;  // the deferred zero grid carries its ghosts
   if (!apply_3d_boundary_conditions_level1(get_u(level), get_bc_at_level(level), hx(level), hy(level), hz(level))) {
     printf("[ERROR] Sol_MultigridPressure3DDevice::apply_boundary_conditions - failed at level %d \n", level);
     add_error();
@@ -352,6 +387,7 @@ void Sol_MultigridPressure3DDevice<T>::apply_boundary_conditions(int level) {
 
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::relax_colour(int level, int color) {
+  materialize(level);
   ocu_mg_level_params p;
   level_params(level, &p);
   ocu_grid3d u = get_u(level).desc(), b = get_b(level).desc();
@@ -364,6 +400,7 @@ void Sol_MultigridPressure3DDevice<T>::relax_colour(int level, int color) {
 // at the end
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::relax_coarsest(int level, int iterations, RelaxOrder order) {
+  materialize(level);
   ocu_mg_level_params p;
   level_params(level, &p);
   ocu_grid3d u = get_u(level).desc(), b = get_b(level).desc();
@@ -391,7 +428,10 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
 
   if (use_fused_relax && _fused_ok) {
     // pairs of out-of-place sweeps u -> alt -> u, so the result always ends
-    // in u; an odd last iteration runs in place, one launch per colour
+    // in u; each sweep also writes the level-1 ghosts of its output (the
+    // fill the reference runs after every iteration), and the first one
+    // starts from the zero grid when u is a deferred clear.  An odd last
+    // iteration runs in place, one launch per colour.
     ocu_mg_level_params p;
     level_params(level, &p);
     ocu_bc_set cbc = to_c_bc(get_bc_at_level(level));
@@ -399,12 +439,17 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
     Grid3DDevice<T> &u = get_u(level);
     Grid3DDevice<T> &alt = *_u_alt[level];
     ocu_grid3d du = u.desc(), da = alt.desc(), db = get_b(level).desc();
+    const double lhx = hx(level), lhy = hy(level), lhz = hz(level);
     int it = 0;
     for (; it + 1 < iterations; it += 2) {
-      ok_status(ocu_mg_relax_fused(&du, &da, &db, &p, &cbc, first_colour(it), s), "Sol_MultigridPressure3DDevice::relax");
-      if (!apply_3d_boundary_conditions_level1(alt, get_bc_at_level(level), hx(level), hy(level), hz(level))) add_error();
-      ok_status(ocu_mg_relax_fused(&da, &du, &db, &p, &cbc, first_colour(it + 1), s), "Sol_MultigridPressure3DDevice::relax");
-      apply_boundary_conditions(level);
+      const int zero = _u_zero[level] ? OCU_FUSED_ZERO_INPUT : 0;
+      _u_zero[level] = false;
+      ok_status(ocu_mg_relax_fused_bc(&du, &da, &db, &p, &cbc, lhx, lhy, lhz, first_colour(it),
+                                      OCU_FUSED_WRITE_GHOSTS | zero, s),
+                "Sol_MultigridPressure3DDevice::relax");
+      ok_status(ocu_mg_relax_fused_bc(&da, &du, &db, &p, &cbc, lhx, lhy, lhz, first_colour(it + 1),
+                                      OCU_FUSED_WRITE_GHOSTS, s),
+                "Sol_MultigridPressure3DDevice::relax");
     }
     for (; it < iterations; it++) {
       int c0 = first_colour(it);
@@ -413,6 +458,7 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
       apply_boundary_conditions(level);
     }
   } else {
+    materialize(level);
     for (int it = 0; it < iterations; it++) {
       for (int rb = 0; rb < 2; rb++) {
         int color = (first_colour(it) == 1) ? !rb : rb;
@@ -430,6 +476,7 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::restrict_residuals(int fine_level, int coarse_level, double *l2,
                                                           double *linf) {
+  materialize(fine_level);
   void *s = ThreadManager::get_compute_stream();
   ocu_grid3d u = get_u(fine_level).desc(), b = get_b(fine_level).desc(), r = get_r(fine_level).desc();
   bool norms = l2 || linf;
@@ -459,9 +506,19 @@ void Sol_MultigridPressure3DDevice<T>::restrict_residuals(int fine_level, int co
 
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::prolong(int coarse_level, int fine_level) {
+  materialize(coarse_level);
+  materialize(fine_level);
   ocu_grid3d uc = get_u(coarse_level).desc(), uf = get_u(fine_level).desc();
-  ok_status(ocu_mg_prolong(&uc, &uf, 1, ThreadManager::get_compute_stream()), "Sol_MultigridPressure3DDevice::prolong");
-  apply_boundary_conditions(fine_level);
+  void *s = ThreadManager::get_compute_stream();
+  if (_fused_ok && uf.nx == 2 * uc.nx && uf.ny == 2 * uc.ny && uf.nz == 2 * uc.nz) {
+    // prolongation and the level-1 fill of the fine grid in one launch
+    ocu_bc_set cbc = to_c_bc(get_bc_at_level(fine_level));
+    ok_status(ocu_mg_prolong_bc(&uc, &uf, 1, &cbc, hx(fine_level), hy(fine_level), hz(fine_level), s),
+              "Sol_MultigridPressure3DDevice::prolong");
+  } else {
+    ok_status(ocu_mg_prolong(&uc, &uf, 1, s), "Sol_MultigridPressure3DDevice::prolong");
+    apply_boundary_conditions(fine_level);
+  }
   if (any_error())
     printf("[ERROR] Sol_MultigridPressure3DDevice::prolong - failed at level %d -> %d\n", coarse_level, fine_level);
 }
@@ -475,11 +532,12 @@ bool Sol_MultigridPressure3DDevice<T>::initialize_storage(int nx_val, int ny_val
     printf("[ERROR] Sol_MultigridPressure3DDevice::initialize_storage - error in grid calculation\n");
     return false;
   }
-  if (_num_levels < 1) {
+  if (_num_levels < 1 || _num_levels > kMaxLevels) {
     printf("[ERROR] Sol_MultigridPressure3DDevice::initialize_storage - (%d,%d,%d) admits no multigrid level\n",
            nx_val, ny_val, nz_val);
     return false;
   }
+  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = false;
   _update_bc_between_colors = bc.xpos.check_type(BC_PERIODIC) || bc.xneg.check_type(BC_PERIODIC) ||
                               bc.ypos.check_type(BC_PERIODIC) || bc.yneg.check_type(BC_PERIODIC) ||
                               bc.zpos.check_type(BC_PERIODIC) || bc.zneg.check_type(BC_PERIODIC);

@@ -192,6 +192,7 @@ bool Sol_MultigridPressureMixed3DDeviceD::do_fmg(double tolerance, int max_iter,
     check_ok(apply_3d_boundary_conditions_level1(_x, bc0, hx(0), hy(0), hz(0)));
     if (_mg_f.any_error()) add_error();
   }
+  _mg_f.settle();
   if (_singular) {
     double xm = 0;
     check_ok(interior_mean(_x, xm));

@@ -129,6 +129,12 @@ def mg_relax_fused(u_in, u_out, b, params, bc, first_color=0):
                                                     first_color, None), "mg_relax_fused")
 
 
+def mg_relax_fused_bc(u_in, u_out, b, params, bc, hx, hy, hz, first_color=0, flags=0):
+    return _lib.check(_lib.hip().ocu_mg_relax_fused_bc(u_in.ptr(), u_out.ptr(), b.ptr(), C.byref(params),
+                                                       C.byref(bc), hx, hy, hz, first_color, flags, None),
+                      "mg_relax_fused_bc")
+
+
 def mg_residual(u, b, r, h, fx, fy, fz):
     _lib.check(_lib.hip().ocu_mg_residual(u.ptr(), b.ptr(), r.ptr(), h, fx, fy, fz, None), "mg_residual")
 
@@ -155,6 +161,11 @@ def mg_residual_restrict(u, b, r, bcoarse, h, fx, fy, fz, with_norm=False):
 
 def mg_prolong(uc, uf, quirk=1):
     _lib.check(_lib.hip().ocu_mg_prolong(uc.ptr(), uf.ptr(), quirk, None), "mg_prolong")
+
+
+def mg_prolong_bc(uc, uf, bc, hx, hy, hz, quirk=1):
+    _lib.check(_lib.hip().ocu_mg_prolong_bc(uc.ptr(), uf.ptr(), quirk, C.byref(bc), hx, hy, hz, None),
+               "mg_prolong_bc")
 
 
 def mg_coarse_relax(u, b, iterations, order, params, bc, hx, hy, hz, mode=0):

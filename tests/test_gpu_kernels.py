@@ -177,6 +177,117 @@ def test_relax_fused_matches_two_colours(gpu, dtype, size, case, first):
     assert_bitexact(gout.to_host(), want, f"fused {case} {size}")
 
 
+FUSED_SIZES = [(16, 16, 16), (9, 20, 13), (40, 30, 70), (12, 16, 200), (8, 26, 256), (6, 14, 130), (5, 13, 300),
+               (1, 3, 2)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", FUSED_SIZES)
+@pytest.mark.parametrize("case", ["neumann0", "dirichlet", "neumann_v"])
+@pytest.mark.parametrize("flags", [T.FUSED_WRITE_GHOSTS, T.FUSED_ZERO_INPUT, T.FUSED_ZERO_INPUT | T.FUSED_WRITE_GHOSTS])
+def test_relax_fused_bc(gpu, dtype, size, case, flags):
+    """One fused iteration that also writes the level-1 ghosts of its output
+    and/or starts from the cleared, ghost-filled grid without reading it:
+    equal to clear_zero + level-1 fill + two colours + level-1 fill."""
+    bc = BC_CASES[case]
+    hx, hy, hz = 1 / 32, 1 / 20, 1 / 16
+    zero = bool(flags & T.FUSED_ZERO_INPUT)
+    if zero:  # the zero-input mode is for homogeneous faces (coarse levels)
+        bc = homogeneous(bc)
+        u = HostGrid(*size, dtype=dtype)
+        O.lib().orc_bc_fill(0, u.ptr(), C.byref(bc), hx, hy, hz)
+    else:
+        u = filled(*size, dtype, 43, bc, 1 / 32)
+    b = random_grid(*size, dtype, 44)
+    p = O.level_params(bc, 1, hx, hy, hz)
+    for first in (0, 1):
+        want = u.copy()
+        O.lib().orc_mg_relax(want.ptr(), b.ptr(), first, C.byref(p), T.RELAX_DEVICE)
+        O.lib().orc_mg_relax(want.ptr(), b.ptr(), 1 - first, C.byref(p), T.RELAX_DEVICE)
+        if flags & T.FUSED_WRITE_GHOSTS:
+            O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), hx, hy, hz)
+        # zero input is not read: hand the kernel garbage there
+        gin = D().DeviceGrid.from_host(random_grid(*size, dtype, 45) if zero else u)
+        gb = D().DeviceGrid.from_host(b)
+        start = u.copy()
+        if flags & T.FUSED_WRITE_GHOSTS:
+            start.buf[:] = (2 * uniform(46, start.buf.shape) - 1).astype(dtype)  # ghosts must all be written
+            start.interior()[:] = u.interior()
+        gout = D().DeviceGrid.from_host(start)
+        D().mg_relax_fused_bc(gin, gout, gb, p, bc, hx, hy, hz, first, flags)
+        got = gout.to_host()
+        if not (flags & T.FUSED_WRITE_GHOSTS):
+            assert np.array_equal(got.interior(), want.interior()), f"fused {case} {size} flags={flags}"
+        else:
+            assert_ghost_layer_bitexact(got, want, f"fused+ghosts {case} {size} flags={flags} first={first}")
+        if zero:  # u_in may be the output itself
+            galias = D().DeviceGrid.from_host(start)
+            D().mg_relax_fused_bc(galias, galias, gb, p, bc, hx, hy, hz, first, flags)
+            a = galias.to_host()
+            assert np.array_equal(a.buf.view(np.uint8), got.buf.view(np.uint8)), "aliased zero-input call"
+
+
+def homogeneous(bc):
+    h = BCSet()
+    for f in BCSet.FACES:
+        b = getattr(bc, f)
+        setattr(h, f, BC(b.type, 0.0, b.aux_value))
+    return h
+
+
+def test_zero_input_rejects_inhomogeneous_faces(gpu):
+    from opencurrent_amd._lib import OcuError
+    bc = BC_CASES["dirichlet"]
+    u = D().DeviceGrid(8, 8, 8, dtype=np.float32)
+    o = D().DeviceGrid(8, 8, 8, dtype=np.float32)
+    p = O.level_params(bc, 0, .1, .1, .1)
+    with pytest.raises(OcuError, match="homogeneous"):
+        D().mg_relax_fused_bc(u, o, u, p, bc, .1, .1, .1, 0, T.FUSED_ZERO_INPUT)
+
+
+def assert_ghost_layer_bitexact(got, want, what):
+    """interior and the first ghost layer (what the level-1 fill defines)"""
+    d = got.desc
+    box = (slice(d.gx - 1, d.gx + d.nx + 1), slice(d.gy - 1, d.gy + d.ny + 1), slice(d.gz - 1, d.gz + d.nz + 1))
+    a, b = got.full()[box], want.full()[box]
+    ia = a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+    ib = b.view(np.uint64 if b.dtype == np.float64 else np.uint32)
+    bad = np.argwhere(ia != ib)
+    if bad.size:
+        i, j, k = bad[0]
+        raise AssertionError(f"{what}: {len(bad)} cells differ, first at ghost-box index {(i, j, k)}: "
+                             f"got {a[i, j, k]!r} want {b[i, j, k]!r}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", ["neumann0", "dirichlet", "neumann_v"])
+@pytest.mark.parametrize("cn", [(8, 6, 10), (1, 2, 3), (16, 16, 32)])
+def test_prolong_bc(gpu, dtype, case, cn):
+    bc = BC_CASES[case]
+    hx, hy, hz = .25, .125, .5
+    uc = filled(*cn, dtype, 63, bc, .1)
+    uf = random_grid(*(2 * s for s in cn), dtype, 64)
+    want = uf.copy()
+    O.lib().orc_mg_prolong(uc.ptr(), want.ptr(), 1)
+    O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), hx, hy, hz)
+    guc, guf = D().DeviceGrid.from_host(uc), D().DeviceGrid.from_host(uf)
+    D().mg_prolong_bc(guc, guf, bc, hx, hy, hz)
+    assert_ghost_layer_bitexact(guf.to_host(), want, f"prolong+ghosts {case} {cn}")
+
+
+def test_fused_bc_rejects_nonlocal_faces(gpu):
+    from opencurrent_amd._lib import OcuError
+    bc = BC_CASES["none_faces"]
+    u = D().DeviceGrid(8, 8, 8, dtype=np.float32)
+    o = D().DeviceGrid(8, 8, 8, dtype=np.float32)
+    p = O.level_params(BC_CASES["neumann0"], 0, .1, .1, .1)
+    bc2 = bcset(T.BC_NONE, T.BC_NEUMANN, T.BC_NEUMANN, T.BC_NEUMANN, T.BC_NEUMANN, T.BC_NEUMANN)
+    with pytest.raises(OcuError):
+        D().mg_relax_fused_bc(u, o, u, p, bc2, .1, .1, .1, 0, T.FUSED_WRITE_GHOSTS)
+    with pytest.raises(OcuError):
+        D().mg_prolong_bc(D().DeviceGrid(4, 4, 4, dtype=np.float32), u, bc, .1, .1, .1)
+
+
 def test_relax_fused_rejects_periodic(gpu):
     from opencurrent_amd._lib import OcuError
     bc = BC_CASES["mixed"]
