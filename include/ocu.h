@@ -27,12 +27,19 @@ extern "C" {
 int ocu_set_compute_stream(void *stream);
 void *ocu_get_compute_stream(void);
 
+/* options of the multigrid solvers (OR-ed; OCU_MG_DEFAULT = all):
+ *   OCU_MG_FUSED_RELAX: one launch per red+black iteration, ghosts written by
+ *     the sweep, deferred zero clears (Neumann/Dirichlet faces);
+ *   OCU_MG_LDS_BOTTOM: the coarse levels of each V-cycle as one launch of a
+ *     single LDS-resident workgroup.
+ * Every option gives results bit-identical to the plain per-level path. */
+enum ocu_mg_options { OCU_MG_FUSED_RELAX = 1, OCU_MG_LDS_BOTTOM = 2, OCU_MG_DEFAULT = 3 };
+
 /* ---- Sol_MultigridPressure3DDevice<T> (ocuequation/sol_mgpressure3d.h:36-181) ----
  * T follows rhs->dtype.  initialize_storage(nx,ny,nz,hx,hy,hz,&rhs) with the
- * public fields taken from cfg.  use_fused_relax selects the one-launch
- * red+black sweep where the boundary conditions allow it. */
+ * public fields taken from cfg. */
 void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
-                    int use_fused_relax);
+                    int options);
 int ocu_mg_pressure(void *h, ocu_grid3d *out);
 /* solve(residual, tol, max_iter) (sol_mgpressure3d.cpp:332-370) */
 int ocu_mg_solve(void *h, double tol, int max_iter, double *residual);
@@ -48,7 +55,7 @@ void ocu_mg_destroy(void *h);
  * cycles_first are V-cycles per refinement iteration; fmg_first starts the
  * first iteration with an FP32 FMG. */
 void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
-                         int cycles, int cycles_first, int fmg_first);
+                         int cycles, int cycles_first, int fmg_first, int options);
 int ocu_mgmixed_pressure(void *h, ocu_grid3d *out);
 /* tol is absolute in cfg->convergence's norm; max_iter bounds the
  * refinement iterations */

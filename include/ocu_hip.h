@@ -158,6 +158,23 @@ int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int qui
  * and every face Neumann or Dirichlet. */
 int ocu_mg_prolong_bc(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk,
                       const ocu_bc_set *bc, double hx, double hy, double hz, void *stream);
+/* The bottom of a V-cycle in one workgroup: levels top..top+nlev-1 (the
+ * last the coarsest) of Sol_MultigridPressure3DDevice, held in LDS.  Does
+ * what the per-level calls do on them (sol_mgpressure3d.cpp:184-239):
+ * relax(nu1) with the OCU_RELAX_DEVICE arithmetic and a level-1 fill after
+ * every iteration, a zero start on the next level, residual + restriction,
+ * the coarsest relaxation (ocu_mg_coarse_relax semantics, coarse_mode 0/1,
+ * coarse_iters red-black iterations), prolongation with the level-1 fill and
+ * relax(nu2); every level's u (with ghosts) and b (below the top) is written
+ * back.  Arrays u[l], b[l], p[l] and h[3l..3l+2] (hx,hy,hz) describe level
+ * top+l; zero_top: the top level's u is the zero grid (not read; needs
+ * homogeneous faces).  Faces must be Neumann or Dirichlet; the boxes must
+ * fit ocu_mg_bottom_lds_bytes() <= 65536. */
+int ocu_mg_bottom_vcycle(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_params *p,
+                         const double *h, int nlev, const ocu_bc_set *bc, int nu1, int nu2,
+                         int coarse_iters, int coarse_mode, int zero_top, void *stream);
+/* LDS bytes a bottom launch over these levels needs (0: invalid request) */
+size_t ocu_mg_bottom_lds_bytes(const ocu_grid3d *u, int nlev, int coarse_mode);
 /* Coarsest-level relaxation in one workgroup (replaces relax_on_host,
  * sol_mgpressure3ddev.cu:134-188, and its two PCIe round trips).
  * mode 0: reference semantics - `iterations` RB-GS iterations with the

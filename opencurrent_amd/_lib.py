@@ -79,6 +79,8 @@ HIP_SIGS = {
     "ocu_norm_finalize": (I, [DP, DP, P]),
     "ocu_mg_prolong": (I, [GP, GP, I, P]),
     "ocu_mg_prolong_bc": (I, [GP, GP, I, BP, D, D, D, P]),
+    "ocu_mg_bottom_vcycle": (I, [GP, GP, MP, DP, I, BP, I, I, I, I, I, P]),
+    "ocu_mg_bottom_lds_bytes": (C.c_size_t, [GP, I, I]),
     "ocu_mg_coarse_relax": (I, [GP, GP, I, I, MP, BP, D, D, D, I, P]),
     "ocu_ir_residual": (I, [GP, GP, D, GP, D, D, D, D, D, DP, P]),
     "ocu_norm3_finalize": (I, [DP, DP, P]),
@@ -119,7 +121,7 @@ HOST_SIGS = {
     "ocu_mg_run_fmg": (I, [P, I]),
     "ocu_mg_stats": (I, [P, IP, IP, DP, I, IP]),
     "ocu_mg_destroy": (None, [P]),
-    "ocu_mgmixed_create": (P, [GP, D, D, D, CP, I, I, I]),
+    "ocu_mgmixed_create": (P, [GP, D, D, D, CP, I, I, I, I]),
     "ocu_mgmixed_pressure": (I, [P, GP]),
     "ocu_mgmixed_solve": (I, [P, D, I, DP]),
     "ocu_mgmixed_stats": (I, [P, IP, IP, DP, DP, I, IP]),

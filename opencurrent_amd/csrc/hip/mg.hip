@@ -5,6 +5,8 @@
 // lanes always walks 64 consecutive k of one (i,j) row.  All arithmetic
 // keeps the reference's operation order; the library is built with
 // -ffp-contract=off so the results match the CPU restatement bit for bit.
+#include <cstring>
+
 #include "ghost_l1.h"
 #include "ocu_hip_common.h"
 
@@ -64,6 +66,9 @@ HostC<T> host_c(const ocu_mg_level_params *p) {
 template <typename C> struct DevCFor;
 template <> struct DevCFor<double> { typedef DevC64 type; };
 template <> struct DevCFor<float> { typedef DevC32 type; };
+template <typename T> typename DevCFor<T>::type dev_c(const ocu_mg_level_params *p);
+template <> DevC64 dev_c<double>(const ocu_mg_level_params *p) { return dev_c64(p); }
+template <> DevC32 dev_c<float>(const ocu_mg_level_params *p) { return dev_c32(p); }
 
 // diagonal with boundary modifications, in the order of the diag table
 // (sol_mgpressure3ddev.cu:27-43) followed by the z terms (devd.cu:585-587)
@@ -680,10 +685,17 @@ struct CoarseGeom {
   int lshift;     // LDS offset of cell (0,0,0)
 };
 
-template <typename T>
-__device__ void lds_relax_color(T *u, const T *b, const CoarseGeom &g, const HostC<T> &c, int color) {
+// The threads that share an LDS phase (the whole block) and their barrier.
+struct BlockTeam {
+  __device__ int rank() const { return threadIdx.x; }
+  __device__ int size() const { return blockDim.x; }
+  __device__ void sync() const { __syncthreads(); }
+};
+
+template <typename T, typename Team>
+__device__ void lds_relax_color(const Team &tm, T *u, const T *b, const CoarseGeom &g, const HostC<T> &c, int color) {
   int n = g.nx * g.ny * g.nz;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+  for (int t = tm.rank(); t < n; t += tm.size()) {
     int k = t % g.nz, j = (t / g.nz) % g.ny, i = t / (g.nz * g.ny);
     if (((i + j + k) & 1) != color) continue;
     int id = g.lshift + i * g.lxs + j * g.lys + k;
@@ -725,12 +737,12 @@ __device__ __forceinline__ int srci(const FaceS &f, bool neg, int n) {
   return neg ? 0 : n - 1;
 }
 
-template <typename T>
-__device__ void lds_fill_nocorners(T *u, const CoarseGeom &g, const FacesS &F, T hx, T hy, T hz) {
+template <typename T, typename Team>
+__device__ void lds_fill_nocorners(const Team &tm, T *u, const CoarseGeom &g, const FacesS &F, T hx, T hy, T hz) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   auto A = [&](int i, int j, int k) -> T & { return u[g.lshift + i * g.lxs + j * g.lys + k]; };
   int nZ = nx * ny, nY = nx * nz, nX = ny * nz;
-  for (int t = threadIdx.x; t < nZ + nY + nX; t += blockDim.x) {
+  for (int t = tm.rank(); t < nZ + nY + nX; t += tm.size()) {
     if (t < nZ) {
       int i = t / ny, j = t % ny;
       if (wr(F.f[5])) A(i, j, -1) = xf_host(F.f[5], true, hz, A(i, j, srci(F.f[5], true, nz)));
@@ -748,28 +760,28 @@ __device__ void lds_fill_nocorners(T *u, const CoarseGeom &g, const FacesS &F, T
 }
 
 // the level-1 fill, pass by pass (block barriers between passes)
-template <typename T>
-__device__ void lds_fill_level1(T *u, const CoarseGeom &g, const FacesS &F, T hx, T hy, T hz) {
+template <typename T, typename Team>
+__device__ void lds_fill_level1(const Team &tm, T *u, const CoarseGeom &g, const FacesS &F, T hx, T hy, T hz) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   auto A = [&](int i, int j, int k) -> T & { return u[g.lshift + i * g.lxs + j * g.lys + k]; };
-  for (int t = threadIdx.x; t < ny * nz; t += blockDim.x) {
+  for (int t = tm.rank(); t < ny * nz; t += tm.size()) {
     int j = t / nz, k = t % nz;
     if (wr(F.f[1])) A(-1, j, k) = xf_l1(F.f[1], true, hx, A(srci(F.f[1], true, nx), j, k));
     if (wr(F.f[0])) A(nx, j, k) = xf_l1(F.f[0], false, hx, A(srci(F.f[0], false, nx), j, k));
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < (nx + 2) * nz; t += blockDim.x) {
+  tm.sync();
+  for (int t = tm.rank(); t < (nx + 2) * nz; t += tm.size()) {
     int i = t / nz - 1, k = t % nz;
     if (wr(F.f[3])) A(i, -1, k) = xf_l1(F.f[3], true, hy, A(i, srci(F.f[3], true, ny), k));
     if (wr(F.f[2])) A(i, ny, k) = xf_l1(F.f[2], false, hy, A(i, srci(F.f[2], false, ny), k));
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) {
+  tm.sync();
+  for (int t = tm.rank(); t < (nx + 2) * (ny + 2); t += tm.size()) {
     int i = t / (ny + 2) - 1, j = t % (ny + 2) - 1;
     if (wr(F.f[5])) A(i, j, -1) = xf_l1(F.f[5], true, hz, A(i, j, srci(F.f[5], true, nz)));
     if (wr(F.f[4])) A(i, j, nz) = xf_l1(F.f[4], false, hz, A(i, j, srci(F.f[4], false, nz)));
   }
-  __syncthreads();
+  tm.sync();
 }
 
 // colour sequence of relax() / relax_on_host (sol_mgpressure3ddev.cu:160-171)
@@ -779,34 +791,330 @@ __device__ __forceinline__ int color_of(int order, int it, int iterations, int r
 
 // TS = grid storage type, TC = compute type.  MODE 0: reference semantics;
 // MODE 1: double arithmetic on float storage with level-1 fills.
+// relax_on_host semantics on LDS boxes u, b (compute type TC)
+template <typename TC, int MODE, typename Team>
+__device__ void coarse_iterations(const Team &tm, TC *u, TC *b, const CoarseGeom &g, const HostC<TC> &c,
+                                  const FacesS &F, TC hx, TC hy, TC hz, int iterations, int order) {
+  for (int it = 0; it < iterations; it++)
+    for (int rb = 0; rb < 2; rb++) {
+      lds_relax_color(tm, u, b, g, c, color_of(order, it, iterations, rb));
+      tm.sync();
+      if (MODE == 0) {
+        lds_fill_nocorners(tm, u, g, F, hx, hy, hz);
+        tm.sync();
+      } else {
+        lds_fill_level1(tm, u, g, F, hx, hy, hz);
+      }
+    }
+  if (MODE == 0) lds_fill_level1(tm, u, g, F, hx, hy, hz);
+}
+
+// TS = grid storage type, TC = compute type.  MODE 0: reference semantics;
+// MODE 1: double arithmetic on float storage with level-1 fills.
 template <typename TS, typename TC, int MODE>
 __global__ __launch_bounds__(256) void k_coarse(DGrid<TS> ug, DGrid<TS> bg, CoarseGeom g, HostC<TC> c, FacesS F,
                                                 TC hx, TC hy, TC hz, int iterations, int order) {
   extern __shared__ unsigned char smem[];
+  const BlockTeam tm;
   const int box = (g.nx + 2) * (g.ny + 2) * (g.nz + 2);
   TC *u = (TC *)smem;
   TC *b = u + box;
-  for (int t = threadIdx.x; t < box; t += blockDim.x) {
+  for (int t = tm.rank(); t < box; t += tm.size()) {
     int k = t % (g.nz + 2) - 1, j = (t / (g.nz + 2)) % (g.ny + 2) - 1, i = t / ((g.nz + 2) * (g.ny + 2)) - 1;
     u[t] = (TC)ug.at(i, j, k);
     b[t] = (TC)bg.at(i, j, k);
   }
-  __syncthreads();
-  for (int it = 0; it < iterations; it++)
-    for (int rb = 0; rb < 2; rb++) {
-      lds_relax_color<TC>(u, b, g, c, color_of(order, it, iterations, rb));
-      __syncthreads();
-      if (MODE == 0) {
-        lds_fill_nocorners<TC>(u, g, F, hx, hy, hz);
-        __syncthreads();
-      } else {
-        lds_fill_level1<TC>(u, g, F, hx, hy, hz);
-      }
-    }
-  if (MODE == 0) lds_fill_level1<TC>(u, g, F, hx, hy, hz);
-  for (int t = threadIdx.x; t < box; t += blockDim.x) {
+  tm.sync();
+  coarse_iterations<TC, MODE>(tm, u, b, g, c, F, hx, hy, hz, iterations, order);
+  for (int t = tm.rank(); t < box; t += tm.size()) {
     int k = t % (g.nz + 2) - 1, j = (t / (g.nz + 2)) % (g.ny + 2) - 1, i = t / ((g.nz + 2) * (g.ny + 2)) - 1;
     ug.at(i, j, k) = (TS)u[t];
+  }
+}
+
+// ------------------------------------------------------------------------
+// the bottom of a V-cycle in one workgroup (LDS resident)
+// ------------------------------------------------------------------------
+//
+// Levels top..C (C the coarsest) are small enough to live in LDS together;
+// one launch performs the V-cycle's part on them exactly as the per-level
+// launches do - relax(nu1) with the device arithmetic, a zero start on the
+// next level, fused residual and restriction, the coarsest relaxation
+// (ocu_mg_coarse_relax semantics), prolongation with the level-1 fill,
+// relax(nu2) - and writes every level back.
+//
+// Ghosts: the 7-point stencil reads a face ghost only from the cell it
+// derives from, so a colour pass writes each updated cell's face ghosts
+// right away (no fill phase between colours or iterations); the edges and
+// corners, which no relaxation reads, are filled once at the end of each
+// relax call.  With Neumann/Dirichlet faces every variant of the reference's
+// fills (level-1, device and host no-corner) gives these face values, so the
+// final state is the reference's.
+
+constexpr int BOT_MAX = 6;  // levels in one bottom launch
+constexpr int BOT_THREADS = 1024;
+
+// x / n for 0 <= x < 2^16 and 1 <= n <= 2^15 by one multiply: with
+// m = ceil(2^31 / n) the error of x*m/2^31 is below x/2^31 <= 1/n
+struct Mag {
+  uint32_t n, m;
+};
+inline Mag mag(uint32_t n) { return {n, (uint32_t)(((1ull << 31) + n - 1) / n)}; }
+__device__ __forceinline__ uint32_t mdiv(uint32_t x, Mag d) { return (uint32_t)(((uint64_t)x * d.m) >> 31); }
+
+struct BotGeom {
+  int nx, ny, nz;
+  int lys, lxs, lshift;  // LDS box strides and the offset of cell (0,0,0)
+  Mag my, mz, mpy, mpz;  // ny, nz, ny+2, nz+2
+  Mag mhd, mhh, mn;      // kupd/2 (device colour slots per row), ceil(nz/2), nx*ny*nz
+};
+
+template <typename T>
+struct BotLevel {
+  DGrid<T> u, b;  // global grids of the level
+  BotGeom g;
+  int off;  // LDS byte offset of this level's u box (its b box follows)
+  typename DevCFor<T>::type dc;
+  ResC<T> rc;
+  GhostL1<T> gw;
+};
+
+template <typename T, typename TC>
+struct BotArgs {
+  BotLevel<T> L[BOT_MAX];
+  int nlev, nu1, nu2, zero_top;
+  HostC<TC> hc;     // coarsest level: relax_on_host arithmetic in TC
+  GhostL1<TC> gwc;  // and its ghost transforms in TC
+  int coarse_iters, coarse_off;  // coarse_off: LDS bytes of the TC boxes (mode 1)
+};
+
+static_assert(sizeof(BotArgs<double, double>) <= 4096 && sizeof(BotArgs<float, double>) <= 4096,
+              "bottom launch arguments exceed the kernel argument segment");
+
+__device__ __forceinline__ int box_cells(const BotGeom &g) { return (g.nx + 2) * (g.ny + 2) * (g.nz + 2); }
+
+// face ghosts of the cell at LDS index id = (i,j,k) with value v
+template <typename T>
+__device__ __forceinline__ void face_ghosts(T *u, const BotGeom &g, const GhostL1<T> &gw, int id, int i, int j,
+                                            int k, T v) {
+  if (i == 0) u[id - g.lxs] = gw.xf(1, v);
+  if (i == g.nx - 1) u[id + g.lxs] = gw.xf(0, v);
+  if (j == 0) u[id - g.lys] = gw.xf(3, v);
+  if (j == g.ny - 1) u[id + g.lys] = gw.xf(2, v);
+  if (k == 0) u[id - 1] = gw.xf(5, v);
+  if (k == g.nz - 1) u[id + 1] = gw.xf(4, v);
+}
+
+// one colour with the device arithmetic (k < 2*(nz/2)), threads over the
+// colour's cells only
+template <typename T>
+__device__ void bot_relax_dev(T *u, const T *b, const BotGeom &g, const typename DevCFor<T>::type &c,
+                              const GhostL1<T> &gw, int color) {
+  const int h = (int)g.mhd.n, n = g.nx * g.ny * h;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int row = mdiv(t, g.mhd), i = mdiv(row, g.my), j = row - i * g.ny;
+    const int k = 2 * (t - row * h) + ((i + j + color) & 1);
+    const int id = g.lshift + i * g.lxs + j * g.lys + k;
+    const T d = diag_dev<T>(c, i, j, k, g.nx, g.ny, g.nz);
+    const T v = relax_pt(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs], u[id + g.lxs],
+                         b[id], d);
+    u[id] = v;
+    face_ghosts(u, g, gw, id, i, j, k, v);
+  }
+}
+
+// one colour with the relax_on_host arithmetic (every k)
+template <typename T>
+__device__ void bot_relax_host(T *u, const T *b, const BotGeom &g, const HostC<T> &c, const GhostL1<T> &gw,
+                               int color) {
+  const int h = (int)g.mhh.n, n = g.nx * g.ny * h;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int row = mdiv(t, g.mhh), i = mdiv(row, g.my), j = row - i * g.ny;
+    const int k = 2 * (t - row * h) + ((i + j + color) & 1);
+    if (k >= g.nz) continue;
+    const int id = g.lshift + i * g.lxs + j * g.lys + k;
+    const T xm_ = (i == 0) ? c.m[1] : (i == g.nx - 1) ? c.m[0] : (T)0;
+    const T ym_ = (j == 0) ? c.m[3] : (j == g.ny - 1) ? c.m[2] : (T)0;
+    const T zm_ = (k == 0) ? c.m[5] : (k == g.nz - 1) ? c.m[4] : (T)0;
+    const T v = relax_pt_host(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs],
+                              u[id + g.lxs], b[id], xm_, ym_, zm_);
+    u[id] = v;
+    face_ghosts(u, g, gw, id, i, j, k, v);
+  }
+}
+
+// the whole level-1 ghost layer (faces, edges, corners) from the interior
+template <typename T>
+__device__ void bot_fill(T *u, const BotGeom &g, const GhostL1<T> &gw) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int nZ = (nx + 2) * (ny + 2), nY = (nx + 2) * nz, nX = ny * nz;
+  for (int t = threadIdx.x; t < 2 * (nZ + nY + nX); t += blockDim.x) {
+    int i, j, k, q = t;
+    if (q < 2 * nZ) {
+      const int side = q >= nZ;
+      q -= side * nZ;
+      i = mdiv(q, g.mpy);
+      j = q - i * (ny + 2) - 1;
+      i -= 1;
+      k = side ? nz : -1;
+    } else if ((q -= 2 * nZ) < 2 * nY) {
+      const int side = q >= nY;
+      q -= side * nY;
+      i = mdiv(q, g.mz);
+      k = q - i * nz;
+      i -= 1;
+      j = side ? ny : -1;
+    } else {
+      q -= 2 * nY;
+      const int side = q >= nX;
+      q -= side * nX;
+      j = mdiv(q, g.mz);
+      k = q - j * nz;
+      i = side ? nx : -1;
+    }
+    const int sx = i < 0 ? -1 : i >= nx ? 1 : 0, sy = j < 0 ? -1 : j >= ny ? 1 : 0, sz = k < 0 ? -1 : k >= nz ? 1 : 0;
+    const int src = g.lshift + (i - sx) * g.lxs + (j - sy) * g.lys + (k - sz);
+    u[g.lshift + i * g.lxs + j * g.lys + k] = gw.chain(sx, sy, sz, u[src]);
+  }
+}
+
+template <typename T>
+__device__ void bot_relax(T *u, const T *b, const BotLevel<T> &L, int iters) {
+  for (int it = 0; it < iters; it++) {
+    bot_relax_dev(u, b, L.g, L.dc, L.gw, 0);
+    __syncthreads();
+    bot_relax_dev(u, b, L.g, L.dc, L.gw, 1);
+    __syncthreads();
+  }
+  bot_fill(u, L.g, L.gw);
+  __syncthreads();
+}
+
+// b_c = restrict(b_f - A u_f) over the coarse interior (k_res_restrict)
+template <typename T>
+__device__ void bot_res_restrict(const T *uf, const T *bf, const BotGeom &gf, const ResC<T> &c, T *bc,
+                                 const BotGeom &gc) {
+  for (int t = threadIdx.x; t < (int)gc.mn.n; t += blockDim.x) {
+    const int row = mdiv(t, gc.mz), i = mdiv(row, gc.my), j = row - i * gc.ny, k = t - row * gc.nz;
+    T rr[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int id = gf.lshift + (2 * i + (q >> 2)) * gf.lxs + (2 * j + ((q >> 1) & 1)) * gf.lys + 2 * k + (q & 1);
+      rr[q] = bf[id] - (c.fz * (uf[id - 1] + uf[id + 1]) + c.fy * (uf[id - gf.lys] + uf[id + gf.lys]) +
+                        c.fx * (uf[id - gf.lxs] + uf[id + gf.lxs]) - c.diag * uf[id]);
+    }
+    bc[gc.lshift + i * gc.lxs + j * gc.lys + k] = restrict_sum(rr[0], rr[1], rr[2], rr[3], rr[4], rr[5], rr[6], rr[7]);
+  }
+}
+
+// u_f += trilinear prolongation of u_c (k_prolong, reference corner choice)
+template <typename T>
+__device__ void bot_prolong(const T *uc, const BotGeom &gc, T *uf, const BotGeom &gf) {
+  const int n = (int)gc.mn.n;
+  for (int t = threadIdx.x; t < 8 * n; t += blockDim.x) {
+    const int q = mdiv(t, gc.mn), ct = t - q * n;  // child-major: consecutive threads walk coarse cells
+    const int row = mdiv(ct, gc.mz), ic = mdiv(row, gc.my), jc = row - ic * gc.ny, kc = ct - row * gc.nz;
+    const int dx = q >> 2, dy = (q >> 1) & 1, dz = q & 1;
+    const int id = gc.lshift + ic * gc.lxs + jc * gc.lys + kc;
+    const int sx = dx ? gc.lxs : -gc.lxs, sy = dy ? gc.lys : -gc.lys, sz = dz ? 1 : -1;
+    const int cx = (dx == 0 && dy == 1 && dz == 1) ? gc.lxs : sx;
+    const T C = uc[id];
+    const T X = uc[id + sx], Y = uc[id + sy], Z = uc[id + sz];
+    const T XY = uc[id + sx + sy], XZ = uc[id + sx + sz], YZ = uc[id + sy + sz];
+    const T XYZ = uc[id + cx + sy + sz];
+    T &f = uf[gf.lshift + (2 * ic + dx) * gf.lxs + (2 * jc + dy) * gf.lys + 2 * kc + dz];
+    f = f + PW<T>::c0 * C + PW<T>::c1 * (X + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+  }
+}
+
+// LDS box <-> global grid (ghosts included), LDS interior -> global
+template <typename T, bool LOAD>
+__device__ void bot_box(T *s, const DGrid<T> &gr, const BotGeom &g) {
+  for (int t = threadIdx.x; t < box_cells(g); t += blockDim.x) {
+    const int row = mdiv(t, g.mpz), i = mdiv(row, g.mpy), j = row - i * (g.ny + 2), k = t - row * (g.nz + 2);
+    if (LOAD)
+      s[t] = gr.at(i - 1, j - 1, k - 1);
+    else
+      gr.at(i - 1, j - 1, k - 1) = s[t];
+  }
+}
+template <typename T>
+__device__ void bot_store_interior(const T *s, const DGrid<T> &gr, const BotGeom &g) {
+  for (int t = threadIdx.x; t < (int)g.mn.n; t += blockDim.x) {
+    const int row = mdiv(t, g.mz), i = mdiv(row, g.my), j = row - i * g.ny, k = t - row * g.nz;
+    gr.at(i, j, k) = s[g.lshift + i * g.lxs + j * g.lys + k];
+  }
+}
+template <typename T>
+__device__ void bot_zero(T *s, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) s[t] = (T)0;
+}
+
+template <typename T, typename TC, int MODE>
+__global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> A) {
+  extern __shared__ unsigned char smem[];
+  auto U = [&](int l) { return (T *)(smem + A.L[l].off); };
+  auto B = [&](int l) { return (T *)(smem + A.L[l].off) + box_cells(A.L[l].g); };
+  const int C = A.nlev - 1;
+
+  // top level: u (or the zero grid with its zero ghosts) and b
+  if (A.zero_top)
+    bot_zero(U(0), box_cells(A.L[0].g));
+  else
+    bot_box<T, true>(U(0), A.L[0].u, A.L[0].g);
+  bot_box<T, true>(B(0), A.L[0].b, A.L[0].g);
+  __syncthreads();
+
+  // down: relax, zero start below, residual -> restriction
+  for (int l = 0; l < C; l++) {
+    bot_relax(U(l), B(l), A.L[l], A.nu1);
+    bot_zero(U(l + 1), 2 * box_cells(A.L[l + 1].g));  // u and b boxes (b ghosts stay zero)
+    __syncthreads();
+    bot_res_restrict(U(l), B(l), A.L[l].g, A.L[l].rc, B(l + 1), A.L[l + 1].g);
+    __syncthreads();
+  }
+
+  // coarsest level: relax_on_host iterations on the zero start, then the
+  // level-1 fill (mode 1: in TC on copies of the boxes)
+  {
+    const BotGeom &g = A.L[C].g;
+    const int box = box_cells(g);
+    TC *uc = (TC *)U(C), *bc = (TC *)B(C);
+    if (MODE == 1) {
+      uc = (TC *)(smem + A.coarse_off);
+      bc = uc + box;
+      for (int t = threadIdx.x; t < box; t += blockDim.x) {
+        uc[t] = (TC)U(C)[t];
+        bc[t] = (TC)B(C)[t];
+      }
+      __syncthreads();
+    }
+    for (int it = 0; it < A.coarse_iters; it++)
+      for (int rb = 0; rb < 2; rb++) {
+        bot_relax_host(uc, bc, g, A.hc, A.gwc, rb);
+        __syncthreads();
+      }
+    bot_fill(uc, g, A.gwc);
+    __syncthreads();
+    if (MODE == 1) {
+      for (int t = threadIdx.x; t < box; t += blockDim.x) U(C)[t] = (T)uc[t];
+      __syncthreads();
+    }
+  }
+
+  // up: prolongation + level-1 fill, relax
+  for (int l = C - 1; l >= 0; l--) {
+    bot_prolong(U(l + 1), A.L[l + 1].g, U(l), A.L[l].g);
+    __syncthreads();
+    bot_fill(U(l), A.L[l].g, A.L[l].gw);
+    __syncthreads();
+    bot_relax(U(l), B(l), A.L[l], A.nu2);
+  }
+
+  // write back every level (u boxes, b interiors below the top)
+  for (int l = 0; l <= C; l++) {
+    bot_box<T, false>(U(l), A.L[l].u, A.L[l].g);
+    if (l > 0) bot_store_interior(B(l), A.L[l].b, A.L[l].g);
   }
 }
 
@@ -866,8 +1174,8 @@ int fused_ichunk(int nx, int tiles_j, int slots) {
     int v = atoi(s);
     if (v > 0) return v;
   }
-  static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
-  int best = 4;
+  static const int cand[] = {1, 2, 4, 6, 8, 12, 16, 24, 32, 48, 64};
+  int best = 1;
   int64_t best_cost = INT64_MAX;
   for (int ic : cand) {
     const int64_t nb = (int64_t)tiles_j * ((nx + ic - 1) / ic);
@@ -995,9 +1303,107 @@ int prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, con
   OCU_HIP_LAUNCHED("ocu_mg_prolong");
 }
 
+// LDS layout of a bottom launch: per level the u box then the b box (T),
+// then (mode 1) the TC copies of the coarsest boxes.  Returns the bytes.
+template <typename T, typename TC>
+size_t bottom_layout(const ocu_grid3d *u, int nlev, int mode, BotArgs<T, TC> *A) {
+  size_t off = 0;
+  for (int l = 0; l < nlev; l++) {
+    const int nx = u[l].nx, ny = u[l].ny, nz = u[l].nz;
+    if (A) {
+      BotGeom &g = A->L[l].g;
+      g.nx = nx; g.ny = ny; g.nz = nz;
+      g.lys = nz + 2;
+      g.lxs = (ny + 2) * g.lys;
+      g.lshift = g.lxs + g.lys + 1;
+      g.my = mag(ny); g.mz = mag(nz); g.mpy = mag(ny + 2); g.mpz = mag(nz + 2);
+      g.mhd = mag(nz >> 1 ? nz >> 1 : 1);
+      g.mhd.n = nz >> 1;  // slot count; the divisor is unused when zero
+      g.mhh = mag((nz + 1) / 2);
+      g.mn = mag(nx * ny * nz);
+      A->L[l].off = (int)off;
+    }
+    off += 2 * (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(T);
+    off = (off + 15) & ~(size_t)15;
+  }
+  if (A) A->coarse_off = (int)off;
+  if (mode == 1) {
+    const ocu_grid3d &c = u[nlev - 1];
+    off += 2 * (size_t)(c.nx + 2) * (c.ny + 2) * (c.nz + 2) * sizeof(TC);
+  }
+  return off;
+}
+
+constexpr size_t BOT_LDS_MAX = 65536;
+
+template <typename T, typename TC, int MODE>
+int launch_bottom(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_params *p, const double *h, int nlev,
+                  const ocu_bc_set *bc, int nu1, int nu2, int coarse_iters, int zero_top, hipStream_t stream) {
+  BotArgs<T, TC> A;
+  memset(&A, 0, sizeof(A));
+  const size_t bytes = bottom_layout<T, TC>(u, nlev, MODE, &A);
+  for (int l = 0; l < nlev; l++) {
+    BotLevel<T> &L = A.L[l];
+    L.u = dgrid<T>(&u[l]);
+    L.b = dgrid<T>(&b[l]);
+    L.dc = dev_c<T>(&p[l]);
+    L.rc = res_c<T>(p[l].h, p[l].fx, p[l].fy, p[l].fz);
+    L.gw = ghost_l1<T>(bc, h[3 * l], h[3 * l + 1], h[3 * l + 2]);
+  }
+  A.nlev = nlev;
+  A.nu1 = nu1;
+  A.nu2 = nu2;
+  A.zero_top = zero_top;
+  A.hc = host_c<TC>(&p[nlev - 1]);
+  A.gwc = ghost_l1<TC>(bc, h[3 * (nlev - 1)], h[3 * (nlev - 1) + 1], h[3 * (nlev - 1) + 2]);
+  A.coarse_iters = coarse_iters;
+  k_mg_bottom<T, TC, MODE><<<1, BOT_THREADS, bytes, stream>>>(A);
+  OCU_HIP_LAUNCHED("ocu_mg_bottom_vcycle");
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t ocu_mg_bottom_lds_bytes(const ocu_grid3d *u, int nlev, int coarse_mode) {
+  if (!u || nlev < 2 || nlev > BOT_MAX) return 0;
+  const bool dbl = u[0].dtype == OCU_F64;
+  if (dbl) return bottom_layout<double, double>(u, nlev, 0, (BotArgs<double, double> *)nullptr);
+  return coarse_mode == 1 ? bottom_layout<float, double>(u, nlev, 1, (BotArgs<float, double> *)nullptr)
+                          : bottom_layout<float, float>(u, nlev, 0, (BotArgs<float, float> *)nullptr);
+}
+
+int ocu_mg_bottom_vcycle(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_params *p, const double *h,
+                         int nlev, const ocu_bc_set *bc, int nu1, int nu2, int coarse_iters, int coarse_mode,
+                         int zero_top, void *stream) {
+  const char *what = "ocu_mg_bottom_vcycle";
+  OCU_REQUIRE(u && b && p && h && bc, "%s: null argument", what);
+  OCU_REQUIRE(nlev >= 2 && nlev <= BOT_MAX, "%s: %d levels (2..%d)", what, nlev, BOT_MAX);
+  OCU_REQUIRE(nu1 >= 0 && nu2 >= 0 && coarse_iters >= 0, "%s: negative iteration count", what);
+  OCU_REQUIRE(coarse_mode == 0 || (coarse_mode == 1 && u[0].dtype == OCU_F32), "%s: bad coarse mode %d", what,
+              coarse_mode);
+  for (int l = 0; l < nlev; l++) {
+    if (int e = check_pair(&u[l], &b[l], what)) return e;
+    OCU_REQUIRE(u[l].dtype == u[0].dtype, "%s: mixed dtypes", what);
+    if (l > 0)
+      OCU_REQUIRE(u[l - 1].nx >= 2 * u[l].nx && u[l - 1].ny >= 2 * u[l].ny && u[l - 1].nz >= 2 * u[l].nz &&
+                      u[l].nx >= 1 && u[l].ny >= 1 && u[l].nz >= 1,
+                  "%s: level %d is not a coarsening of level %d", what, l, l - 1);
+  }
+  if (!gfaces_local(bc)) {
+    set_error("%s: needs Neumann or Dirichlet faces", what);
+    return OCU_ERR_UNSUPPORTED;
+  }
+  OCU_REQUIRE(!zero_top || gfaces_homogeneous(bc), "%s: zero start needs homogeneous faces", what);
+  const size_t bytes = ocu_mg_bottom_lds_bytes(u, nlev, coarse_mode);
+  OCU_REQUIRE(bytes > 0 && bytes <= BOT_LDS_MAX, "%s: %zu bytes of LDS exceed %zu", what, bytes, BOT_LDS_MAX);
+  if (u[0].dtype == OCU_F64)
+    return launch_bottom<double, double, 0>(u, b, p, h, nlev, bc, nu1, nu2, coarse_iters, zero_top, S(stream));
+  if (coarse_mode == 1)
+    return launch_bottom<float, double, 1>(u, b, p, h, nlev, bc, nu1, nu2, coarse_iters, zero_top, S(stream));
+  return launch_bottom<float, float, 0>(u, b, p, h, nlev, bc, nu1, nu2, coarse_iters, zero_top, S(stream));
+}
+
 
 int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color, const ocu_mg_level_params *p,
                  int variant, void *stream) {

@@ -123,7 +123,7 @@ void *ocu_get_compute_stream(void) { return ThreadManager::get_compute_stream();
 // ---------------------------------------------------------------- multigrid
 
 void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
-                    int use_fused_relax) {
+                    int options) {
   if (!rhs || !cfg || (rhs->dtype != OCU_F32 && rhs->dtype != OCU_F64)) return nullptr;
   MGHandle *h = new (std::nothrow) MGHandle();
   if (!h) return nullptr;
@@ -133,14 +133,16 @@ void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, cons
     h->d = new Sol_MultigridPressure3DDevice<double>();
     apply_cfg(*h->d, cfg);
     h->d->disable_relax_on_host = cfg->disable_relax_on_host != 0;
-    h->d->use_fused_relax = use_fused_relax != 0;
+    h->d->use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
+    h->d->use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
     ok = h->rhs_d.init_external(*rhs) &&
          h->d->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_d);
   } else {
     h->f = new Sol_MultigridPressure3DDevice<float>();
     apply_cfg(*h->f, cfg);
     h->f->disable_relax_on_host = cfg->disable_relax_on_host != 0;
-    h->f->use_fused_relax = use_fused_relax != 0;
+    h->f->use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
+    h->f->use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
     ok = h->rhs_f.init_external(*rhs) &&
          h->f->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_f);
   }
@@ -194,7 +196,7 @@ void ocu_mg_destroy(void *hh) { delete (MGHandle *)hh; }
 // ------------------------------------------------------------------ mixed IR
 
 void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
-                         int cycles, int cycles_first, int fmg_first) {
+                         int cycles, int cycles_first, int fmg_first, int options) {
   if (!rhs || !cfg || rhs->dtype != OCU_F64) return nullptr;
   MixedHandle *h = new (std::nothrow) MixedHandle();
   if (!h) return nullptr;
@@ -202,6 +204,8 @@ void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz,
   h->s.ir_cycles = cycles;
   h->s.ir_cycles_first = cycles_first;
   h->s.ir_fmg_first = fmg_first != 0;
+  h->s.use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
+  h->s.use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
   if (!h->rhs.init_external(*rhs) || !h->s.initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs)) {
     delete h;
     return nullptr;

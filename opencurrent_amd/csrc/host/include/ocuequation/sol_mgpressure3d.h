@@ -64,6 +64,10 @@ class Sol_MultigridPressure3DBase : public Solver {
   // bring every level's storage to the state the reference would hold
   // (implementations may defer work between the steps above)
   virtual void settle() {}
+  // first level of a one-launch bottom of the V-cycle (levels bottom..coarse
+  // in one workgroup), or -1; bottom_vcycle(top) runs that part from `top`
+  virtual int bottom_start() const { return -1; }
+  virtual void bottom_vcycle(int top) {}
 
  public:
   BoundaryConditionSet bc;
@@ -133,6 +137,9 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   bool defer_ok(int level) const;
   void materialize(int level);
   virtual void settle();
+  int _bottom_level;  // first level of the LDS bottom, -1 if none fits
+  virtual int bottom_start() const;
+  virtual void bottom_vcycle(int top);
 
  public:
   bool disable_relax_on_host;
@@ -140,6 +147,8 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   bool use_fused_relax;
   // also store the residual grids in restrict_residuals (read_r())
   bool store_residuals;
+  // run the small coarse levels of each V-cycle as one LDS-resident launch
+  bool use_bottom;
 
   Sol_MultigridPressure3DDevice();
   ~Sol_MultigridPressure3DDevice();

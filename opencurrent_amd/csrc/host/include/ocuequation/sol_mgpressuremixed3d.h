@@ -44,6 +44,9 @@ class Sol_MultigridPressureMixed3DDeviceD : public Sol_MultigridPressure3DBase {
   int ir_cycles;
   int ir_cycles_first;
   bool ir_fmg_first;
+  // options of the FP32 hierarchy (Sol_MultigridPressure3DDevice fields)
+  bool use_fused_relax;
+  bool use_bottom;
   // statistics of the last solve
   int ir_iterations;
   int total_vcycles;

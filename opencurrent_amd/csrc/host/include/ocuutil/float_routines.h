@@ -1,4 +1,4 @@
-// ocuutil/float_routines.h: check_float, min3, max3
+// ocuutil/float_routines.h: check_float, min3, max3 (max2 added)
 #ifndef __OCU_UTIL_FLOAT_ROUTINES_H__
 #define __OCU_UTIL_FLOAT_ROUTINES_H__
 
@@ -11,6 +11,11 @@ inline bool check_float(double f) { return std::isfinite(f); }
 template <typename T>
 T min3(T a, T b, T c) {
   return a < b ? (a < c ? a : c) : (b < c ? b : c);
+}
+
+template <typename T>
+T max2(T a, T b) {
+  return a > b ? a : b;
 }
 
 template <typename T>

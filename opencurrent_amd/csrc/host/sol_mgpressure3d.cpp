@@ -128,11 +128,15 @@ bool Sol_MultigridPressure3DBase::do_fmg(double tolerance, int max_iter, double 
     restrict_residuals(level, level + 1, 0, 0);
   }
 
+  const int bottom = bottom_start();
   for (int fine_level = _num_levels - 1; fine_level >= 0; fine_level--) {
     int level_ncyc = (fine_level == 0) ? max_iter + 1 : 1;
+    // levels top..coarse_level run as one launch when a bottom exists
+    const int top = (bottom >= 1 && fine_level < coarse_level) ? max2(bottom, fine_level) : -1;
+    const int stop = top >= 0 ? top : coarse_level;
     for (int i_cyc = 0; i_cyc < level_ncyc; i_cyc++) {
       if (fine_level == 0) num_vcycles++;
-      for (int level = fine_level; level < coarse_level; level++) {
+      for (int level = fine_level; level < stop; level++) {
         relax(level, nu1, RO_RED_BLACK);
         clear_zero(level + 1);
         apply_boundary_conditions(level + 1);
@@ -151,10 +155,14 @@ bool Sol_MultigridPressure3DBase::do_fmg(double tolerance, int max_iter, double 
           restrict_residuals(level, level + 1, 0, 0);
         }
       }
-      int cl = coarse_level;
-      int coarse_iters = max3(nx(cl) * ny(cl), ny(cl) * nz(cl), nx(cl) * nz(cl)) / 2;
-      relax(coarse_level, coarse_iters, make_symmetric_operator ? RO_SYMMETRIC : RO_RED_BLACK);
-      for (int level = coarse_level - 1; level >= fine_level; level--) {
+      if (top >= 0) {
+        bottom_vcycle(top);
+      } else {
+        int cl = coarse_level;
+        int coarse_iters = max3(nx(cl) * ny(cl), ny(cl) * nz(cl), nx(cl) * nz(cl)) / 2;
+        relax(coarse_level, coarse_iters, make_symmetric_operator ? RO_SYMMETRIC : RO_RED_BLACK);
+      }
+      for (int level = stop - 1; level >= fine_level; level--) {
         prolong(level + 1, level);
         if (level > 0) relax(level, nu2, make_symmetric_operator ? RO_BLACK_RED : RO_RED_BLACK);
       }
@@ -180,9 +188,11 @@ bool Sol_MultigridPressure3DBase::do_vcycle(double tolerance, int max_iter, doub
     if (convergence & CONVERGENCE_CALC_LINF) result_linf = orig_linf;
     return true;
   }
+  const int bottom = bottom_start();
+  const int stop = bottom >= 1 ? bottom : coarse_level;
   for (int i_cyc = 0; i_cyc < max_iter; i_cyc++) {
     num_vcycles++;
-    for (int level = 0; level < coarse_level; level++) {
+    for (int level = 0; level < stop; level++) {
       relax(level, nu1, RO_RED_BLACK);
       clear_zero(level + 1);
       apply_boundary_conditions(level + 1);
@@ -199,10 +209,14 @@ bool Sol_MultigridPressure3DBase::do_vcycle(double tolerance, int max_iter, doub
         restrict_residuals(level, level + 1, 0, 0);
       }
     }
-    int cl = coarse_level;
-    int coarse_iters = max3(nx(cl) * ny(cl), ny(cl) * nz(cl), nx(cl) * nz(cl)) / 2;
-    relax(coarse_level, coarse_iters, make_symmetric_operator ? RO_SYMMETRIC : RO_RED_BLACK);
-    for (int level = coarse_level - 1; level >= 0; level--) {
+    if (bottom >= 1) {
+      bottom_vcycle(bottom);
+    } else {
+      int cl = coarse_level;
+      int coarse_iters = max3(nx(cl) * ny(cl), ny(cl) * nz(cl), nx(cl) * nz(cl)) / 2;
+      relax(coarse_level, coarse_iters, make_symmetric_operator ? RO_SYMMETRIC : RO_RED_BLACK);
+    }
+    for (int level = stop - 1; level >= 0; level--) {
       prolong(level + 1, level);
       if (level > 0) relax(level, nu2, make_symmetric_operator ? RO_BLACK_RED : RO_RED_BLACK);
     }
@@ -288,6 +302,8 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   disable_relax_on_host = false;
   use_fused_relax = true;
   store_residuals = false;
+  use_bottom = true;
+  _bottom_level = -1;
   for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = false;
 }
 
@@ -364,6 +380,38 @@ void Sol_MultigridPressure3DDevice<T>::materialize(int level) {
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::settle() {
   for (int l = 0; l < _num_levels; l++) materialize(l);
+}
+
+template <typename T>
+int Sol_MultigridPressure3DDevice<T>::bottom_start() const {
+  if (!use_bottom || _bottom_level < 1 || !use_fused_relax || !_fused_ok || store_residuals ||
+      disable_relax_on_host || make_symmetric_operator)
+    return -1;
+  return _bottom_level;
+}
+
+// sol_mgpressure3d.cpp:184-239 from level `top` down to the coarsest and back,
+// in one launch
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::bottom_vcycle(int top) {
+  const int C = _num_levels - 1, nlev = C - top + 1;
+  ocu_grid3d u[kMaxLevels], b[kMaxLevels];
+  ocu_mg_level_params p[kMaxLevels];
+  double h[3 * kMaxLevels];
+  for (int l = 0; l < nlev; l++) {
+    u[l] = get_u(top + l).desc();
+    b[l] = get_b(top + l).desc();
+    level_params(top + l, &p[l]);
+    h[3 * l] = hx(top + l);
+    h[3 * l + 1] = hy(top + l);
+    h[3 * l + 2] = hz(top + l);
+  }
+  ocu_bc_set cbc = to_c_bc(get_bc_at_level(top));
+  int coarse_iters = max3(nx(C) * ny(C), ny(C) * nz(C), nx(C) * nz(C)) / 2;
+  ok_status(ocu_mg_bottom_vcycle(u, b, p, h, nlev, &cbc, nu1, nu2, coarse_iters, _coarse_mode, _u_zero[top] ? 1 : 0,
+                                 ThreadManager::get_compute_stream()),
+            "Sol_MultigridPressure3DDevice::bottom_vcycle");
+  for (int l = top; l <= C; l++) _u_zero[l] = false;
 }
 
 template <typename T>
@@ -614,6 +662,18 @@ bool Sol_MultigridPressure3DDevice<T>::initialize_storage(int nx_val, int ny_val
   }
   _norm_ws = (double *)ws;
   _norm_host = (double *)hh;
+
+  // the smallest level (>= 1) from which every coarser level fits one
+  // workgroup's LDS together
+  _bottom_level = -1;
+  const int C = _num_levels - 1;
+  for (int top = C - 1; top >= 1; top--) {
+    ocu_grid3d d[kMaxLevels];
+    for (int l = top; l <= C; l++) d[l - top] = get_u(l).desc();
+    size_t bytes = ocu_mg_bottom_lds_bytes(d, C - top + 1, _coarse_mode);
+    if (bytes == 0 || bytes > 65536) break;
+    _bottom_level = top;
+  }
   return !any_error();
 }
 

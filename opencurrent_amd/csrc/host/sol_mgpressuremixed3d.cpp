@@ -21,6 +21,8 @@ Sol_MultigridPressureMixed3DDeviceD::Sol_MultigridPressureMixed3DDeviceD() {
   ir_cycles = 4;
   ir_cycles_first = 4;
   ir_fmg_first = false;
+  use_fused_relax = true;
+  use_bottom = true;
   ir_iterations = 0;
   total_vcycles = 0;
   rhs_l2 = 0;
@@ -59,6 +61,8 @@ bool Sol_MultigridPressureMixed3DDeviceD::initialize_storage(int nx_val, int ny_
   _mg_f.nu2 = nu2;
   _mg_f.convergence = CONVERGENCE_NONE;
   _mg_f._coarse_mode = 1;
+  _mg_f.use_fused_relax = use_fused_relax;
+  _mg_f.use_bottom = use_bottom;
   if (!_mg_f.initialize_storage(nx_val, ny_val, nz_val, hx_val, hy_val, hz_val, &_r32)) {
     printf("[ERROR] Sol_MultigridPressureMixed3DDeviceD::initialize_storage - failed to initialize float solver\n");
     return false;
@@ -106,15 +110,23 @@ bool Sol_MultigridPressureMixed3DDeviceD::interior_mean(const Grid3DDevice<doubl
 // test; the coarsest level is relaxed in FP64)
 void Sol_MultigridPressureMixed3DDeviceD::vcycle_f32(int fine_level) {
   int coarse = _num_levels - 1;
-  for (int level = fine_level; level < coarse; level++) {
+  // levels top..coarse in one launch when the FP32 solver has a bottom
+  const int bottom = _mg_f.bottom_start();
+  const int top = (bottom >= 1 && fine_level < coarse) ? max2(bottom, fine_level) : -1;
+  const int stop = top >= 0 ? top : coarse;
+  for (int level = fine_level; level < stop; level++) {
     _mg_f.relax(level, nu1, RO_RED_BLACK);
     _mg_f.clear_zero(level + 1);
     _mg_f.apply_boundary_conditions(level + 1);
     _mg_f.restrict_residuals(level, level + 1, 0, 0);
   }
-  int coarse_iters = max3(nx(coarse) * ny(coarse), ny(coarse) * nz(coarse), nx(coarse) * nz(coarse)) / 2;
-  _mg_f.relax(coarse, coarse_iters, RO_RED_BLACK);
-  for (int level = coarse - 1; level >= fine_level; level--) {
+  if (top >= 0) {
+    _mg_f.bottom_vcycle(top);
+  } else {
+    int coarse_iters = max3(nx(coarse) * ny(coarse), ny(coarse) * nz(coarse), nx(coarse) * nz(coarse)) / 2;
+    _mg_f.relax(coarse, coarse_iters, RO_RED_BLACK);
+  }
+  for (int level = stop - 1; level >= fine_level; level--) {
     _mg_f.prolong(level + 1, level);
     if (level > 0) _mg_f.relax(level, nu2, RO_RED_BLACK);
   }

@@ -17,6 +17,13 @@ from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, OCU_F32, OCU_F
 from .device import desc_to_host, host_to_desc
 
 
+MG_FUSED_RELAX, MG_LDS_BOTTOM = 1, 2
+
+
+def _options(use_fused_relax, use_bottom):
+    return (MG_FUSED_RELAX if use_fused_relax else 0) | (MG_LDS_BOTTOM if use_bottom else 0)
+
+
 def _history(fn, h, extra=()):
     cap = 4096
     hist = (C.c_double * cap)()
@@ -28,10 +35,11 @@ class MultigridPressure3D:
     """Sol_MultigridPressure3DDevice<T> (ocuequation/sol_mgpressure3d.h:130-181)."""
 
     def __init__(self, rhs, hx, hy, hz, bc, nu1=2, nu2=2, convergence=CONVERGENCE_L2, sym=False,
-                 disable_relax_on_host=False, use_fused_relax=True):
+                 disable_relax_on_host=False, use_fused_relax=True, use_bottom=True):
         self._rhs = rhs  # keep alive
         self.cfg = MGConfig.make(bc, nu1, nu2, convergence, sym, disable_relax_on_host)
-        self.h = _lib.host().ocu_mg_create(rhs.ptr(), hx, hy, hz, C.byref(self.cfg), int(use_fused_relax))
+        self.h = _lib.host().ocu_mg_create(rhs.ptr(), hx, hy, hz, C.byref(self.cfg),
+                                           _options(use_fused_relax, use_bottom))
         if not self.h:
             raise _lib.OcuError(f"Sol_MultigridPressure3DDevice::initialize_storage failed: {_lib.last_error()}")
 
@@ -73,11 +81,11 @@ class MultigridPressureMixed3D:
     """Sol_MultigridPressureMixed3DDeviceD: FP32 V-cycles inside FP64 iterative refinement."""
 
     def __init__(self, rhs, hx, hy, hz, bc, nu1=2, nu2=2, convergence=CONVERGENCE_L2, cycles=4,
-                 cycles_first=4, fmg_first=False):
+                 cycles_first=4, fmg_first=False, use_fused_relax=True, use_bottom=True):
         self._rhs = rhs
         self.cfg = MGConfig.make(bc, nu1, nu2, convergence)
         self.h = _lib.host().ocu_mgmixed_create(rhs.ptr(), hx, hy, hz, C.byref(self.cfg), cycles, cycles_first,
-                                                int(fmg_first))
+                                                int(fmg_first), _options(use_fused_relax, use_bottom))
         if not self.h:
             raise _lib.OcuError(f"Sol_MultigridPressureMixed3DDeviceD::initialize_storage failed: {_lib.last_error()}")
 

@@ -64,10 +64,18 @@ def init_guess(b, rand_init):
     return p
 
 
+OPTIONS = {"plain": (False, False), "fused": (True, False), "fused+bottom": (True, True)}
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("fused", [True, False])
-def test_mg_solve_matches_oracle(gpu, dtype, case, fused):
+@pytest.mark.parametrize("opts", list(OPTIONS))
+def test_mg_solve_matches_oracle(gpu, dtype, case, opts):
+    """The drop-in solver's FMG solve against the oracle's restatement of the
+    reference driver: same V-cycle count and residual history, bit-identical
+    pressure, with each of the solver's launch options (the per-level path,
+    the fused sweeps, the LDS-resident bottom of the V-cycle)."""
+    fused, bottom = OPTIONS[opts]
     n = (32, 32, 32)
     h = (4.0 / 32, 4.0 / 32, 4.0 / 32)
     tol = 1e-8 if dtype == np.float64 else 1e-3
@@ -78,7 +86,7 @@ def test_mg_solve_matches_oracle(gpu, dtype, case, fused):
     ok_o, res_o, vc_o, hist_o = O.mg_solve(want, b, h, cfg, tol, 16)
     assert ok_o == 1
     gb = D().DeviceGrid.from_host(b)
-    mg = S().MultigridPressure3D(gb, *h, bc, use_fused_relax=fused)
+    mg = S().MultigridPressure3D(gb, *h, bc, use_fused_relax=fused, use_bottom=bottom)
     mg.set_pressure(p0)
     ok, res = mg.solve(tol, 16)
     st = mg.stats()
@@ -191,6 +199,46 @@ def test_mixed_ir_neumann(gpu, n, fmg_first):
     assert np.abs(x - xr).max() <= 1e-9 * scale
     assert np.abs(x - xo.interior()).max() <= 1e-9 * scale
     assert abs(x.mean()) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("n", [(32, 32, 32), (64, 48, 40), (16, 16, 16)])
+@pytest.mark.parametrize("fmg_first", [False, True])
+def test_mixed_ir_launch_options_agree(gpu, n, fmg_first):
+    """The fused sweeps and the LDS-resident bottom give the refinement the
+    same FP32 corrections bit for bit as the per-level launches."""
+    h = (1.0 / n[0], 1.0 / n[1], 1.0 / n[2])
+    bc = BCSet.uniform(T.BC_NEUMANN)
+    b = divergence_rhs(*n, h[0])
+    tol = 1e-10 * np.sqrt((b.interior() ** 2).mean())
+    gb = D().DeviceGrid.from_host(b)
+    out = {}
+    for opts, (fused, bottom) in OPTIONS.items():
+        mx = S().MultigridPressureMixed3D(gb, *h, bc, fmg_first=fmg_first, use_fused_relax=fused, use_bottom=bottom)
+        ok, res = mx.solve(tol, 10)
+        assert ok, opts
+        out[opts] = (mx.pressure().buf.copy(), mx.stats()["ir_iterations"], res)
+    ref = out["plain"]
+    for opts, got in out.items():
+        assert got[1] == ref[1] and got[2] == ref[2], opts
+        assert np.array_equal(got[0], ref[0]), opts
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(64, 48, 40), (16, 8, 32)])
+def test_mg_vcycles_launch_options_agree(gpu, dtype, n):
+    """run_vcycles and run_fmg with the bottom launch against the oracle on
+    non-cubic grids (levels of different shapes)."""
+    bc = bc_isotropic(2, .321)
+    h = (1.0 / n[0], 1.0 / n[1], 1.0 / n[2])
+    b = trig_rhs(*n, *h, axis=2, dtype=dtype)
+    for mode, fn in ((1, "run_vcycles"), (2, "run_fmg")):
+        cfg = MGConfig.make(bc, convergence=T.CONVERGENCE_NONE)
+        want = HostGrid.congruent(b)
+        O.mg_solve(want, b, h, cfg, 0.0, 2, mode=mode)
+        gb = D().DeviceGrid.from_host(b)
+        mg = S().MultigridPressure3D(gb, *h, bc, convergence=T.CONVERGENCE_NONE)
+        assert getattr(mg, fn)(2)
+        assert np.array_equal(mg.pressure().buf, want.buf), fn
 
 
 def test_mixed_ir_dirichlet(gpu):
