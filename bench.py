@@ -71,8 +71,9 @@ class Events:
 
 def time_fused_sweep(n, reps, dtype=np.float32):
     """Average duration of the level-0 fused red+black sweep at n^3 (the
-    dominant kernel of the V-cycle), launched back to back on the solver's
-    stream and timed with HIP events."""
+    dominant kernel of the V-cycle) as the solver launches it - writing the
+    output's ghost layer too - back to back on the solver's stream, timed with
+    HIP events."""
     from opencurrent_amd import _lib
     from opencurrent_amd import _types as T
     from opencurrent_amd.device import DeviceGrid
@@ -88,13 +89,14 @@ def time_fused_sweep(n, reps, dtype=np.float32):
     p = O.level_params(bc, 0, 1.0 / n, 1.0 / n, 1.0 / n)
     import torch
     torch.cuda.synchronize()
-    args = (u.ptr(), o.ptr(), b.ptr(), C.byref(p), C.byref(bc), 0, stream)
+    args = (u.ptr(), o.ptr(), b.ptr(), C.byref(p), C.byref(bc), 1.0 / n, 1.0 / n, 1.0 / n, 0, T.FUSED_WRITE_GHOSTS,
+            stream)
     for _ in range(5):
-        _lib.check(hip.ocu_mg_relax_fused(*args), "fused sweep")
+        _lib.check(hip.ocu_mg_relax_fused_bc(*args), "fused sweep")
     ev = Events(hip, stream)
     ev.start()
     for _ in range(reps):
-        hip.ocu_mg_relax_fused(*args)
+        hip.ocu_mg_relax_fused_bc(*args)
     ms = ev.stop_ms() / reps
     w = np.dtype(dtype).itemsize
     alg_bytes = 3 * w * n ** 3  # read u and b, write u, once per cell
@@ -241,7 +243,7 @@ def main():
     achieved = kbytes / (kms / 1e3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                       "kernel": "k_relax_fused<float> (level-0 red+black sweep)",
+                       "kernel": "k_relax_fused_rows<float,4,2,10> (level-0 red+black sweep with ghost writes)",
                        "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes}
 
     if not args.no_cpu:

@@ -305,6 +305,42 @@ __device__ __forceinline__ void write_ghost_row(const DGrid<T> &o, const GhostL1
   if (k0 + V == o.nz) row[o.nz] = gw.xf(4, w.v[V - 1]);
 }
 
+// diagonal of row (i,j) before the z terms: diag_dev's x and y mods in order
+template <typename T, typename C>
+__device__ __forceinline__ T diag_xy(const C &c, int i, int j, int nx, int ny) {
+  T d = c.ld;
+  if (i == nx - 1) d += c.m[0];
+  if (i == 0) d += c.m[1];
+  if (j == ny - 1) d += c.m[2];
+  if (j == 0) d += c.m[3];
+  return d;
+}
+
+// One colour on a row held as V cells per lane (k = k0..k0+V-1): the cells
+// with k%2 == kpar get relax_pt (diag_dev's z mods added to dxy in order),
+// the others keep their value in out.  k0 is even for V >= 2, so the colour
+// of a cell depends on e alone and the branches are uniform.
+template <typename T, int V, typename C>
+__device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, int kupd, int nz, const Vec<T, V> &cen,
+                                          T lo, T hi, const Vec<T, V> &ym, const Vec<T, V> &yp,
+                                          const Vec<T, V> &xm, const Vec<T, V> &xp, const Vec<T, V> &b,
+                                          Vec<T, V> &out) {
+#pragma unroll
+  for (int e = 0; e < V; e++) {
+    const int k = k0 + e;
+    if (V % 2 == 0) {
+      if ((e & 1) != kpar) continue;
+    } else if ((k & 1) != kpar || k >= kupd) {
+      continue;
+    }
+    T d = dxy;
+    if (k == nz - 1) d += c.m[4];
+    if (k == 0) d += c.m[5];
+    out.v[e] = relax_pt(c, cen.v[e], e > 0 ? cen.v[e - 1] : lo, e < V - 1 ? cen.v[e + 1] : hi, ym.v[e], yp.v[e],
+                        xm.v[e], xp.v[e], b.v[e], d);
+  }
+}
+
 // Rows of a block: NW waves of R rows each; the outer two rows on each side
 // are the halo, so a block writes R*NW-4 rows.
 //
@@ -326,7 +362,7 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
   if (logical >= ntiles) return;  // uniform per block
   const int tj = logical % tiles_j, ti = logical / tiles_j;
 
-  const int lane = threadIdx.x, w = threadIdx.y;
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one row set per wave
   const int t0 = w * R;  // first block row of this wave
   const int nx = uin.nx, ny = uin.ny, nz = uin.nz;
   const int jb = tj * (NR - 4) - 2 + t0;  // j of the wave's first row
@@ -373,12 +409,20 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
   // U1 planes p-2, p-1, p; b planes p-1, p and the prefetched p+1
   VT a[R], cc[R], n[R], nn[R], u1m2[R], u1m1[R], u1[R], bprev[R], bcur[R], bn[R];
   T ga[R], gc[R], gn[R], gnn[R];
+#ifdef OCU_FUSED_DEEP
+  VT n3[R], bn2[R];
+  T gn3[R];
+#endif
 #pragma unroll
   for (int r = 0; r < R; r++) {
     load(r, i0 - 2, a[r], ga[r]);
     load(r, i0 - 1, cc[r], gc[r]);
     load(r, i0, n[r], gn[r]);
     bn[r] = loadb(r, i0 - 1);
+#ifdef OCU_FUSED_DEEP
+    load(r, i0 + 1, nn[r], gnn[r]);
+    bn2[r] = loadb(r, i0);
+#endif
 #pragma unroll
     for (int e = 0; e < V; e++) u1m2[r].v[e] = u1m1[r].v[e] = bprev[r].v[e] = (T)0;
   }
@@ -387,9 +431,16 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
     const bool p_in = p >= 0 && p < nx;
 #pragma unroll
     for (int r = 0; r < R; r++) {
+#ifdef OCU_FUSED_DEEP
+      load(r, p + 3, n3[r], gn3[r]);
+      bcur[r] = bn[r];
+      bn[r] = bn2[r];
+      bn2[r] = loadb(r, p + 2);
+#else
       load(r, p + 2, nn[r], gnn[r]);
       bcur[r] = bn[r];
       bn[r] = loadb(r, p + 1);
+#endif
       sU0[t0 + r][lane] = cc[r];
     }
     __syncthreads();
@@ -404,15 +455,8 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
         const int j = jb + r;
         const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[t0 - 1][lane];
         const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[t0 + R][lane];
-#pragma unroll
-        for (int e = 0; e < V; e++) {
-          const int k = k0 + e;
-          if (k < kupd && ((p + j + k) & 1) == first_color) {
-            const T d = diag_dev<T>(c, p, j, k, nx, ny, nz);
-            u1[r].v[e] = relax_pt(c, cc[r].v[e], e > 0 ? cc[r].v[e - 1] : lo, e < V - 1 ? cc[r].v[e + 1] : hi,
-                                  ym.v[e], yp.v[e], a[r].v[e], n[r].v[e], bcur[r].v[e], d);
-          }
-        }
+        relax_row<T, V>(c, diag_xy<T>(c, p, j, nx, ny), (first_color ^ p ^ j) & 1, k0, kupd, nz, cc[r], lo, hi, ym,
+                        yp, a[r], n[r], bcur[r], u1[r]);
       }
       sU1[p & 1][t0 + r][lane] = u1[r];
     }
@@ -429,15 +473,8 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
         const VT ym = r > 0 ? u1m1[r > 0 ? r - 1 : 0] : sU1[q & 1][t0 - 1][lane];
         const VT yp = r < R - 1 ? u1m1[r < R - 1 ? r + 1 : 0] : sU1[q & 1][t0 + R][lane];
         VT v = u1m1[r];
-#pragma unroll
-        for (int e = 0; e < V; e++) {
-          const int k = k0 + e;
-          if (k < kupd && ((q + j + k) & 1) != first_color) {
-            const T d = diag_dev<T>(c, q, j, k, nx, ny, nz);
-            v.v[e] = relax_pt(c, u1m1[r].v[e], e > 0 ? u1m1[r].v[e - 1] : lo, e < V - 1 ? u1m1[r].v[e + 1] : hi,
-                              ym.v[e], yp.v[e], u1m2[r].v[e], u1[r].v[e], bprev[r].v[e], d);
-          }
-        }
+        relax_row<T, V>(c, diag_xy<T>(c, q, j, nx, ny), (first_color ^ q ^ j ^ 1) & 1, k0, kupd, nz, u1m1[r], lo, hi,
+                        ym, yp, u1m2[r], u1[r], bprev[r], v);
         if (active) {
           *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
           if (wghost) {  // ghosts of this row: z ends, x and y face rows
@@ -455,6 +492,9 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
       a[r] = cc[r]; ga[r] = gc[r];
       cc[r] = n[r]; gc[r] = gn[r];
       n[r] = nn[r]; gn[r] = gnn[r];
+#ifdef OCU_FUSED_DEEP
+      nn[r] = n3[r]; gnn[r] = gn3[r];
+#endif
       u1m2[r] = u1m1[r]; u1m1[r] = u1[r];
       bprev[r] = bcur[r];
     }
