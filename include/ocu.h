@@ -85,6 +85,9 @@ typedef struct ocu_ns_params {
   double gravity, bouyancy, thermal_diffusion, viscosity;
   int32_t advection_scheme, time_step;
   double cfl_factor;
+  /* not in the reference: FP64 equations projected with FP32 V-cycles in
+   * FP64 iterative refinement (dtype OCU_F64 only) */
+  int32_t mixed_projection;
 } ocu_ns_params;
 
 /* init_* are host buffers laid out by Eqn_IncompressibleNS3DParams::init_grids

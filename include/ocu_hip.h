@@ -67,6 +67,9 @@ int ocu_grid_clear(const ocu_grid3d *g, double value, void *stream);
 /* dst[n] = (T_dst) src[n] over the allocation; copy_all_data<T,S>
  * (grid3ddev.cu:25-33, 216-281).  Layouts must match. */
 int ocu_grid_copy(const ocu_grid3d *dst, const ocu_grid3d *src, void *stream);
+/* zero the first ghost layer (faces, edges, corners); no reference
+ * counterpart (adjoint fields carry no ghost values) */
+int ocu_grid_clear_ghosts(const ocu_grid3d *g, void *stream);
 /* r = a1*g1 (+ a2*g2); Grid3DDevice<T>::linear_combination
  * (grid3ddev.cu:283-349) including its 3%-waste rule for choosing between
  * an interior-only and a whole-allocation update.  g2 may be NULL. */
@@ -225,6 +228,28 @@ int ocu_self_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *
 int ocu_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
                        const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
                        double hx, double hy, double hz, int scheme, void *stream);
+/* ---- adjoints (no reference counterpart; for gradients of losses through
+ * the advection-projection path) ----
+ * Adjoint of ocu_passive_advect with the velocity's upwind selection fixed:
+ * lam is the adjoint of dphi on the interior (its ghost layer must be zero);
+ * writes lam_out = lam + A_phi^T(scale*lam) over the ghost box (with scale =
+ * dt this is the adjoint of phi + dt*A(u,phi)) and accumulates
+ * adj_u/v/w += A_u^T(scale*lam) on the faces [0,n] of interior rows.  All
+ * grids congruent with u; lam_out must not alias lam. */
+int ocu_passive_advect_adjoint(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                               const ocu_grid3d *phi, const ocu_grid3d *lam, const ocu_grid3d *lam_out,
+                               const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const ocu_grid3d *adj_w,
+                               double scale, int nx, int ny, int nz, double hx, double hy, double hz,
+                               int scheme, void *stream);
+/* Adjoint of ocu_bc_fill_nocorners: folds every face ghost's adjoint into
+ * the interior cell it derives from (periodic/Neumann +1, Dirichlet -1,
+ * BC_NONE dropped) and zeroes the face ghosts. */
+int ocu_bc_adjoint_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, void *stream);
+/* Adjoint of the normal-face part of ocu_bc_fill_mac_level1 on the faces an
+ * advection reads: periodic high face n duplicates face 0 (adj(0) +=
+ * adj(n)); forced-inflow low face 0 is fixed (adj(0) = 0); adj(n) = 0. */
+int ocu_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                           const ocu_bc_set *bc, void *stream);
 /* Sol_LaplacianCentered3DDevice_stencil (sol_laplaciancent3ddev.cu:27-62) */
 int ocu_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
                       double hx, double hy, double hz, double coefficient, void *stream);

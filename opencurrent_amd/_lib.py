@@ -79,6 +79,10 @@ HIP_SIGS = {
     "ocu_norm_finalize": (I, [DP, DP, P]),
     "ocu_mg_prolong": (I, [GP, GP, I, P]),
     "ocu_mg_prolong_bc": (I, [GP, GP, I, BP, D, D, D, P]),
+    "ocu_grid_clear_ghosts": (I, [GP, P]),
+    "ocu_passive_advect_adjoint": (I, [GP, GP, GP, GP, GP, GP, GP, GP, GP, D, I, I, I, D, D, D, I, P]),
+    "ocu_bc_adjoint_nocorners": (I, [GP, BP, P]),
+    "ocu_mac_adjoint_normal": (I, [GP, GP, GP, BP, P]),
     "ocu_mg_bottom_vcycle": (I, [GP, GP, MP, DP, I, BP, I, I, I, I, I, P]),
     "ocu_mg_bottom_lds_bytes": (C.c_size_t, [GP, I, I]),
     "ocu_mg_coarse_relax": (I, [GP, GP, I, I, MP, BP, D, D, D, I, P]),
@@ -108,6 +112,7 @@ class NSParams(C.Structure):
         ("thermal_diffusion", C.c_double), ("viscosity", C.c_double),
         ("advection_scheme", C.c_int32), ("time_step", C.c_int32),
         ("cfl_factor", C.c_double),
+        ("mixed_projection", C.c_int32),
     ]
 
 

@@ -46,6 +46,21 @@ __global__ void k_lincomb_interior(DGrid<T> r, T a1, DGrid<T> g1, T a2, DGrid<T>
   r.p[id] = g1.p[id] * a1 + g2.p[id] * a2;
 }
 
+// zero the first ghost layer: one thread per (i,j) column of the ghost box;
+// ghost columns are zeroed whole, interior columns at k = -1 and k = nz
+template <typename T>
+__global__ void k_clear_ghosts(DGrid<T> g) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (g.nx + 2) * (g.ny + 2)) return;
+  const int i = t / (g.ny + 2) - 1, j = t % (g.ny + 2) - 1;
+  if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) {
+    for (int k = -1; k <= g.nz; k++) g.at(i, j, k) = (T)0;
+  } else {
+    g.at(i, j, -1) = (T)0;
+    g.at(i, j, g.nz) = (T)0;
+  }
+}
+
 template <typename T>
 __global__ void k_add_scalar(DGrid<T> g, T v) {
   int k = blockIdx.x * TK + threadIdx.x;
@@ -134,6 +149,17 @@ int ocu_grid_clear(const ocu_grid3d *g, double value, void *stream) {
   else if (g->dtype == OCU_F32) k_fill<float><<<nb, 256, 0, S(stream)>>>((float *)g->base, (float)value, g->allocated);
   else k_fill<int><<<nb, 256, 0, S(stream)>>>((int *)g->base, (int)value, g->allocated);
   OCU_HIP_LAUNCHED("ocu_grid_clear");
+}
+
+int ocu_grid_clear_ghosts(const ocu_grid3d *g, void *stream) {
+  if (int e = check_grid(g, "ocu_grid_clear_ghosts")) return e;
+  OCU_REQUIRE(g->gx >= 1 && g->gy >= 1 && g->gz >= 1, "ocu_grid_clear_ghosts: needs one ghost layer");
+  OCU_REQUIRE(g->dtype != OCU_I32, "ocu_grid_clear_ghosts: int grid");
+  const int n = (g->nx + 2) * (g->ny + 2), nb = (n + 255) / 256;
+  if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) return OCU_OK;
+  if (g->dtype == OCU_F64) k_clear_ghosts<double><<<nb, 256, 0, S(stream)>>>(dgrid<double>(g));
+  else k_clear_ghosts<float><<<nb, 256, 0, S(stream)>>>(dgrid<float>(g));
+  OCU_HIP_LAUNCHED("ocu_grid_clear_ghosts");
 }
 
 int ocu_grid_copy(const ocu_grid3d *dst, const ocu_grid3d *src, void *stream) {

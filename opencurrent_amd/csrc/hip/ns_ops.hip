@@ -120,6 +120,131 @@ __global__ __launch_bounds__(256) void k_passive_advect(const T *__restrict__ u,
   d[id] = (-du * ih.x) - (dv * ih.y) - (dw * ih.z);
 }
 
+// ---- adjoints (no reference counterpart) ----
+//
+// passive advection is dphi_c = -sum_axes (F_{c+1} - F_c)/h with face fluxes
+// F_f = u_f * I(u_f, phi_{f-1}, phi_f), linear in phi for fixed u and in u for
+// a fixed upwind selection.  For lam (adjoint of dphi, zero ghost layer) and
+// g_f = scale * ((lam_f - lam_{f-1}) / h), one thread per cell of the ghost
+// box gathers  lam_out = lam + sum_faces (g_f * u_f) * dI/dphi_c  and, as the
+// owner of its low faces, accumulates  adj_u_f += g_f * I(u_f, ...).
+template <typename T>
+__device__ __forceinline__ T interp_w0(int scheme, T u) {
+  if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return (T).5;
+  return u < 0 ? (T)0 : (T)1;
+}
+template <typename T>
+__device__ __forceinline__ T interp_w1(int scheme, T u) {
+  if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return (T).5;
+  return u < 0 ? (T)1 : (T)0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_passive_advect_adjoint(
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w, const T *__restrict__ phi,
+    const T *__restrict__ lam, T *__restrict__ out, T *__restrict__ au, T *__restrict__ av, T *__restrict__ aw,
+    Inv<T> ih, T sc, int64_t xs, int64_t ys, int nx, int ny, int nz, int scheme) {
+  const int k = (int)(blockIdx.x * TK + threadIdx.x) - 1;
+  const int j = (int)(blockIdx.y * TJ + threadIdx.y) - 1;
+  const int i = (int)blockIdx.z - 1;
+  if (j > ny || k > nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  const bool ii = i >= 0 && i < nx, ji = j >= 0 && j < ny, ki = k >= 0 && k < nz;
+  const T lc = lam[id];
+  T acc = 0;
+  if (ji && ki) {
+    if (i >= 0) {
+      const T g = sc * (ih.x * (lc - lam[id - xs]));
+      const T f = u[id];
+      acc += (g * f) * interp_w1(scheme, f);
+      au[id] += g * interp(scheme, f, phi[id - xs], phi[id]);
+    }
+    if (i < nx) {
+      const T g = sc * (ih.x * (lam[id + xs] - lc));
+      const T f = u[id + xs];
+      acc += (g * f) * interp_w0(scheme, f);
+    }
+  }
+  if (ii && ki) {
+    if (j >= 0) {
+      const T g = sc * (ih.y * (lc - lam[id - ys]));
+      const T f = v[id];
+      acc += (g * f) * interp_w1(scheme, f);
+      av[id] += g * interp(scheme, f, phi[id - ys], phi[id]);
+    }
+    if (j < ny) {
+      const T g = sc * (ih.y * (lam[id + ys] - lc));
+      const T f = v[id + ys];
+      acc += (g * f) * interp_w0(scheme, f);
+    }
+  }
+  if (ii && ji) {
+    if (k >= 0) {
+      const T g = sc * (ih.z * (lc - lam[id - 1]));
+      const T f = w[id];
+      acc += (g * f) * interp_w1(scheme, f);
+      aw[id] += g * interp(scheme, f, phi[id - 1], phi[id]);
+    }
+    if (k < nz) {
+      const T g = sc * (ih.z * (lam[id + 1] - lc));
+      const T f = w[id + 1];
+      acc += (g * f) * interp_w0(scheme, f);
+    }
+  }
+  out[id] = lc + acc;
+}
+
+// adjoint of the device no-corner fill: each face ghost is c * (source) +
+// const with c = 1 (periodic copy, Neumann), -1 (Dirichlet), 0 (none); the
+// source cell gathers c * adj(ghost) (x, y, z; low then high face) and zeroes
+// the ghost it read - every face ghost has exactly one source.
+struct GhostCoef {
+  float c[6];   // xpos, xneg, ypos, yneg, zpos, zneg
+  int per[6];   // periodic: the ghost's source is the opposite end
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_adjoint_nocorners(T *__restrict__ a, GhostCoef gc, int64_t xs,
+                                                              int64_t ys, int nx, int ny, int nz) {
+  const int k = blockIdx.x * TK + threadIdx.x;
+  const int j = blockIdx.y * TJ + threadIdx.y;
+  const int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  const bool xl = i == (gc.per[1] ? nx - 1 : 0), xh = i == (gc.per[0] ? 0 : nx - 1);
+  const bool yl = j == (gc.per[3] ? ny - 1 : 0), yh = j == (gc.per[2] ? 0 : ny - 1);
+  const bool zl = k == (gc.per[5] ? nz - 1 : 0), zh = k == (gc.per[4] ? 0 : nz - 1);
+  if (!(xl || xh || yl || yh || zl || zh)) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  T acc = a[id];
+  auto take = [&](bool on, int face, int64_t gid) {
+    if (!on) return;
+    acc += (T)gc.c[face] * a[gid];
+    a[gid] = (T)0;
+  };
+  take(xl, 1, (int64_t)(-1) * xs + (int64_t)j * ys + k);
+  take(xh, 0, (int64_t)nx * xs + (int64_t)j * ys + k);
+  take(yl, 3, (int64_t)i * xs + (int64_t)(-1) * ys + k);
+  take(yh, 2, (int64_t)i * xs + (int64_t)ny * ys + k);
+  take(zl, 5, (int64_t)i * xs + (int64_t)j * ys - 1);
+  take(zh, 4, (int64_t)i * xs + (int64_t)j * ys + nz);
+  a[id] = acc;
+}
+
+// adjoint of the MAC fill on the normal faces [0,n]: a periodic high face
+// duplicates face 0 (adj(0) += adj(n)), a forced-inflow low face fixes face 0
+// (adj(0) dropped); face n is never a parameter (adj(n) = 0)
+template <typename T>
+__global__ void k_mac_adjoint_normal(T *__restrict__ a, int64_t s_n, int64_t s_a, int64_t s_b, int n, int na,
+                                     int nb, int per_hi, int inflow_lo) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= na * nb) return;
+  const int ia = q / nb, ib = q % nb;
+  T *p = a + (int64_t)ia * s_a + (int64_t)ib * s_b;
+  if (per_hi) p[0] += p[(int64_t)n * s_n];
+  if (inflow_lo) p[0] = (T)0;
+  p[(int64_t)n * s_n] = (T)0;
+}
+
 // Sol_LaplacianCentered3DDevice_stencil (sol_laplaciancent3ddev.cu:27-62)
 template <typename T>
 __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ phi, T *__restrict__ d, Inv<T> ih2, T coef,
@@ -267,6 +392,88 @@ int ocu_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3
                                                                   at0<float>(phi), at0<float>(dphi),
                                                                   inv<float>(hx, hy, hz), xs, ys, nx, ny, nz, scheme);
   OCU_HIP_LAUNCHED("ocu_passive_advect");
+}
+
+int ocu_passive_advect_adjoint(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *phi,
+                               const ocu_grid3d *lam, const ocu_grid3d *lam_out, const ocu_grid3d *adj_u,
+                               const ocu_grid3d *adj_v, const ocu_grid3d *adj_w, double scale, int nx, int ny, int nz,
+                               double hx, double hy, double hz, int scheme, void *stream) {
+  const char *what = "ocu_passive_advect_adjoint";
+  if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
+  if (int e = check_mac(adj_u, adj_v, adj_w, nx, ny, nz, what)) return e;
+  if (int e = check_same(u, adj_u, what)) return e;
+  if (int e = check_same(u, phi, what)) return e;
+  if (int e = check_same(u, lam, what)) return e;
+  if (int e = check_same(u, lam_out, what)) return e;
+  OCU_REQUIRE(lam->base != lam_out->base, "%s - lam_out must not alias lam", what);
+  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED,
+              "%s - invalid interpolation type %d", what, scheme);
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  const int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
+  const dim3 g((nz + 2 + TK - 1) / TK, (ny + 2 + TJ - 1) / TJ, nx + 2);
+  if (u->dtype == OCU_F64)
+    k_passive_advect_adjoint<double><<<g, stencil_block(), 0, S(stream)>>>(
+        at0<double>(u), at0<double>(v), at0<double>(w), at0<double>(phi), at0<double>(lam), at0<double>(lam_out),
+        at0<double>(adj_u), at0<double>(adj_v), at0<double>(adj_w), inv<double>(hx, hy, hz), (double)scale, xs, ys,
+        nx, ny, nz, scheme);
+  else
+    k_passive_advect_adjoint<float><<<g, stencil_block(), 0, S(stream)>>>(
+        at0<float>(u), at0<float>(v), at0<float>(w), at0<float>(phi), at0<float>(lam), at0<float>(lam_out),
+        at0<float>(adj_u), at0<float>(adj_v), at0<float>(adj_w), inv<float>(hx, hy, hz), (float)scale, xs, ys, nx,
+        ny, nz, scheme);
+  OCU_HIP_LAUNCHED(what);
+}
+
+int ocu_bc_adjoint_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, void *stream) {
+  const char *what = "ocu_bc_adjoint_nocorners";
+  if (int e = check_grid(g, what)) return e;
+  OCU_REQUIRE(bc, "%s: null bc", what);
+  OCU_REQUIRE(g->gx >= 1 && g->gy >= 1 && g->gz >= 1, "%s - must have at least 1 ghost point", what);
+  OCU_REQUIRE(g->dtype != OCU_I32, "%s: int grid", what);
+  const ocu_bc *all[6] = {&bc->xpos, &bc->xneg, &bc->ypos, &bc->yneg, &bc->zpos, &bc->zneg};
+  GhostCoef gc;
+  for (int f = 0; f < 6; f++) {
+    const int t = all[f]->type;
+    OCU_REQUIRE(t == OCU_BC_NONE || t == OCU_BC_PERIODIC || t == OCU_BC_DIRICHELET || t == OCU_BC_NEUMANN,
+                "%s - invalid boundary condition type %d", what, t);
+    gc.c[f] = t == OCU_BC_DIRICHELET ? -1.0f : t == OCU_BC_NONE ? 0.0f : 1.0f;
+    gc.per[f] = t == OCU_BC_PERIODIC;
+  }
+  if ((int64_t)g->nx * g->ny * g->nz == 0) return OCU_OK;
+  const int64_t xs = (int64_t)g->pny * g->pnz, ys = g->pnz;
+  const dim3 grid = stencil_grid(g->nx, g->ny, g->nz);
+  if (g->dtype == OCU_F64)
+    k_bc_adjoint_nocorners<double><<<grid, stencil_block(), 0, S(stream)>>>(at0<double>(g), gc, xs, ys, g->nx, g->ny,
+                                                                             g->nz);
+  else
+    k_bc_adjoint_nocorners<float><<<grid, stencil_block(), 0, S(stream)>>>(at0<float>(g), gc, xs, ys, g->nx, g->ny,
+                                                                            g->nz);
+  OCU_HIP_LAUNCHED(what);
+}
+
+int ocu_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc,
+                           void *stream) {
+  const char *what = "ocu_mac_adjoint_normal";
+  OCU_REQUIRE(u && v && w && bc, "%s: null argument", what);
+  const int nx = u->nx - 1, ny = u->ny, nz = u->nz;
+  if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  const int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
+  auto launch = [&](const ocu_grid3d *g, int64_t s_n, int64_t s_a, int64_t s_b, int n, int na, int nb, int per_hi,
+                    int inflow_lo) {
+    const int cnt = na * nb, nb_ = (cnt + 255) / 256;
+    if (g->dtype == OCU_F64)
+      k_mac_adjoint_normal<double><<<nb_, 256, 0, S(stream)>>>(at0<double>(g), s_n, s_a, s_b, n, na, nb, per_hi,
+                                                                inflow_lo);
+    else
+      k_mac_adjoint_normal<float><<<nb_, 256, 0, S(stream)>>>(at0<float>(g), s_n, s_a, s_b, n, na, nb, per_hi,
+                                                               inflow_lo);
+  };
+  const int IN = OCU_BC_FORCED_INFLOW_VARIABLE_SLIP, PER = OCU_BC_PERIODIC;
+  launch(u, xs, ys, 1, nx, ny, nz, bc->xpos.type == PER, bc->xneg.type == IN);
+  launch(v, ys, xs, 1, ny, nx, nz, bc->ypos.type == PER, bc->yneg.type == IN);
+  launch(w, 1, xs, ys, nz, nx, ny, bc->zpos.type == PER, bc->zneg.type == IN);
+  OCU_HIP_LAUNCHED(what);
 }
 
 int ocu_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz, double hx, double hy,

@@ -101,6 +101,7 @@ bool fill_params(Eqn_IncompressibleNS3DParams<T> &P, const ocu_ns_params *p, con
   P.advection_scheme = (InterpolationType)p->advection_scheme;
   P.time_step = (TimeStepType)p->time_step;
   P.cfl_factor = p->cfl_factor;
+  P.mixed_precision_projection = p->mixed_projection != 0;
   size_t n = (size_t)P.init_u.num_allocated_elements() * sizeof(T);
   memcpy(P.init_u.buffer(), iu, n);
   memcpy(P.init_v.buffer(), iv, n);

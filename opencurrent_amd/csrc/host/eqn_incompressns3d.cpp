@@ -67,7 +67,7 @@ bool Eqn_IncompressibleNS3DBase<T>::set_base_parameters(const Eqn_Incompressible
 }
 
 template <typename T>
-Eqn_IncompressibleNS3D<T>::Eqn_IncompressibleNS3D() {}
+Eqn_IncompressibleNS3D<T>::Eqn_IncompressibleNS3D() : _mixed_projection(false) {}
 
 // eqn_incompressns3d.cu:119-155: the vertical component gets
 // .5 * (dirmult * g * beta) * (T + T_{-vertical})
@@ -126,8 +126,21 @@ bool Eqn_IncompressibleNS3D<T>::set_parameters(const Eqn_IncompressibleNS3DParam
     printf("[ERROR] Eqn_IncompressibleNS3D::set_parameters - failed on _advection_solver initialization\n");
     return false;
   }
-  if (!_projection_solver.initialize_storage(this->_nx, this->_ny, this->_nz, this->_hx, this->_hy, this->_hz, &_u,
-                                             &_v, &_w)) {
+  _mixed_projection = params.mixed_precision_projection;
+  if (_mixed_projection) {
+    if constexpr (std::is_same<T, double>::value) {
+      _mixed_projection_solver.bc = params.flow_bc;
+      if (!_mixed_projection_solver.initialize_storage(this->_nx, this->_ny, this->_nz, this->_hx, this->_hy,
+                                                       this->_hz, &_u, &_v, &_w)) {
+        printf("[ERROR] Eqn_IncompressibleNS3D::set_parameters - failed on _mixed_projection_solver initialization\n");
+        return false;
+      }
+    } else {
+      printf("[ERROR] Eqn_IncompressibleNS3D::set_parameters - mixed-precision projection needs T = double\n");
+      return false;
+    }
+  } else if (!_projection_solver.initialize_storage(this->_nx, this->_ny, this->_nz, this->_hx, this->_hy, this->_hz,
+                                                    &_u, &_v, &_w)) {
     printf("[ERROR] Eqn_IncompressibleNS3D::set_parameters - failed on _projection_solver initialization\n");
     return false;
   }
@@ -254,6 +267,12 @@ bool Eqn_IncompressibleNS3D<T>::advance_one_step(double dt) {
     _last_deriv_udt.copy_all_data(_deriv_udt);
     _last_deriv_vdt.copy_all_data(_deriv_vdt);
     _last_deriv_wdt.copy_all_data(_deriv_wdt);
+  }
+  if constexpr (std::is_same<T, double>::value) {
+    if (_mixed_projection) {
+      this->check_ok(_mixed_projection_solver.solve(this->_max_divergence));
+      return !this->any_error();
+    }
   }
   this->check_ok(_projection_solver.solve(this->_max_divergence));
   return !this->any_error();

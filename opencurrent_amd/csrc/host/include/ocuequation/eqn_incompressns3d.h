@@ -12,6 +12,8 @@
 #include "ocuutil/direction.h"
 #include "ocuutil/timestep.h"
 
+#include <type_traits>
+
 namespace ocu {
 
 template <typename T>
@@ -32,6 +34,9 @@ class Eqn_IncompressibleNS3DParams : public Parameters {
   InterpolationType advection_scheme;
   TimeStepType time_step;
   double cfl_factor;
+  // not in the reference: project with FP32 V-cycles inside FP64 iterative
+  // refinement (Sol_ProjectDivergenceMixed3DDeviceD); T = double only
+  bool mixed_precision_projection;
 
   // defaults of eqn_incompressns3d.h:59-74
   Eqn_IncompressibleNS3DParams() {
@@ -46,6 +51,7 @@ class Eqn_IncompressibleNS3DParams : public Parameters {
     bouyancy = 1;
     time_step = TS_FORWARD_EULER;
     vertical_direction = DIR_YPOS;
+    mixed_precision_projection = false;
   }
 
   // eqn_incompressns3d.h:78-86: paddings (0,1,1), (1,0,1), (1,1,0), (1,1,1)
@@ -94,6 +100,8 @@ template <typename T>
 class Eqn_IncompressibleNS3D : public Eqn_IncompressibleNS3DBase<T> {
  protected:
   Sol_ProjectDivergence3DDevice<T> _projection_solver;
+  Sol_ProjectDivergenceMixed3DDeviceD _mixed_projection_solver;  // used when _mixed_projection
+  bool _mixed_projection;
   Sol_SelfAdvection3DDevice<T> _advection_solver;
   Sol_PassiveAdvection3DDevice<T> _thermal_solver;
   Sol_LaplacianCentered3DDevice<T> _thermal_diffusion;
@@ -124,7 +132,12 @@ class Eqn_IncompressibleNS3D : public Eqn_IncompressibleNS3DBase<T> {
   const Grid3DDevice<T> &get_v() const { return _v; }
   const Grid3DDevice<T> &get_w() const { return _w; }
   const Grid3DDevice<T> &get_temperature() const { return _temp; }
-  const Grid3DDevice<T> &get_pressure() const { return _projection_solver.pressure_solver.pressure(); }
+  const Grid3DDevice<T> &get_pressure() const {
+    if constexpr (std::is_same<T, double>::value)
+      if (_mixed_projection) return _mixed_projection_solver.pressure_solver.pressure();
+    return _projection_solver.pressure_solver.pressure();
+  }
+  bool mixed_projection() const { return _mixed_projection; }
   Sol_ProjectDivergence3DDevice<T> &projection_solver() { return _projection_solver; }
 };
 

@@ -191,6 +191,10 @@ def grid_copy(dst, src):
     _lib.check(_lib.hip().ocu_grid_copy(dst.ptr(), src.ptr(), None), "grid_copy")
 
 
+def grid_clear_ghosts(g):
+    _lib.check(_lib.hip().ocu_grid_clear_ghosts(g.ptr(), None), "grid_clear_ghosts")
+
+
 def grid_clear(g, value=0.0):
     _lib.check(_lib.hip().ocu_grid_clear(g.ptr(), value, None), "grid_clear")
 
@@ -213,6 +217,20 @@ def self_advect(u, v, w, du, dv, dw, nx, ny, nz, hx, hy, hz, scheme):
 def passive_advect(u, v, w, phi, dphi, nx, ny, nz, hx, hy, hz, scheme):
     _lib.check(_lib.hip().ocu_passive_advect(u.ptr(), v.ptr(), w.ptr(), phi.ptr(), dphi.ptr(), nx, ny, nz,
                                              hx, hy, hz, scheme, None), "passive_advect")
+
+
+def passive_advect_adjoint(u, v, w, phi, lam, lam_out, au, av, aw, scale, nx, ny, nz, hx, hy, hz, scheme):
+    _lib.check(_lib.hip().ocu_passive_advect_adjoint(u.ptr(), v.ptr(), w.ptr(), phi.ptr(), lam.ptr(), lam_out.ptr(),
+                                                     au.ptr(), av.ptr(), aw.ptr(), scale, nx, ny, nz, hx, hy, hz,
+                                                     scheme, None), "passive_advect_adjoint")
+
+
+def bc_adjoint_nocorners(g, bc):
+    _lib.check(_lib.hip().ocu_bc_adjoint_nocorners(g.ptr(), C.byref(bc), None), "bc_adjoint_nocorners")
+
+
+def mac_adjoint_normal(u, v, w, bc):
+    _lib.check(_lib.hip().ocu_mac_adjoint_normal(u.ptr(), v.ptr(), w.ptr(), C.byref(bc), None), "mac_adjoint_normal")
 
 
 def laplacian_acc(phi, dphi, nx, ny, nz, hx, hy, hz, coef):

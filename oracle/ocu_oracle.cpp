@@ -290,3 +290,27 @@ int orc_scalar_advect_step(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_g
   return ok ? 1 : 0;
 }
 
+
+int orc_passive_advect_adjoint(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                               const ocu_grid3d *phi, const ocu_grid3d *lam, const ocu_grid3d *lam_out,
+                               const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const ocu_grid3d *adj_w,
+                               double scale, int nx, int ny, int nz, double hx, double hy, double hz,
+                               int scheme) {
+  if (!u || !v || !w || !phi || !lam || !lam_out || !adj_u || !adj_v || !adj_w) return -1;
+  return DISPATCH(u->dtype,
+                  (passive_advect_adjoint<float>(*u, *v, *w, *phi, *lam, *lam_out, *adj_u, *adj_v, *adj_w, scale, nx,
+                                                 ny, nz, hx, hy, hz, scheme), 1),
+                  (passive_advect_adjoint<double>(*u, *v, *w, *phi, *lam, *lam_out, *adj_u, *adj_v, *adj_w, scale,
+                                                  nx, ny, nz, hx, hy, hz, scheme), 1));
+}
+
+int orc_bc_adjoint_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc) {
+  if (!g || !bc) return -1;
+  return DISPATCH(g->dtype, (bc_adjoint_nocorners<float>(*g, *bc), 1), (bc_adjoint_nocorners<double>(*g, *bc), 1));
+}
+
+int orc_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc) {
+  if (!u || !v || !w || !bc) return -1;
+  return DISPATCH(u->dtype, (mac_adjoint_normal<float>(*u, *v, *w, *bc), 1),
+                  (mac_adjoint_normal<double>(*u, *v, *w, *bc), 1));
+}

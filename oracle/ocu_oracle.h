@@ -66,6 +66,16 @@ int orc_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int
 int orc_thermal_force(const ocu_grid3d *d, const ocu_grid3d *t, int nx, int ny, int nz,
                       int vertical_direction, double coefficient);
 
+/* Adjoints (no reference counterpart): see oracle_ns.h.
+ * lam_out = lam + A_phi^T(scale*lam); adj_u/v/w += A_u^T(scale*lam). */
+int orc_passive_advect_adjoint(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                               const ocu_grid3d *phi, const ocu_grid3d *lam, const ocu_grid3d *lam_out,
+                               const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const ocu_grid3d *adj_w,
+                               double scale, int nx, int ny, int nz, double hx, double hy, double hz,
+                               int scheme);
+int orc_bc_adjoint_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc);
+int orc_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc);
+
 /* One projection, Sol_ProjectDivergence3DDevice<T>::solve(tol) on a fresh
  * solver (zero pressure guess).  pressure_out (congruent with u) receives p. */
 int orc_project(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc,

@@ -143,6 +143,149 @@ void passive_advect(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d
       }
 }
 
+/* ---- adjoints (no reference counterpart; derived from passive_advect) ----
+ *
+ * passive_advect is dphi_c = -sum_axes (F_{c+1} - F_c) / h with the face
+ * flux F_f = u_f * I(u_f, phi_{f-1}, phi_f); for fixed upwind selection it is
+ * linear in phi and in u.  With lam the adjoint of dphi (interior; the ghost
+ * layer of lam must be zero) and g_f = scale * ((lam_f - lam_{f-1}) / h):
+ *   adj_u_f   += g_f * I(u_f, phi_{f-1}, phi_f)
+ *   adj_phi_c += sum over the faces of c of (g_f * u_f) * dI/dphi_c
+ * (dI/dx0 = u<0 ? 0 : 1, dI/dx1 = u<0 ? 1 : 0 upwind; .5 centred).
+ * lam_out = lam_in + adj_phi over the whole ghost box (the identity part of
+ * phi_{n+1} = phi_n + dt*dphi with scale = dt); adj_u/v/w accumulate at the
+ * faces [0,n] whose other two indices are interior. */
+template <typename T>
+inline T interp_w0(int scheme, T u) {
+  if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return (T).5;
+  return u < 0 ? (T)0 : (T)1;
+}
+template <typename T>
+inline T interp_w1(int scheme, T u) {
+  if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return (T).5;
+  return u < 0 ? (T)1 : (T)0;
+}
+
+template <typename T>
+void passive_advect_adjoint(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
+                            const ocu_grid3d &phig, const ocu_grid3d &lamg, const ocu_grid3d &outg,
+                            const ocu_grid3d &aug, const ocu_grid3d &avg, const ocu_grid3d &awg, double scaled,
+                            int nx, int ny, int nz, double hx, double hy, double hz, int scheme) {
+  View<T> u(ug), v(vg), w(wg), phi(phig), lam(lamg), out(outg), au(aug), av(avg), aw(awg);
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz), sc = (T)scaled;
+#pragma omp parallel for schedule(static)
+  for (int i = -1; i <= nx; i++)
+    for (int j = -1; j <= ny; j++)
+      for (int k = -1; k <= nz; k++) {
+        const bool ii = i >= 0 && i < nx, ji = j >= 0 && j < ny, ki = k >= 0 && k < nz;
+        T acc = 0;
+        if (ji && ki) {
+          if (i >= 0) {  // face i (cell c is its upper side)
+            T g = sc * (ix * (lam.at(i, j, k) - lam.at(i - 1, j, k)));
+            T f = u.at(i, j, k);
+            acc += (g * f) * interp_w1<T>(scheme, f);
+            au.at(i, j, k) += g * interp<T>(scheme, f, phi.at(i - 1, j, k), phi.at(i, j, k));
+          }
+          if (i < nx) {  // face i+1 (cell c is its lower side)
+            T g = sc * (ix * (lam.at(i + 1, j, k) - lam.at(i, j, k)));
+            T f = u.at(i + 1, j, k);
+            acc += (g * f) * interp_w0<T>(scheme, f);
+          }
+        }
+        if (ii && ki) {
+          if (j >= 0) {
+            T g = sc * (iy * (lam.at(i, j, k) - lam.at(i, j - 1, k)));
+            T f = v.at(i, j, k);
+            acc += (g * f) * interp_w1<T>(scheme, f);
+            av.at(i, j, k) += g * interp<T>(scheme, f, phi.at(i, j - 1, k), phi.at(i, j, k));
+          }
+          if (j < ny) {
+            T g = sc * (iy * (lam.at(i, j + 1, k) - lam.at(i, j, k)));
+            T f = v.at(i, j + 1, k);
+            acc += (g * f) * interp_w0<T>(scheme, f);
+          }
+        }
+        if (ii && ji) {
+          if (k >= 0) {
+            T g = sc * (iz * (lam.at(i, j, k) - lam.at(i, j, k - 1)));
+            T f = w.at(i, j, k);
+            acc += (g * f) * interp_w1<T>(scheme, f);
+            aw.at(i, j, k) += g * interp<T>(scheme, f, phi.at(i, j, k - 1), phi.at(i, j, k));
+          }
+          if (k < nz) {
+            T g = sc * (iz * (lam.at(i, j, k + 1) - lam.at(i, j, k)));
+            T f = w.at(i, j, k + 1);
+            acc += (g * f) * interp_w0<T>(scheme, f);
+          }
+        }
+        out.at(i, j, k) = lam.at(i, j, k) + acc;
+      }
+}
+
+/* Adjoint of bc_fill_nocorners_device: every face ghost is c * (interior
+ * source) + const with c = 1 (periodic copy, Neumann) or -1 (Dirichlet), so
+ * each interior cell gathers c * adj(ghost) from the face ghosts derived from
+ * it (x, then y, then z; low face then high face), and the face ghosts'
+ * adjoints are zeroed.  BC_NONE ghosts are constants: their adjoint is
+ * dropped. */
+template <typename T>
+void bc_adjoint_nocorners(const ocu_grid3d &g, const ocu_bc_set &bc) {
+  View<T> a(g);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  auto coef = [](const ocu_bc &b) -> T {
+    return b.type == OCU_BC_DIRICHELET ? (T)-1 : (b.type == OCU_BC_NONE ? (T)0 : (T)1);
+  };
+  auto per = [](const ocu_bc &b) { return b.type == OCU_BC_PERIODIC; };
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        T acc = a.at(i, j, k);
+        // ghost (-1) derives from 0 (or nx-1 if periodic), ghost (n) from n-1 (or 0)
+        if (i == (per(bc.xneg) ? nx - 1 : 0)) acc += coef(bc.xneg) * a.at(-1, j, k);
+        if (i == (per(bc.xpos) ? 0 : nx - 1)) acc += coef(bc.xpos) * a.at(nx, j, k);
+        if (j == (per(bc.yneg) ? ny - 1 : 0)) acc += coef(bc.yneg) * a.at(i, -1, k);
+        if (j == (per(bc.ypos) ? 0 : ny - 1)) acc += coef(bc.ypos) * a.at(i, ny, k);
+        if (k == (per(bc.zneg) ? nz - 1 : 0)) acc += coef(bc.zneg) * a.at(i, j, -1);
+        if (k == (per(bc.zpos) ? 0 : nz - 1)) acc += coef(bc.zpos) * a.at(i, j, nz);
+        a.at(i, j, k) = acc;
+      }
+  for (int j = 0; j < ny; j++)
+    for (int k = 0; k < nz; k++) a.at(-1, j, k) = a.at(nx, j, k) = 0;
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz; k++) a.at(i, -1, k) = a.at(i, ny, k) = 0;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) a.at(i, j, -1) = a.at(i, j, nz) = 0;
+}
+
+/* Adjoint of the normal-face part of apply_3d_mac_boundary_conditions_level1
+ * for the faces an advection reads (the normal components at [0,n]):
+ * a periodic high face duplicates face 0 at face n, so adj(0) += adj(n); a
+ * forced-inflow low face sets face 0 to the boundary value, so adj(0) is
+ * dropped.  Afterwards adj(n) = 0 in both cases (face n is not a parameter). */
+template <typename T>
+void mac_adjoint_normal(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg, const ocu_bc_set &bc) {
+  View<T> u(ug), v(vg), w(wg);
+  const int nx = ug.nx - 1, ny = ug.ny, nz = ug.nz;
+  for (int j = 0; j < ny; j++)
+    for (int k = 0; k < nz; k++) {
+      if (bc.xpos.type == OCU_BC_PERIODIC) u.at(0, j, k) += u.at(nx, j, k);
+      if (bc.xneg.type == OCU_BC_FORCED_INFLOW_VARIABLE_SLIP) u.at(0, j, k) = 0;
+      u.at(nx, j, k) = 0;
+    }
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz; k++) {
+      if (bc.ypos.type == OCU_BC_PERIODIC) v.at(i, 0, k) += v.at(i, ny, k);
+      if (bc.yneg.type == OCU_BC_FORCED_INFLOW_VARIABLE_SLIP) v.at(i, 0, k) = 0;
+      v.at(i, ny, k) = 0;
+    }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      if (bc.zpos.type == OCU_BC_PERIODIC) w.at(i, j, 0) += w.at(i, j, nz);
+      if (bc.zneg.type == OCU_BC_FORCED_INFLOW_VARIABLE_SLIP) w.at(i, j, 0) = 0;
+      w.at(i, j, nz) = 0;
+    }
+}
+
 /* Sol_LaplacianCentered3DDevice_stencil, sol_laplaciancent3ddev.cu:27-62:
  * dphi += c * (z, y, x second differences), 1/h^2 rounded to T (:100-102). */
 template <typename T>

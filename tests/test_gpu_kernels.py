@@ -462,6 +462,55 @@ def test_advection(gpu, dtype, scheme, size):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("scheme", [T.IT_FIRST_ORDER_UPWIND, T.IT_SECOND_ORDER_CENTERED])
+@pytest.mark.parametrize("size", NS_SIZES)
+def test_advection_adjoint(gpu, dtype, scheme, size):
+    """the adjoint kernels against the oracle's (pinned on the CPU by the
+    dot-product identity, tests/test_oracle_adjoint.py), bit for bit"""
+    nx, ny, nz = size
+    u, v, w, t = mac_grids(nx, ny, nz, dtype)
+    for q, g in enumerate((u, v, w, t)):
+        g.buf[:] = (2 * uniform(120 + q, g.buf.shape) - 1).astype(dtype)
+    lam = HostGrid.congruent(t)
+    lam.interior()[:] = (2 * uniform(125, lam.interior().shape) - 1).astype(dtype)
+    au, av, aw, _ = mac_grids(nx, ny, nz, dtype)
+    for q, g in enumerate((au, av, aw)):
+        g.buf[:] = (2 * uniform(126 + q, g.buf.shape) - 1).astype(dtype)
+    out = HostGrid.congruent(t)
+    hx, hy, hz = .5, .25, .125
+    g_au, g_av, g_aw = (D().DeviceGrid.from_host(x) for x in (au, av, aw))
+    O.lib().orc_passive_advect_adjoint(u.ptr(), v.ptr(), w.ptr(), t.ptr(), lam.ptr(), out.ptr(), au.ptr(), av.ptr(),
+                                       aw.ptr(), .03125, nx, ny, nz, hx, hy, hz, scheme)
+    du, dv, dw, dt, dl = (D().DeviceGrid.from_host(x) for x in (u, v, w, t, lam))
+    dout = D().DeviceGrid.from_host(HostGrid.congruent(t))
+    D().passive_advect_adjoint(du, dv, dw, dt, dl, dout, g_au, g_av, g_aw, .03125, nx, ny, nz, hx, hy, hz, scheme)
+    for got, want, n in ((dout, out, "lam_out"), (g_au, au, "adj_u"), (g_av, av, "adj_v"), (g_aw, aw, "adj_w")):
+        assert_bitexact(got.to_host(), want, f"advection adjoint {n}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", ["neumann_v", "dirichlet", "mixed", "periodic", "none_faces"])
+def test_ghost_adjoints(gpu, dtype, case):
+    bc = BC_CASES[case]
+    g = random_grid(9, 7, 13, dtype, 130)
+    want = g.copy()
+    O.lib().orc_bc_adjoint_nocorners(want.ptr(), C.byref(bc))
+    dg = D().DeviceGrid.from_host(g)
+    D().bc_adjoint_nocorners(dg, bc)
+    assert_bitexact(dg.to_host(), want, f"ghost adjoint {case}")
+    mbc = mac_bc("inflow") if case == "mixed" else MAC_CASES["periodic"]
+    u, v, w, _ = mac_grids(8, 6, 10, dtype)
+    for q, x in enumerate((u, v, w)):
+        x.buf[:] = (2 * uniform(131 + q, x.buf.shape) - 1).astype(dtype)
+    wu, wv, ww = u.copy(), v.copy(), w.copy()
+    O.lib().orc_mac_adjoint_normal(wu.ptr(), wv.ptr(), ww.ptr(), C.byref(mbc))
+    du, dv, dw = (D().DeviceGrid.from_host(x) for x in (u, v, w))
+    D().mac_adjoint_normal(du, dv, dw, mbc)
+    for got, want_, n in ((du, wu, "u"), (dv, wv, "v"), (dw, ww, "w")):
+        assert_bitexact(got.to_host(), want_, f"mac adjoint {n}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_laplacian_thermal(gpu, dtype):
     nx, ny, nz = 16, 12, 20
     u, v, w, t = mac_grids(nx, ny, nz, dtype)

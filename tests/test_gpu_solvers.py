@@ -387,3 +387,32 @@ def test_ns_steps_match_oracle(gpu, dtype, scheme, ts):
             assert np.array_equal(got.buf, want.buf), f"field {which}: {np.abs(got.buf - want.buf).max()}"
     finally:
         O.lib().orc_ns_destroy(ho)
+
+
+def test_ns_mixed_projection_tracks_fp64(gpu):
+    """FP64 Rayleigh-Benard steps with the projection's pressure solved by FP32
+    V-cycles in FP64 refinement stay within the divergence tolerance of the
+    all-FP64 run."""
+    n = (16, 16, 16)
+    runs = []
+    for mixed in (0, 1):
+        p = ns_params(n, np.float64)
+        p.mixed_projection = mixed
+        u, v, w, t = rb_initial(n, np.float64)
+        ns = S().IncompressibleNS3D(p, u, v, w, t)
+        for step in range(5):
+            assert ns.advance_one_step(ns.get_max_stable_timestep())
+        runs.append([ns.field(q).interior().copy() for q in range(4)])
+    for q in range(4):
+        a, b = runs[0][q], runs[1][q]
+        assert np.abs(a - b).max() <= 1e-6 * max(1.0, np.abs(a).max()), q
+
+
+def test_ns_mixed_projection_needs_fp64(gpu):
+    from opencurrent_amd._lib import OcuError
+    n = (8, 8, 8)
+    p = ns_params(n, np.float32)
+    p.mixed_projection = 1
+    u, v, w, t = rb_initial(n, np.float32)
+    with pytest.raises(OcuError):
+        S().IncompressibleNS3D(p, u, v, w, t)
