@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--no-extra", action="store_true", help="skip the C1/C3/C4 side measurements")
     return ap.parse_args()
 
 
@@ -101,6 +102,115 @@ def time_fused_sweep(n, reps, dtype=np.float32):
     w = np.dtype(dtype).itemsize
     alg_bytes = 3 * w * n ** 3  # read u and b, write u, once per cell
     return ms, alg_bytes
+
+
+def time_fp64_mg_64(hip, stream):
+    """config C1: plain FP64 multigrid on the 64^3 Neumann problem to 1e-8
+    relative residual (the reference algorithm, FMG)"""
+    from opencurrent_amd import _types as T
+    from opencurrent_amd import device, solvers
+    n = 64
+    b = rhs(n)
+    rms = float(np.sqrt((b.interior() ** 2).mean()))
+    gb = device.DeviceGrid.from_host(b)
+    mg = solvers.MultigridPressure3D(gb, 1 / n, 1 / n, 1 / n, T.BCSet.uniform(T.BC_NEUMANN))
+    pd = mg.pressure_desc()
+    ev = Events(hip, stream)
+    times = []
+    for rep in range(6):
+        hip.ocu_grid_clear(C.byref(pd), 0.0, stream)
+        ev.start()
+        ok, res = mg.solve(1e-8 * rms, 20)
+        ms = ev.stop_ms()
+        if not ok:
+            raise RuntimeError("C1 solve did not converge")
+        if rep:
+            times.append(ms)
+    ms = float(np.median(times))
+    return {"workload": "FP64 MG 64^3 Neumann to 1e-8 rel. residual", "ms": round(ms, 4),
+            "mcells_s": round(n ** 3 / (ms / 1e3) / 1e6, 2), "vcycles": mg.stats()["vcycles"]}
+
+
+def time_ns_step(steps=10, warmup=2, n=128):
+    """config C3: Boussinesq Rayleigh-Benard at n^3 (periodic x/y, no-slip z,
+    Ra = 1e5, Pr = 0.71, upwind, CFL 0.5, forward Euler), FP64 with the FP64
+    multigrid projection and FP64 with the FP32-refined projection"""
+    import fields
+    from opencurrent_amd import _types as T
+    from opencurrent_amd import solvers
+    from opencurrent_amd._lib import NSParams
+    out = {}
+    for name, mixed in (("fp64", 0), ("fp32_refined", 1)):
+        p = NSParams()
+        p.nx = p.ny = p.nz = n
+        p.hx = p.hy = p.hz = 1.0 / n
+        p.max_divergence = 1e-8
+        tb = T.BCSet.uniform(T.BC_PERIODIC)
+        tb.zneg = T.BC(T.BC_DIRICHELET, 1.0, 0.0)
+        tb.zpos = T.BC(T.BC_DIRICHELET, 0.0, 0.0)
+        p.temp_bc = tb
+        fb = T.BCSet.uniform(T.BC_PERIODIC)
+        fb.zneg = T.BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0, 1.0)
+        fb.zpos = T.BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0, 1.0)
+        p.flow_bc = fb
+        p.vertical_direction = T.DIR_ZPOS
+        p.gravity, p.bouyancy, p.viscosity, p.thermal_diffusion = -1.0, 1e5 * .71, .71, 1.0
+        p.advection_scheme, p.time_step, p.cfl_factor = T.IT_FIRST_ORDER_UPWIND, T.TS_FORWARD_EULER, .5
+        p.mixed_projection = mixed
+        u, v, w, t = fields.mac_grids(n, n, n)
+        z = (np.arange(n) + .5) / n
+        t.interior()[:] = (1 - z)[None, None, :] + 1e-3 * (2 * fields.uniform(2001, t.interior().shape) - 1)
+        ns = solvers.IncompressibleNS3D(p, u, v, w, t)
+        for _ in range(warmup):
+            ns.advance_one_step(ns.get_max_stable_timestep())
+        import torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if not ns.advance_one_step(ns.get_max_stable_timestep()):
+                raise RuntimeError(f"NS step failed ({name})")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[name] = {"ms_per_step": round(dt * 1e3, 3), "mcells_s": round(n ** 3 / dt / 1e6, 2)}
+        del ns
+    out["workload"] = f"Rayleigh-Benard {n}^3, Ra 1e5, Pr 0.71, upwind, CFL 0.5, FE, projection to 1e-8 Linf"
+    out["speedup_fp32_refined"] = round(out["fp64"]["ms_per_step"] / out["fp32_refined"]["ms_per_step"], 3)
+    return out
+
+
+def time_adjoint(n=128, steps=20):
+    """config C4: 20 passive-advection steps after a projection, loss vs a
+    target blob, gradient w.r.t. the initial velocity with checkpoints every
+    5 steps (forward + backward, FP32)"""
+    import torch
+    import fields
+    from opencurrent_amd import adjoint
+    h = 1.0 / n
+    p = adjoint.AdvectProjectProblem(n, n, n, h, h, h, dt=.25 * h, steps=steps, dtype=np.float32,
+                                     checkpoint_every=5)
+    x = (np.arange(n) + .5) * h
+    r2 = (x[:, None, None] - .4) ** 2 + (x[None, :, None] - .5) ** 2 + (x[None, None, :] - .6) ** 2
+    r2t = (x[:, None, None] - .6) ** 2 + (x[None, :, None] - .45) ** 2 + (x[None, None, :] - .5) ** 2
+    p.set_scalars(np.exp(-r2 / .02), np.exp(-r2t / .02))
+    vel = []
+    for q, g in enumerate((p.u, p.v, p.w)):
+        t = torch.as_tensor(2 * fields.uniform(1001 + q, tuple(g.tensor.shape)) - 1, dtype=g.tensor.dtype,
+                            device=g.tensor.device)
+        t[~p.masks[q]] = 0
+        vel.append(t)
+    p.forward(*vel)
+    p.backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = p.forward(*vel)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    p.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"workload": f"passive advection {n}^3 x {steps} steps + projection, gradient w.r.t. initial velocity, "
+                        f"checkpoints every 5, FP32",
+            "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2), "loss": loss}
 
 
 def cpu_baseline(n, rel_tol, budget_s=25.0):
@@ -245,6 +355,11 @@ def main():
                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                        "kernel": "k_relax_fused_rows<float,4,2,10> (level-0 red+black sweep with ghost writes)",
                        "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes}
+
+    if not args.no_extra:
+        out["c1_fp64_mg_64"] = time_fp64_mg_64(hip, stream)
+        out["c3_ns_step"] = time_ns_step()
+        out["c4_adjoint"] = time_adjoint()
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.rel_tol)

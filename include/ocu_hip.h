@@ -37,6 +37,9 @@ int ocu_free(void *ptr);
 /* pinned host memory (hipHostMalloc), host_malloc(pinned=true) in memory.cu */
 int ocu_host_malloc(void **ptr, size_t bytes);
 int ocu_host_free(void *ptr);
+/* device address of ocu_host_malloc memory (kernels write results there
+ * directly; read them after synchronising the stream) */
+int ocu_host_device_pointer(void **dev, void *host);
 /* kind: 0 host->device, 1 device->host, 2 device->device, 3 host->host.
  * Asynchronous on `stream` (cudaMemcpy in grid3ddev.cu:205, :245-265). */
 int ocu_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void *stream);
@@ -201,6 +204,9 @@ int ocu_ir_residual(const ocu_grid3d *x, const ocu_grid3d *b, double bshift,
                     const ocu_grid3d *r32, double scale, double h, double fx, double fy,
                     double fz, double *ws, void *stream);
 int ocu_norm3_finalize(const double *ws, double *out, void *stream);
+/* the same triples for an FP64 grid itself (sum g^2, max|g|, sum g over
+ * the interior): the three statistics of the right-hand side in one pass */
+int ocu_grid_stats3(const ocu_grid3d *g, double *ws, void *stream);
 /* x += (double)e * inv_scale over the interior (x FP64, e FP32) */
 int ocu_ir_correct(const ocu_grid3d *x, const ocu_grid3d *e, double inv_scale, void *stream);
 /* out[0] = interior sum of g (double) via ws (OCU_NORM_PARTS doubles) */

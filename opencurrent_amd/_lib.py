@@ -47,6 +47,7 @@ HIP_SIGS = {
     "ocu_free": (I, [P]),
     "ocu_host_malloc": (I, [C.POINTER(P), C.c_size_t]),
     "ocu_host_free": (I, [P]),
+    "ocu_host_device_pointer": (I, [C.POINTER(P), P]),
     "ocu_memcpy_async": (I, [P, P, C.c_size_t, I, P]),
     "ocu_stream_create": (I, [C.POINTER(P)]),
     "ocu_stream_destroy": (I, [P]),
@@ -80,6 +81,7 @@ HIP_SIGS = {
     "ocu_mg_prolong": (I, [GP, GP, I, P]),
     "ocu_mg_prolong_bc": (I, [GP, GP, I, BP, D, D, D, P]),
     "ocu_grid_clear_ghosts": (I, [GP, P]),
+    "ocu_grid_stats3": (I, [GP, DP, P]),
     "ocu_passive_advect_adjoint": (I, [GP, GP, GP, GP, GP, GP, GP, GP, GP, D, I, I, I, D, D, D, I, P]),
     "ocu_bc_adjoint_nocorners": (I, [GP, BP, P]),
     "ocu_mac_adjoint_normal": (I, [GP, GP, GP, BP, P]),

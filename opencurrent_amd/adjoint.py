@@ -64,7 +64,8 @@ class AdvectProjectProblem:
         self.h = (hx, hy, hz)
         self.dt, self.steps, self.scheme, self.ce = float(dt), int(steps), int(scheme), int(checkpoint_every)
         self.dtype = np.dtype(dtype)
-        self.tol = proj_tol if proj_tol is not None else (1e-12 if self.dtype == np.float64 else 1e-5)
+        # max |div| after the projection: FP32 multigrid reaches ~1e-3 on O(1/h) divergences
+        self.tol = proj_tol if proj_tol is not None else (1e-12 if self.dtype == np.float64 else 1e-3)
         self.flow_bc = BCSet.uniform(T.BC_PERIODIC)
         self.phi_bc = BCSet.uniform(T.BC_PERIODIC)
         self.u, self.v, self.w, self.phi = _mac(nx, ny, nz, dtype)

@@ -49,6 +49,36 @@ __global__ __launch_bounds__(256) void k_ir_residual(DGrid<double> x, DGrid<doub
   }
 }
 
+// interior (sum g^2, max|g|, sum g) partials per block, for ocu_norm3_finalize
+__global__ __launch_bounds__(256) void k_stats3(DGrid<double> g, int tiles_k, int tiles_j, int ntiles,
+                                                double *parts) {
+  double s2 = 0, mx = 0, sm = 0;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int tk = t % tiles_k, rem = t / tiles_k, tj = rem % tiles_j, i = rem / tiles_j;
+    int k = tk * TK + threadIdx.x;
+    int j = tj * TJ + threadIdx.y;
+    if (j >= g.ny || k >= g.nz) continue;
+    const double v = g.p[g.idx(i, j, k)];
+    s2 += v * v;
+    mx = fmax(mx, fabs(v));
+    sm += v;
+  }
+  __shared__ double sh[3][4];
+  int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  s2 = wave_sum(s2);
+  mx = wave_max(mx);
+  sm = wave_sum(sm);
+  if ((tid & 63) == 0) { sh[0][tid >> 6] = s2; sh[1][tid >> 6] = mx; sh[2][tid >> 6] = sm; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, m = 0, q = 0;
+    for (int w = 0; w < 4; w++) { a += sh[0][w]; m = fmax(m, sh[1][w]); q += sh[2][w]; }
+    parts[3 * blockIdx.x] = a;
+    parts[3 * blockIdx.x + 1] = m;
+    parts[3 * blockIdx.x + 2] = q;
+  }
+}
+
 __global__ void k_norm3_finalize(const double *parts, int n, double *out) {
   __shared__ double sh[3][8];
   double s = 0, m = 0, q = 0;
@@ -102,6 +132,16 @@ int ocu_ir_residual(const ocu_grid3d *x, const ocu_grid3d *b, double bshift, con
   k_ir_residual<<<NORM_PARTS, stencil_block(), 0, S(stream)>>>(dgrid<double>(x), dgrid<double>(b), bshift,
                                                                 dgrid<float>(r32), scale, c, tiles_k, tiles_j, ntiles, ws);
   OCU_HIP_LAUNCHED("ocu_ir_residual");
+}
+
+int ocu_grid_stats3(const ocu_grid3d *g, double *ws, void *stream) {
+  if (int e = check_grid(g, "ocu_grid_stats3")) return e;
+  OCU_REQUIRE(g->dtype == OCU_F64, "ocu_grid_stats3: FP64 grids only");
+  OCU_REQUIRE(ws, "ocu_grid_stats3: null workspace");
+  const int tiles_k = (g->nz + TK - 1) / TK, tiles_j = (g->ny + TJ - 1) / TJ;
+  const int ntiles = tiles_k * tiles_j * g->nx;
+  k_stats3<<<NORM_PARTS, stencil_block(), 0, S(stream)>>>(dgrid<double>(g), tiles_k, tiles_j, ntiles, ws);
+  OCU_HIP_LAUNCHED("ocu_grid_stats3");
 }
 
 int ocu_norm3_finalize(const double *ws, double *out, void *stream) {

@@ -317,9 +317,9 @@ __device__ __forceinline__ T diag_xy(const C &c, int i, int j, int nx, int ny) {
 }
 
 // One colour on a row held as V cells per lane (k = k0..k0+V-1): the cells
-// with k%2 == kpar get relax_pt (diag_dev's z mods added to dxy in order),
-// the others keep their value in out.  k0 is even for V >= 2, so the colour
-// of a cell depends on e alone and the branches are uniform.
+// with k%2 == kpar and k < kupd get relax_pt (diag_dev's z mods added to dxy
+// in order), the others keep their value in out.  k0 is even for V >= 2, so
+// the colour of a cell depends on e alone and the branches are uniform.
 template <typename T, int V, typename C>
 __device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, int kupd, int nz, const Vec<T, V> &cen,
                                           T lo, T hi, const Vec<T, V> &ym, const Vec<T, V> &yp,
@@ -330,14 +330,17 @@ __device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, i
     const int k = k0 + e;
     if (V % 2 == 0) {
       if ((e & 1) != kpar) continue;
-    } else if ((k & 1) != kpar || k >= kupd) {
+    } else if ((k & 1) != kpar) {
       continue;
     }
     T d = dxy;
     if (k == nz - 1) d += c.m[4];
     if (k == 0) d += c.m[5];
-    out.v[e] = relax_pt(c, cen.v[e], e > 0 ? cen.v[e - 1] : lo, e < V - 1 ? cen.v[e + 1] : hi, ym.v[e], yp.v[e],
-                        xm.v[e], xp.v[e], b.v[e], d);
+    const T nv = relax_pt(c, cen.v[e], e > 0 ? cen.v[e - 1] : lo, e < V - 1 ? cen.v[e + 1] : hi, ym.v[e],
+                          yp.v[e], xm.v[e], xp.v[e], b.v[e], d);
+    // lanes past the row (and k >= 2*(nz/2)) keep their value: one of them
+    // carries the k = nz ghost for its neighbour
+    out.v[e] = k < kupd ? nv : out.v[e];
   }
 }
 

@@ -88,12 +88,18 @@ int ocu_host_malloc(void **ptr, size_t bytes) {
   OCU_REQUIRE(ptr, "ocu_host_malloc: null pointer");
   *ptr = nullptr;
   if (bytes == 0) return OCU_OK;
-  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocMapped);
   if (e != hipSuccess) {
     set_error("ocu_host_malloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     *ptr = nullptr;
     return OCU_ERR_NOMEM;
   }
+  return OCU_OK;
+}
+
+int ocu_host_device_pointer(void **dev, void *host) {
+  OCU_REQUIRE(dev && host, "ocu_host_device_pointer: null pointer");
+  OCU_HIP_TRY(hipHostGetDevicePointer(dev, host, 0));
   return OCU_OK;
 }
 

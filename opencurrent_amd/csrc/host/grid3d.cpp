@@ -296,12 +296,14 @@ namespace {
 struct ReduceScratch {
   void *ws = nullptr;
   double *host = nullptr;
+  double *host_dev = nullptr;  // host as the device sees it
   bool ready() {
-    if (ws) return true;
-    if (ocu_malloc(&ws, ocu_reduce_workspace_bytes() + 64) != OCU_OK) return false;
-    void *h = nullptr;
-    if (ocu_host_malloc(&h, 64) != OCU_OK) return false;
+    if (ws && host_dev) return true;
+    if (!ws && ocu_malloc(&ws, ocu_reduce_workspace_bytes() + 64) != OCU_OK) return false;
+    void *h = nullptr, *hd = nullptr;
+    if (ocu_host_malloc(&h, 64) != OCU_OK || ocu_host_device_pointer(&hd, h) != OCU_OK) return false;
     host = (double *)h;
+    host_dev = (double *)hd;
     return true;
   }
 };
@@ -316,9 +318,8 @@ bool Grid3DDevice<T>::reduce_double(int op, double &result) const {
   }
   void *s = ThreadManager::get_compute_stream();
   ocu_grid3d d = this->desc();
-  double *out = (double *)((char *)g_red.ws + ocu_reduce_workspace_bytes());
-  if (!ocu_status_ok(ocu_grid_reduce(&d, op, g_red.ws, out, s), "Grid3DDevice::reduce")) return false;
-  if (!ocu_status_ok(ocu_memcpy_async(g_red.host, out, sizeof(double), 1, s), "Grid3DDevice::reduce")) return false;
+  // the result goes straight into pinned host memory
+  if (!ocu_status_ok(ocu_grid_reduce(&d, op, g_red.ws, g_red.host_dev, s), "Grid3DDevice::reduce")) return false;
   if (!ocu_status_ok(ocu_stream_synchronize(s), "Grid3DDevice::reduce")) return false;
   result = *g_red.host;
   return true;

@@ -112,6 +112,7 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   int _coarse_mode;
   double *_norm_ws;    // device: NORM_PARTS (sum, max) pairs + 2 results
   double *_norm_host;  // pinned host: 2 results
+  double *_norm_dev;   // the same memory as the device sees it
   // _u_zero[l]: u at level l is logically the zero grid with its level-1
   // ghosts (clear_zero + apply_boundary_conditions) but memory was not
   // written; the next fused relaxation starts from zero without reading u

@@ -24,6 +24,7 @@ class Sol_MultigridPressureMixed3DDeviceD : public Sol_MultigridPressure3DBase {
   Grid3DDevice<float> _r32;                    // scaled FP32 residual (level-0 rhs of _mg_f)
   double *_ws;                                 // device: NORM_PARTS triples + results
   double *_host;                               // pinned: results
+  double *_host_dev;                           // the same memory as the device sees it
   bool _singular;
 
   // never called: the refinement loop drives _mg_f directly

@@ -299,6 +299,7 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   _coarse_mode = 0;
   _norm_ws = 0;
   _norm_host = 0;
+  _norm_dev = 0;
   disable_relax_on_host = false;
   use_fused_relax = true;
   store_residuals = false;
@@ -326,6 +327,7 @@ void Sol_MultigridPressure3DDevice<T>::free_storage() {
   _u_grid = _b_grid = _r_grid = _u_alt = 0;
   ocu_free(_norm_ws);
   ocu_host_free(_norm_host);
+  _norm_dev = 0;
   _norm_ws = 0;
   _norm_host = 0;
 }
@@ -539,9 +541,9 @@ void Sol_MultigridPressure3DDevice<T>::restrict_residuals(int fine_level, int co
               "Sol_MultigridPressure3DDevice::restrict_residuals");
   }
   if (norms) {
-    double *res = _norm_ws + 2 * ocu_norm_parts();
-    ok_status(ocu_norm_finalize(_norm_ws, res, s), "Sol_MultigridPressure3DDevice::restrict_residuals");
-    ok_status(ocu_memcpy_async(_norm_host, res, 2 * sizeof(double), 1, s), "restrict_residuals");
+    // the finalize kernel writes the two results straight into pinned host
+    // memory: no copy launch, one stream synchronisation
+    ok_status(ocu_norm_finalize(_norm_ws, _norm_dev, s), "Sol_MultigridPressure3DDevice::restrict_residuals");
     ok_status(ocu_stream_synchronize(s), "Sol_MultigridPressure3DDevice::restrict_residuals");
     double n = (double)nx(fine_level) * ny(fine_level) * nz(fine_level);
     if (l2) *l2 = sqrt(_norm_host[0] / n);
@@ -662,6 +664,12 @@ bool Sol_MultigridPressure3DDevice<T>::initialize_storage(int nx_val, int ny_val
   }
   _norm_ws = (double *)ws;
   _norm_host = (double *)hh;
+  void *hd = nullptr;
+  if (ocu_host_device_pointer(&hd, hh) != OCU_OK) {
+    printf("[ERROR] Sol_MultigridPressure3DDevice::initialize_storage - pinned results not device-visible\n");
+    return false;
+  }
+  _norm_dev = (double *)hd;
 
   // the smallest level (>= 1) from which every coarser level fits one
   // workgroup's LDS together

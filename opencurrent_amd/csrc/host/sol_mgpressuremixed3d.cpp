@@ -17,6 +17,7 @@ Sol_MultigridPressureMixed3DDeviceD::Sol_MultigridPressureMixed3DDeviceD() {
   _rhs = 0;
   _ws = 0;
   _host = 0;
+  _host_dev = 0;
   _singular = false;
   ir_cycles = 4;
   ir_cycles_first = 4;
@@ -78,6 +79,12 @@ bool Sol_MultigridPressureMixed3DDeviceD::initialize_storage(int nx_val, int ny_
   }
   _ws = (double *)w;
   _host = (double *)h;
+  void *hd = nullptr;
+  if (ocu_host_device_pointer(&hd, h) != OCU_OK) {
+    printf("[ERROR] Sol_MultigridPressureMixed3DDeviceD::initialize_storage - pinned results not device-visible\n");
+    return false;
+  }
+  _host_dev = (double *)hd;
   return true;
 }
 
@@ -87,10 +94,8 @@ bool Sol_MultigridPressureMixed3DDeviceD::fp64_residual(double bshift, double sc
                                                         double &sum) {
   void *s = ThreadManager::get_compute_stream();
   ocu_grid3d x = _x.desc(), b = _rhs->desc(), r = _r32.desc();
-  double *res = _ws + 3 * ocu_norm_parts();
   bool ok = ok_status(ocu_ir_residual(&x, &b, bshift, &r, scale, get_h(0), _fx, _fy, _fz, _ws, s), "ir_residual");
-  ok = ok && ok_status(ocu_norm3_finalize(_ws, res, s), "ir_residual");
-  ok = ok && ok_status(ocu_memcpy_async(_host, res, 3 * sizeof(double), 1, s), "ir_residual");
+  ok = ok && ok_status(ocu_norm3_finalize(_ws, _host_dev, s), "ir_residual");  // into pinned host memory
   ok = ok && ok_status(ocu_stream_synchronize(s), "ir_residual");
   sumsq = _host[0];
   maxabs = _host[1];
@@ -156,10 +161,12 @@ bool Sol_MultigridPressureMixed3DDeviceD::do_fmg(double tolerance, int max_iter,
   const double n = (double)nx(0) * ny(0) * nz(0);
   void *s = ThreadManager::get_compute_stream();
 
-  double bsum = 0, bsq = 0, bmax = 0;
-  check_ok(_rhs->reduce_double(3, bsum));
-  check_ok(_rhs->reduce_double(4, bsq));
-  check_ok(_rhs->reduce_double(0, bmax));
+  // sum, sum of squares and max|b| in one pass
+  ocu_grid3d bd = _rhs->desc();
+  check_ok(ok_status(ocu_grid_stats3(&bd, _ws, s), "rhs statistics") &&
+           ok_status(ocu_norm3_finalize(_ws, _host_dev, s), "rhs statistics") &&
+           ok_status(ocu_stream_synchronize(s), "rhs statistics"));
+  const double bsq = _host[0], bmax = _host[1], bsum = _host[2];
   double bm = _singular ? bsum / n : 0.0;
   double var = bsq / n - bm * bm;
   rhs_l2 = sqrt(var > 0 ? var : 0);
