@@ -313,7 +313,8 @@ def main():
         "config": {"workload": f"mixed-precision MG Poisson {n}^3, unit cube, Neumann faces, to "
                                f"{args.rel_tol:g} relative residual (FP32 V-cycles in FP64 IR)",
                    "grid": [n, n, n], "levels": 7 if n == 256 else None, "nu1": 2, "nu2": 2,
-                   "v_cycles_per_refinement": args.cycles},
+                   "v_cycles_per_refinement": f"{args.cycles} in the first refinement iteration, then as many as "
+                                              f"the observed contraction needs (at most {args.cycles})"},
         "rel_residual": res / rms_b,
         "ir_iterations": st["ir_iterations"],
         "vcycles": st["vcycles"],

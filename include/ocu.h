@@ -31,9 +31,12 @@ void *ocu_get_compute_stream(void);
  *   OCU_MG_FUSED_RELAX: one launch per red+black iteration, ghosts written by
  *     the sweep, deferred zero clears (Neumann/Dirichlet faces);
  *   OCU_MG_LDS_BOTTOM: the coarse levels of each V-cycle as one launch of a
- *     single LDS-resident workgroup.
- * Every option gives results bit-identical to the plain per-level path. */
-enum ocu_mg_options { OCU_MG_FUSED_RELAX = 1, OCU_MG_LDS_BOTTOM = 2, OCU_MG_DEFAULT = 3 };
+ *     single LDS-resident workgroup;
+ *   (these two give results bit-identical to the plain per-level path)
+ *   OCU_MG_ADAPTIVE_IR (mixed solver): after the first refinement iteration,
+ *     as many V-cycles (at most `cycles`) as the observed contraction per
+ *     cycle needs to reach half the tolerance. */
+enum ocu_mg_options { OCU_MG_FUSED_RELAX = 1, OCU_MG_LDS_BOTTOM = 2, OCU_MG_ADAPTIVE_IR = 4, OCU_MG_DEFAULT = 7 };
 
 /* ---- Sol_MultigridPressure3DDevice<T> (ocuequation/sol_mgpressure3d.h:36-181) ----
  * T follows rhs->dtype.  initialize_storage(nx,ny,nz,hx,hy,hz,&rhs) with the

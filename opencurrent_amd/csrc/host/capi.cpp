@@ -207,6 +207,7 @@ void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz,
   h->s.ir_fmg_first = fmg_first != 0;
   h->s.use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
   h->s.use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
+  h->s.ir_adaptive = (options & OCU_MG_ADAPTIVE_IR) != 0;
   if (!h->rhs.init_external(*rhs) || !h->s.initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs)) {
     delete h;
     return nullptr;

@@ -45,6 +45,9 @@ class Sol_MultigridPressureMixed3DDeviceD : public Sol_MultigridPressure3DBase {
   int ir_cycles;
   int ir_cycles_first;
   bool ir_fmg_first;
+  // after the first iteration, run as many V-cycles (at most ir_cycles) as the
+  // observed contraction per cycle needs to reach half the tolerance
+  bool ir_adaptive;
   // options of the FP32 hierarchy (Sol_MultigridPressure3DDevice fields)
   bool use_fused_relax;
   bool use_bottom;

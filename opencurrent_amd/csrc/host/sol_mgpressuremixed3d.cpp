@@ -22,6 +22,7 @@ Sol_MultigridPressureMixed3DDeviceD::Sol_MultigridPressureMixed3DDeviceD() {
   ir_cycles = 4;
   ir_cycles_first = 4;
   ir_fmg_first = false;
+  ir_adaptive = true;
   use_fused_relax = true;
   use_bottom = true;
   ir_iterations = 0;
@@ -110,6 +111,19 @@ bool Sol_MultigridPressureMixed3DDeviceD::interior_mean(const Grid3DDevice<doubl
   return ok;
 }
 
+// V-cycles for refinement iteration it >= 1 when the previous iteration ran
+// prev_cycles V-cycles and took the residual from prev_res to res: the
+// observed contraction per cycle rho = (res/prev_res)^(1/prev_cycles) is
+// extrapolated to reach half the tolerance, within [1, cap]
+static int adaptive_cycles(double res, double prev_res, int prev_cycles, double tol, int cap) {
+  if (!(prev_res > 0) || !(res > 0) || !(res < prev_res) || prev_cycles <= 0) return cap;
+  const double rho = pow(res / prev_res, 1.0 / prev_cycles);
+  const double need = log(0.5 * tol / res) / log(rho);
+  if (!(need > 1)) return 1;
+  const double n = ceil(need);
+  return n > cap ? cap : (int)n;
+}
+
 // one FP32 V-cycle from fine_level down and back (the structure of
 // do_fmg's inner loop, sol_mgpressure3d.cpp:184-239, without a convergence
 // test; the coarsest level is relaxed in FP64)
@@ -174,7 +188,8 @@ bool Sol_MultigridPressureMixed3DDeviceD::do_fmg(double tolerance, int max_iter,
   BoundaryConditionSet bc0 = bc;
   check_ok(apply_3d_boundary_conditions_level1(_x, bc0, hx(0), hy(0), hz(0)));
 
-  double prev_max = bmax;
+  double prev_max = bmax, prev_res = 0;
+  int prev_cycles = 0;
   bool converged = false;
   for (int it = 0;; it++) {
     int ex = 0;
@@ -200,11 +215,15 @@ bool Sol_MultigridPressureMixed3DDeviceD::do_fmg(double tolerance, int max_iter,
     _mg_f.clear_zero(0);
     if (it == 0 && ir_fmg_first) {
       fmg_f32(ir_cycles_first);
+      prev_cycles = 0;
     } else {
       _mg_f.apply_boundary_conditions(0);
       int nc = it == 0 ? ir_cycles_first : ir_cycles;
+      if (ir_adaptive && it > 0) nc = adaptive_cycles(res, prev_res, prev_cycles, tolerance, nc);
       for (int c = 0; c < nc; c++) vcycle_f32(0);
+      prev_cycles = nc;
     }
+    prev_res = res;
     _mg_f.relax(0, nu1, RO_RED_BLACK);
     ocu_grid3d x = _x.desc(), e = _mg_f.pressure().desc();
     ok_status(ocu_ir_correct(&x, &e, inv_scale, s), "ir_correct");

@@ -17,7 +17,7 @@ from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, OCU_F32, OCU_F
 from .device import desc_to_host, host_to_desc
 
 
-MG_FUSED_RELAX, MG_LDS_BOTTOM = 1, 2
+MG_FUSED_RELAX, MG_LDS_BOTTOM, MG_ADAPTIVE_IR = 1, 2, 4
 
 
 def _options(use_fused_relax, use_bottom):
@@ -81,11 +81,12 @@ class MultigridPressureMixed3D:
     """Sol_MultigridPressureMixed3DDeviceD: FP32 V-cycles inside FP64 iterative refinement."""
 
     def __init__(self, rhs, hx, hy, hz, bc, nu1=2, nu2=2, convergence=CONVERGENCE_L2, cycles=4,
-                 cycles_first=4, fmg_first=False, use_fused_relax=True, use_bottom=True):
+                 cycles_first=4, fmg_first=False, use_fused_relax=True, use_bottom=True, adaptive=True):
         self._rhs = rhs
         self.cfg = MGConfig.make(bc, nu1, nu2, convergence)
         self.h = _lib.host().ocu_mgmixed_create(rhs.ptr(), hx, hy, hz, C.byref(self.cfg), cycles, cycles_first,
-                                                int(fmg_first), _options(use_fused_relax, use_bottom))
+                                                int(fmg_first), _options(use_fused_relax, use_bottom) |
+                                                (MG_ADAPTIVE_IR if adaptive else 0))
         if not self.h:
             raise _lib.OcuError(f"Sol_MultigridPressureMixed3DDeviceD::initialize_storage failed: {_lib.last_error()}")
 

@@ -130,7 +130,7 @@ int orc_mg_solve(const ocu_grid3d *u0, const ocu_grid3d *b0, double hx, double h
 
 int orc_ir_solve(const ocu_grid3d *x, const ocu_grid3d *b, double hx, double hy, double hz,
                  const ocu_mg_config *cfg, double tol, int max_ir, int cycles, int cycles_first,
-                 int fmg_first, double *residual, int *ir_iters, int *vcycles) {
+                 int fmg_first, int adaptive, double *residual, int *ir_iters, int *vcycles) {
   if (x->dtype != OCU_F64 || b->dtype != OCU_F64) return -1;
   MixedIR s;
   s.bc = cfg->bc;
@@ -138,6 +138,7 @@ int orc_ir_solve(const ocu_grid3d *x, const ocu_grid3d *b, double hx, double hy,
   s.convergence = cfg->convergence;
   s.prolong_quirk = cfg->prolong_quirk;
   s.cycles = cycles; s.cycles_first = cycles_first; s.fmg_first = fmg_first != 0;
+  s.adaptive = adaptive != 0;
   if (!s.initialize_storage(b->nx, b->ny, b->nz, hx, hy, hz, *b, *x)) return -1;
   double res = 0;
   bool ok = s.solve(res, tol, max_ir);

@@ -49,7 +49,7 @@ int orc_mg_solve(const ocu_grid3d *u0, const ocu_grid3d *b0, double hx, double h
  * FP64.  tol is absolute in the cfg->convergence norm. */
 int orc_ir_solve(const ocu_grid3d *x, const ocu_grid3d *b, double hx, double hy, double hz,
                  const ocu_mg_config *cfg, double tol, int max_ir, int cycles, int cycles_first,
-                 int fmg_first, double *residual, int *ir_iters, int *vcycles);
+                 int fmg_first, int adaptive, double *residual, int *ir_iters, int *vcycles);
 
 int orc_divergence(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *div,
                    int nx, int ny, int nz, double hx, double hy, double hz);

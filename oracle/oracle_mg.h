@@ -646,6 +646,19 @@ inline bool is_singular(const ocu_bc_set &bc) {
  *     x += (double)e32 / s ; BC(x)
  *   if singular: x -= mean(x); BC(x)
  */
+/* V-cycles for refinement iteration it >= 1 when the previous iteration
+ * ran prev_cycles V-cycles and took the residual from prev_res to res: the
+ * observed contraction per cycle rho = (res/prev_res)^(1/prev_cycles) is
+ * extrapolated to reach half the tolerance, within [1, cap]. */
+inline int adaptive_cycles(double res, double prev_res, int prev_cycles, double tol, int cap) {
+  if (!(prev_res > 0) || !(res > 0) || !(res < prev_res) || prev_cycles <= 0) return cap;
+  const double rho = std::pow(res / prev_res, 1.0 / prev_cycles);
+  const double need = std::log(0.5 * tol / res) / std::log(rho);
+  if (!(need > 1)) return 1;
+  const double n = std::ceil(need);
+  return n > cap ? cap : (int)n;
+}
+
 struct MixedIR {
   MGSolver<float> mg;  // FP32 hierarchy; level-0 u = e32, b = r32
   HostGrid<float> e32, b32;
@@ -654,6 +667,7 @@ struct MixedIR {
   int nu1 = 2, nu2 = 2;
   int cycles = 4, cycles_first = 4;
   bool fmg_first = false;
+  bool adaptive = false;  // later iterations: adaptive_cycles(...) capped by cycles
   int convergence = OCU_CONVERGENCE_L2;
   int prolong_quirk = 1;
   bool singular = false;
@@ -756,7 +770,8 @@ struct MixedIR {
     bool ok = false;
     // the cast scale of iteration it comes from max|r| of iteration it-1
     // (max|b| for the first), so the scaled cast can share the residual pass
-    double prev_max = 0;
+    double prev_max = 0, prev_res = 0;
+    int prev_cycles = 0;
     {
       View<double> bb(b);
       for (int i = 0; i < nx; i++)
@@ -784,11 +799,15 @@ struct MixedIR {
             bv.at(i, j, k) = (float)(r[((size_t)i * ny + j) * nz + k] * scale);
       if (it == 0 && fmg_first) {
         fmg_f32(cycles_first);
+        prev_cycles = 0;
       } else {
         int nc = it == 0 ? cycles_first : cycles;
+        if (adaptive && it > 0) nc = adaptive_cycles(res, prev_res, prev_cycles, tol, nc);
         mg.apply_boundary_conditions(0);
         for (int c = 0; c < nc; c++) { vcycle_f32(0); total_vcycles++; }
+        prev_cycles = nc;
       }
+      prev_res = res;
       mg.relax(0, nu1, MGSolver<float>::RO_RED_BLACK);
       for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++)

@@ -36,7 +36,7 @@ SIGS = {
     "orc_mg_restrict": (I, [GP, GP]),
     "orc_mg_prolong": (I, [GP, GP, I]),
     "orc_mg_solve": (I, [GP, GP, D, D, D, C.POINTER(MGConfig), D, I, I, DP, IP, DP, I, IP]),
-    "orc_ir_solve": (I, [GP, GP, D, D, D, C.POINTER(MGConfig), D, I, I, I, I, DP, IP, IP]),
+    "orc_ir_solve": (I, [GP, GP, D, D, D, C.POINTER(MGConfig), D, I, I, I, I, I, DP, IP, IP]),
     "orc_divergence": (I, [GP, GP, GP, GP, I, I, I, D, D, D]),
     "orc_gradient_sub": (I, [GP, GP, GP, GP, D, D, D, D]),
     "orc_self_advect": (I, [GP, GP, GP, GP, GP, GP, I, I, I, D, D, D, I]),
@@ -83,11 +83,11 @@ def mg_solve(u, b, h, cfg, tol, max_iter, mode=0):
     return ok, res.value, vc.value, [hist[i] for i in range(min(n.value, 1024))]
 
 
-def ir_solve(x, b, h, cfg, tol, max_ir, cycles=4, cycles_first=4, fmg_first=False):
+def ir_solve(x, b, h, cfg, tol, max_ir, cycles=4, cycles_first=4, fmg_first=False, adaptive=True):
     hx, hy, hz = h
     res, it, vc = C.c_double(), C.c_int(), C.c_int()
     ok = lib().orc_ir_solve(x.ptr(), b.ptr(), hx, hy, hz, C.byref(cfg), tol, max_ir, cycles, cycles_first,
-                            int(fmg_first), C.byref(res), C.byref(it), C.byref(vc))
+                            int(fmg_first), int(adaptive), C.byref(res), C.byref(it), C.byref(vc))
     return ok, res.value, it.value, vc.value
 
 
