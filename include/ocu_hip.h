@@ -256,6 +256,21 @@ int ocu_bc_adjoint_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, void *st
  * adj(n)); forced-inflow low face 0 is fixed (adj(0) = 0); adj(n) = 0. */
 int ocu_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
                            const ocu_bc_set *bc, void *stream);
+/* Adjoint of ocu_divergence (sol_divergence3ddev.cu:23-45): adj_u/v/w +=
+ * D^T lam on the normal faces [0,n] of interior rows, i.e. adj_f +=
+ * (1/h)*(lam_{f-1} - lam_f); lam is read on its interior only (values off
+ * the interior count as zero).  lam congruent with adj_u. */
+int ocu_divergence_adjoint(const ocu_grid3d *lam, const ocu_grid3d *adj_u, const ocu_grid3d *adj_v,
+                           const ocu_grid3d *adj_w, int nx, int ny, int nz, double hx, double hy,
+                           double hz, void *stream);
+/* Adjoint in p of ocu_gradient_sub (sol_gradient3ddev.cu:24-47): adj_p +=
+ * G^T(adj_u, adj_v, adj_w) over p's interior and its low face ghosts (the
+ * cells the gradient reads), with coefficient and 1/h as in the forward
+ * call; adj_u/v/w are read on the faces [0,n-1] of interior rows.  The
+ * adjoint in u is the identity. */
+int ocu_gradient_adjoint(const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const ocu_grid3d *adj_w,
+                         const ocu_grid3d *adj_p, double coefficient, double hx, double hy, double hz,
+                         void *stream);
 /* Sol_LaplacianCentered3DDevice_stencil (sol_laplaciancent3ddev.cu:27-62) */
 int ocu_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
                       double hx, double hy, double hz, double coefficient, void *stream);

@@ -85,6 +85,8 @@ HIP_SIGS = {
     "ocu_passive_advect_adjoint": (I, [GP, GP, GP, GP, GP, GP, GP, GP, GP, D, I, I, I, D, D, D, I, P]),
     "ocu_bc_adjoint_nocorners": (I, [GP, BP, P]),
     "ocu_mac_adjoint_normal": (I, [GP, GP, GP, BP, P]),
+    "ocu_divergence_adjoint": (I, [GP, GP, GP, GP, I, I, I, D, D, D, P]),
+    "ocu_gradient_adjoint": (I, [GP, GP, GP, GP, D, D, D, D, P]),
     "ocu_mg_bottom_vcycle": (I, [GP, GP, MP, DP, I, BP, I, I, I, I, I, P]),
     "ocu_mg_bottom_lds_bytes": (C.c_size_t, [GP, I, I]),
     "ocu_mg_coarse_relax": (I, [GP, GP, I, I, MP, BP, D, D, D, I, P]),

@@ -10,16 +10,20 @@ Eqn_ScalarAdvection3D::advance_one_step chains it, eqn_scalaradvection3d.cpp):
 with A the passive advection (Sol_PassiveAdvection3DDevice) and fill the
 device no-corner ghost fill.  The reference has no adjoint; this one is built
 from hand-written kernels of libocu_hip.so: ocu_passive_advect_adjoint,
-ocu_bc_adjoint_nocorners, ocu_mac_adjoint_normal, and the projection itself,
-which is self-adjoint on a periodic MAC grid (D^T = -G, so
-P = I - G (DG)^-1 D = P^T) and so reuses the multigrid on the adjoint field.
+ocu_bc_adjoint_nocorners, ocu_mac_adjoint_normal, ocu_gradient_adjoint,
+ocu_divergence_adjoint.  The projection P = M (I - G B S D) M (MAC fill M,
+divergence D, pressure solve S, pressure ghost fill B, gradient G) has the
+adjoint P^T = M^T (I - D^T S B^T G^T) M^T with S^T = S, since the
+cell-centred Poisson operator with the projection's boundary conditions is
+symmetric: the backward pass runs one more multigrid solve on the adjoint
+right-hand side.  On a periodic grid D^T = -G and P is self-adjoint.
 The states are checkpointed every `checkpoint_every` steps; the backward pass
 recomputes one segment at a time.
 
 `AdvectProjectLoss.apply(u, v, w, problem)` exposes the pair as a
 torch.autograd.Function over the flat Grid3D buffers of the MAC velocity.
-The parameters are the independent faces of the periodic grid (normal faces
-0..n-1 of interior rows); the gradient is zero everywhere else.
+The parameters are the normal faces 0..n-1 of interior rows (faces the MAC
+fill overwrites get a zero gradient); the gradient is zero everywhere else.
 """
 import ctypes as C
 
@@ -28,8 +32,8 @@ import numpy as np
 from . import _lib
 from . import device as D
 from . import _types as T
-from ._types import BCSet
-from .solvers import ProjectDivergence3D
+from ._types import BC, BCSet, CONVERGENCE_LINF
+from .solvers import MultigridPressure3D, ProjectDivergence3D
 
 
 def _mac(nx, ny, nz, dtype):
@@ -55,24 +59,51 @@ def parameter_mask(g, axis):
     return m
 
 
+def poisson_bc(flow_bc):
+    """convert_bc_to_poisson_eqn (sol_project3d.cpp:26-43): periodic stays
+    periodic, forced inflow becomes homogeneous Neumann"""
+    s = BCSet.uniform(T.BC_PERIODIC)
+    for f in ("xpos", "xneg", "ypos", "yneg", "zpos", "zneg"):
+        t = getattr(flow_bc, f).type
+        if t == T.BC_FORCED_INFLOW_VARIABLE_SLIP:
+            setattr(s, f, BC(T.BC_NEUMANN, 0.0, 0.0))
+        elif t != T.BC_PERIODIC:
+            raise _lib.OcuError(f"invalid flow boundary condition type {t}")
+    return s
+
+
+class _Desc:
+    def __init__(self, d):
+        self.desc = d
+
+    def ptr(self):
+        return C.byref(self.desc)
+
+
 class AdvectProjectProblem:
-    """The model above on an nx*ny*nz periodic grid, with its adjoint."""
+    """The model above on an nx*ny*nz grid, with its adjoint.  flow_bc (MAC
+    velocity: periodic or forced inflow) and phi_bc (scalar) default to
+    periodic."""
 
     def __init__(self, nx, ny, nz, hx, hy, hz, dt, steps, scheme=T.IT_SECOND_ORDER_CENTERED, dtype=np.float64,
-                 checkpoint_every=5, proj_tol=None):
+                 checkpoint_every=5, proj_tol=None, flow_bc=None, phi_bc=None):
         self.n = (nx, ny, nz)
         self.h = (hx, hy, hz)
         self.dt, self.steps, self.scheme, self.ce = float(dt), int(steps), int(scheme), int(checkpoint_every)
         self.dtype = np.dtype(dtype)
         # max |div| after the projection: FP32 multigrid reaches ~1e-3 on O(1/h) divergences
         self.tol = proj_tol if proj_tol is not None else (1e-12 if self.dtype == np.float64 else 1e-3)
-        self.flow_bc = BCSet.uniform(T.BC_PERIODIC)
-        self.phi_bc = BCSet.uniform(T.BC_PERIODIC)
+        self.flow_bc = flow_bc.copy() if flow_bc is not None else BCSet.uniform(T.BC_PERIODIC)
+        self.phi_bc = phi_bc.copy() if phi_bc is not None else BCSet.uniform(T.BC_PERIODIC)
         self.u, self.v, self.w, self.phi = _mac(nx, ny, nz, dtype)
         self.au, self.av, self.aw, _ = _mac(nx, ny, nz, dtype)
         self.proj = ProjectDivergence3D(self.u, self.v, self.w, self.flow_bc, nx, ny, nz, hx, hy, hz)
-        self.proj_adj = ProjectDivergence3D(self.au, self.av, self.aw, self.flow_bc, nx, ny, nz, hx, hy, hz)
         like = self.phi
+        # the adjoint pressure solve: right-hand side lam_p, the projection's
+        # Poisson boundary conditions and stopping rule
+        self.lam_p = D.DeviceGrid.congruent(like)
+        self.mg_adj = MultigridPressure3D(self.lam_p, hx, hy, hz, poisson_bc(self.flow_bc),
+                                          convergence=CONVERGENCE_LINF)
         self.phi0, self.target, self.dphi = (D.DeviceGrid.congruent(like) for _ in range(3))
         self.lam, self.lam2 = D.DeviceGrid.congruent(like), D.DeviceGrid.congruent(like)
         nseg = (self.steps + self.ce - 1) // self.ce
@@ -89,6 +120,24 @@ class AdvectProjectProblem:
         ok, res = proj.solve(self.tol)
         if not ok:
             raise _lib.OcuError(f"projection did not converge to {self.tol:g} (residual {res:g})")
+
+    def _project_adjoint(self):
+        """(au, av, aw) <- P^T (au, av, aw), see the module doc"""
+        nx, ny, nz = self.n
+        D.mac_adjoint_normal(self.au, self.av, self.aw, self.flow_bc)
+        self.lam_p.tensor.zero_()
+        D.gradient_adjoint(self.au, self.av, self.aw, self.lam_p, -1.0, *self.h)
+        D.bc_adjoint_nocorners(self.lam_p, self.mg_adj.cfg.bc)
+        pd = self.mg_adj.pressure_desc()
+        _lib.check(_lib.hip().ocu_grid_clear(C.byref(pd), 0.0, None), "clear adjoint pressure")
+        # the forward tolerance bounds |div| of an O(1) velocity; the adjoint
+        # right-hand side carries the loss's scale, so the bound scales with it
+        tol = self.tol * max(1.0, D.grid_reduce(self.lam_p, 0))
+        ok, res = self.mg_adj.solve(tol, 15)
+        if not ok:
+            raise _lib.OcuError(f"adjoint pressure solve did not converge to {tol:g} (residual {res:g})")
+        D.divergence_adjoint(_Desc(pd), self.au, self.av, self.aw, nx, ny, nz, *self.h)
+        D.mac_adjoint_normal(self.au, self.av, self.aw, self.flow_bc)
 
     def _step(self, src, dst):
         nx, ny, nz = self.n
@@ -143,8 +192,7 @@ class AdvectProjectProblem:
                 lam, lam2 = lam2, lam
         D.bc_adjoint_nocorners(lam, self.phi_bc)  # the initial fill of phi0
         grad_phi0 = lam.interior().clone()
-        D.mac_adjoint_normal(self.au, self.av, self.aw, self.flow_bc)
-        self._project(self.proj_adj)
+        self._project_adjoint()
         grads = []
         for g, m in zip((self.au, self.av, self.aw), self.masks):
             x = g.tensor.clone()

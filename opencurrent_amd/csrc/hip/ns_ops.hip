@@ -245,6 +245,48 @@ __global__ void k_mac_adjoint_normal(T *__restrict__ a, int64_t s_n, int64_t s_a
   p[(int64_t)n * s_n] = (T)0;
 }
 
+// adjoint of the divergence: one thread per index (i,j,k) of the face box
+// [0,nx]x[0,ny]x[0,nz] updates the u, v and w faces at that index that lie on
+// interior rows: adj_f += ih * (lam_{f-1} - lam_f), lam = 0 off the interior
+template <typename T>
+__global__ __launch_bounds__(256) void k_divergence_adjoint(const T *__restrict__ lam, T *__restrict__ au,
+                                                            T *__restrict__ av, T *__restrict__ aw, Inv<T> ih,
+                                                            int64_t xs, int64_t ys, int nx, int ny, int nz) {
+  const int k = blockIdx.x * TK + threadIdx.x;
+  const int j = blockIdx.y * TJ + threadIdx.y;
+  const int i = blockIdx.z;
+  if (j > ny || k > nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  const bool ii = i < nx, ji = j < ny, ki = k < nz;
+  const T c = (ii && ji && ki) ? lam[id] : (T)0;
+  if (ji && ki) au[id] += ih.x * ((i > 0 ? lam[id - xs] : (T)0) - c);
+  if (ii && ki) av[id] += ih.y * ((j > 0 ? lam[id - ys] : (T)0) - c);
+  if (ii && ji) aw[id] += ih.z * ((k > 0 ? lam[id - 1] : (T)0) - c);
+}
+
+// adjoint of the gradient subtraction in p: one thread per cell of
+// [-1,nx-1]x[-1,ny-1]x[-1,nz-1] with at most one index at -1 (the interior and
+// the low face ghosts the gradient reads) gathers
+// adj_p += (cx*(U_c - U_{c+x}) + cy*(V_c - V_{c+y})) + cz*(W_c - W_{c+z}),
+// U/V/W = the velocity adjoints on the written faces [0,n-1], 0 elsewhere
+template <typename T>
+__global__ __launch_bounds__(256) void k_gradient_adjoint(const T *__restrict__ au, const T *__restrict__ av,
+                                                          const T *__restrict__ aw, T *__restrict__ ap, Inv<T> cf,
+                                                          int64_t xs, int64_t ys, int nx, int ny, int nz) {
+  const int k = (int)(blockIdx.x * TK + threadIdx.x) - 1;
+  const int j = (int)(blockIdx.y * TJ + threadIdx.y) - 1;
+  const int i = (int)blockIdx.z - 1;
+  if (j >= ny || k >= nz) return;
+  if ((i < 0) + (j < 0) + (k < 0) > 1) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  const bool in = i >= 0 && j >= 0 && k >= 0;
+  const T u0 = in ? au[id] : (T)0, v0 = in ? av[id] : (T)0, w0 = in ? aw[id] : (T)0;
+  const T u1 = (j >= 0 && k >= 0 && i + 1 < nx) ? au[id + xs] : (T)0;
+  const T v1 = (i >= 0 && k >= 0 && j + 1 < ny) ? av[id + ys] : (T)0;
+  const T w1 = (i >= 0 && j >= 0 && k + 1 < nz) ? aw[id + 1] : (T)0;
+  ap[id] += (cf.x * (u0 - u1) + cf.y * (v0 - v1)) + cf.z * (w0 - w1);
+}
+
 // Sol_LaplacianCentered3DDevice_stencil (sol_laplaciancent3ddev.cu:27-62)
 template <typename T>
 __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ phi, T *__restrict__ d, Inv<T> ih2, T coef,
@@ -473,6 +515,53 @@ int ocu_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_g
   launch(u, xs, ys, 1, nx, ny, nz, bc->xpos.type == PER, bc->xneg.type == IN);
   launch(v, ys, xs, 1, ny, nx, nz, bc->ypos.type == PER, bc->yneg.type == IN);
   launch(w, 1, xs, ys, nz, nx, ny, bc->zpos.type == PER, bc->zneg.type == IN);
+  OCU_HIP_LAUNCHED(what);
+}
+
+int ocu_divergence_adjoint(const ocu_grid3d *lam, const ocu_grid3d *adj_u, const ocu_grid3d *adj_v,
+                           const ocu_grid3d *adj_w, int nx, int ny, int nz, double hx, double hy, double hz,
+                           void *stream) {
+  const char *what = "ocu_divergence_adjoint";
+  if (int e = check_mac(adj_u, adj_v, adj_w, nx, ny, nz, what)) return e;
+  if (int e = check_same(adj_u, lam, what)) return e;
+  OCU_REQUIRE(lam->nx == nx && lam->ny == ny && lam->nz == nz, "%s - invalid dimensions for lam", what);
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  const int64_t xs = (int64_t)adj_u->pny * adj_u->pnz, ys = adj_u->pnz;
+  const dim3 g((nz + 1 + TK - 1) / TK, (ny + 1 + TJ - 1) / TJ, nx + 1);
+  if (lam->dtype == OCU_F64)
+    k_divergence_adjoint<double><<<g, stencil_block(), 0, S(stream)>>>(
+        at0<double>(lam), at0<double>(adj_u), at0<double>(adj_v), at0<double>(adj_w), inv<double>(hx, hy, hz), xs, ys,
+        nx, ny, nz);
+  else
+    k_divergence_adjoint<float><<<g, stencil_block(), 0, S(stream)>>>(
+        at0<float>(lam), at0<float>(adj_u), at0<float>(adj_v), at0<float>(adj_w), inv<float>(hx, hy, hz), xs, ys, nx,
+        ny, nz);
+  OCU_HIP_LAUNCHED(what);
+}
+
+int ocu_gradient_adjoint(const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const ocu_grid3d *adj_w,
+                         const ocu_grid3d *adj_p, double coefficient, double hx, double hy, double hz, void *stream) {
+  const char *what = "ocu_gradient_adjoint";
+  if (int e = check_grid(adj_p, what)) return e;
+  const int nx = adj_p->nx, ny = adj_p->ny, nz = adj_p->nz;
+  if (int e = check_mac(adj_u, adj_v, adj_w, nx, ny, nz, what)) return e;
+  if (int e = check_same(adj_u, adj_p, what)) return e;
+  if ((int64_t)nx * ny * nz == 0) return OCU_OK;
+  const int64_t xs = (int64_t)adj_u->pny * adj_u->pnz, ys = adj_u->pnz;
+  const dim3 g((nz + 1 + TK - 1) / TK, (ny + 1 + TJ - 1) / TJ, nx + 1);
+  if (adj_p->dtype == OCU_F64) {
+    const Inv<double> ih = inv<double>(hx, hy, hz), c = {coefficient * ih.x, coefficient * ih.y, coefficient * ih.z};
+    k_gradient_adjoint<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(adj_u), at0<double>(adj_v),
+                                                                      at0<double>(adj_w), at0<double>(adj_p), c, xs,
+                                                                      ys, nx, ny, nz);
+  } else {
+    const Inv<float> ih = inv<float>(hx, hy, hz);
+    const float cf = (float)coefficient;
+    const Inv<float> c = {cf * ih.x, cf * ih.y, cf * ih.z};
+    k_gradient_adjoint<float><<<g, stencil_block(), 0, S(stream)>>>(at0<float>(adj_u), at0<float>(adj_v),
+                                                                     at0<float>(adj_w), at0<float>(adj_p), c, xs, ys,
+                                                                     nx, ny, nz);
+  }
   OCU_HIP_LAUNCHED(what);
 }
 

@@ -233,6 +233,16 @@ def mac_adjoint_normal(u, v, w, bc):
     _lib.check(_lib.hip().ocu_mac_adjoint_normal(u.ptr(), v.ptr(), w.ptr(), C.byref(bc), None), "mac_adjoint_normal")
 
 
+def divergence_adjoint(lam, au, av, aw, nx, ny, nz, hx, hy, hz):
+    _lib.check(_lib.hip().ocu_divergence_adjoint(lam.ptr(), au.ptr(), av.ptr(), aw.ptr(), nx, ny, nz, hx, hy, hz,
+                                                 None), "divergence_adjoint")
+
+
+def gradient_adjoint(au, av, aw, ap, coef, hx, hy, hz):
+    _lib.check(_lib.hip().ocu_gradient_adjoint(au.ptr(), av.ptr(), aw.ptr(), ap.ptr(), coef, hx, hy, hz, None),
+               "gradient_adjoint")
+
+
 def laplacian_acc(phi, dphi, nx, ny, nz, hx, hy, hz, coef):
     _lib.check(_lib.hip().ocu_laplacian_acc(phi.ptr(), dphi.ptr(), nx, ny, nz, hx, hy, hz, coef, None),
                "laplacian_acc")

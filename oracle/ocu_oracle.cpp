@@ -315,3 +315,18 @@ int orc_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_g
   return DISPATCH(u->dtype, (mac_adjoint_normal<float>(*u, *v, *w, *bc), 1),
                   (mac_adjoint_normal<double>(*u, *v, *w, *bc), 1));
 }
+
+int orc_divergence_adjoint(const ocu_grid3d *lam, const ocu_grid3d *adj_u, const ocu_grid3d *adj_v,
+                           const ocu_grid3d *adj_w, int nx, int ny, int nz, double hx, double hy, double hz) {
+  if (!lam || !adj_u || !adj_v || !adj_w) return -1;
+  return DISPATCH(lam->dtype, (divergence_adjoint<float>(*lam, *adj_u, *adj_v, *adj_w, nx, ny, nz, hx, hy, hz), 1),
+                  (divergence_adjoint<double>(*lam, *adj_u, *adj_v, *adj_w, nx, ny, nz, hx, hy, hz), 1));
+}
+
+int orc_gradient_adjoint(const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const ocu_grid3d *adj_w,
+                         const ocu_grid3d *adj_p, double coefficient, double hx, double hy, double hz) {
+  if (!adj_u || !adj_v || !adj_w || !adj_p) return -1;
+  return DISPATCH(adj_p->dtype,
+                  (gradient_adjoint<float>(*adj_u, *adj_v, *adj_w, *adj_p, (float)coefficient, hx, hy, hz), 1),
+                  (gradient_adjoint<double>(*adj_u, *adj_v, *adj_w, *adj_p, (double)coefficient, hx, hy, hz), 1));
+}

@@ -75,6 +75,10 @@ int orc_passive_advect_adjoint(const ocu_grid3d *u, const ocu_grid3d *v, const o
                                int scheme);
 int orc_bc_adjoint_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc);
 int orc_mac_adjoint_normal(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc);
+int orc_divergence_adjoint(const ocu_grid3d *lam, const ocu_grid3d *adj_u, const ocu_grid3d *adj_v,
+                           const ocu_grid3d *adj_w, int nx, int ny, int nz, double hx, double hy, double hz);
+int orc_gradient_adjoint(const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const ocu_grid3d *adj_w,
+                         const ocu_grid3d *adj_p, double coefficient, double hx, double hy, double hz);
 
 /* One projection, Sol_ProjectDivergence3DDevice<T>::solve(tol) on a fresh
  * solver (zero pressure guess).  pressure_out (congruent with u) receives p. */

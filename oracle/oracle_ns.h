@@ -286,6 +286,58 @@ void mac_adjoint_normal(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_gr
     }
 }
 
+/* Adjoint of divergence (no reference counterpart).  d_c = ix*(u_{c+1} -
+ * u_c) + iy*(...) + iz*(...) over the interior cells, so every normal face f
+ * in [0,n] of an interior row gathers
+ *   adj_u_f += ix * (lam_{f-1} - lam_f)
+ * with lam taken as 0 outside the interior (lam's ghosts are not read). */
+template <typename T>
+void divergence_adjoint(const ocu_grid3d &lamg, const ocu_grid3d &aug, const ocu_grid3d &avg, const ocu_grid3d &awg,
+                        int nx, int ny, int nz, double hx, double hy, double hz) {
+  View<T> lam(lamg), au(aug), av(avg), aw(awg);
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+  auto L = [&](int i, int j, int k) -> T {
+    return (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) ? lam.at(i, j, k) : (T)0;
+  };
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i <= nx; i++)
+    for (int j = 0; j <= ny; j++)
+      for (int k = 0; k <= nz; k++) {
+        const T c = L(i, j, k);
+        if (j < ny && k < nz) au.at(i, j, k) += ix * (L(i - 1, j, k) - c);
+        if (i < nx && k < nz) av.at(i, j, k) += iy * (L(i, j - 1, k) - c);
+        if (i < nx && j < ny) aw.at(i, j, k) += iz * (L(i, j, k - 1) - c);
+      }
+}
+
+/* Adjoint of gradient_sub in p (no reference counterpart).  The forward
+ * step is u_c += (coef*ix) * (p_c - p_{c-1}) over the cells c of p, so with
+ * cx = coef*ix and U the adjoint of u on the written faces [0,n-1] of
+ * interior rows (0 elsewhere), every cell of p's interior and of its three
+ * low face-ghost layers gathers
+ *   adj_p_c += (cx*(U_c - U_{c+x}) + cy*(V_c - V_{c+y})) + cz*(W_c - W_{c+z}).
+ * The adjoint in u is the identity (u is updated in place). */
+template <typename T>
+void gradient_adjoint(const ocu_grid3d &aug, const ocu_grid3d &avg, const ocu_grid3d &awg, const ocu_grid3d &apg,
+                      T coefficient, double hx, double hy, double hz) {
+  View<T> au(aug), av(avg), aw(awg), ap(apg);
+  const int nx = apg.nx, ny = apg.ny, nz = apg.nz;
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+  const T cx = coefficient * ix, cy = coefficient * iy, cz = coefficient * iz;
+  auto in = [&](int i, int j, int k) { return i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz; };
+  auto U = [&](int i, int j, int k) -> T { return in(i, j, k) ? au.at(i, j, k) : (T)0; };
+  auto V = [&](int i, int j, int k) -> T { return in(i, j, k) ? av.at(i, j, k) : (T)0; };
+  auto W = [&](int i, int j, int k) -> T { return in(i, j, k) ? aw.at(i, j, k) : (T)0; };
+#pragma omp parallel for schedule(static)
+  for (int i = -1; i < nx; i++)
+    for (int j = -1; j < ny; j++)
+      for (int k = -1; k < nz; k++) {
+        if ((i < 0) + (j < 0) + (k < 0) > 1) continue;  // edges and corners are never read
+        ap.at(i, j, k) += (cx * (U(i, j, k) - U(i + 1, j, k)) + cy * (V(i, j, k) - V(i, j + 1, k))) +
+                          cz * (W(i, j, k) - W(i, j, k + 1));
+      }
+}
+
 /* Sol_LaplacianCentered3DDevice_stencil, sol_laplaciancent3ddev.cu:27-62:
  * dphi += c * (z, y, x second differences), 1/h^2 rounded to T (:100-102). */
 template <typename T>

@@ -44,6 +44,8 @@ SIGS = {
     "orc_passive_advect_adjoint": (I, [GP, GP, GP, GP, GP, GP, GP, GP, GP, D, I, I, I, D, D, D, I]),
     "orc_bc_adjoint_nocorners": (I, [GP, BP]),
     "orc_mac_adjoint_normal": (I, [GP, GP, GP, BP]),
+    "orc_divergence_adjoint": (I, [GP, GP, GP, GP, I, I, I, D, D, D]),
+    "orc_gradient_adjoint": (I, [GP, GP, GP, GP, D, D, D, D]),
     "orc_laplacian_acc": (I, [GP, GP, I, I, I, D, D, D, D]),
     "orc_thermal_force": (I, [GP, GP, I, I, I, I, D]),
     "orc_project": (I, [GP, GP, GP, BP, I, I, I, D, D, D, D, GP, DP]),

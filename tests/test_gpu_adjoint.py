@@ -1,12 +1,14 @@
 """Gradients through advection + projection (config C4's model, no
 reference counterpart): the hand-written adjoint kernels chained with
-checkpointing and the self-adjoint projection, checked against central
-finite differences in FP64, against themselves without checkpoints, across
-precisions, and through torch.autograd."""
+checkpointing and the projection's adjoint (one more pressure solve),
+checked against central finite differences in FP64 on periodic and walled
+domains, by the dot-product identity, against themselves without
+checkpoints, across precisions, and through torch.autograd."""
 import numpy as np
 import pytest
 
 from opencurrent_amd import _types as T
+from opencurrent_amd._types import BC, BCSet
 from fields import uniform
 
 pytestmark = pytest.mark.gpu
@@ -26,10 +28,24 @@ def blob(n, h, centre, sigma):
     return np.exp(-r2 / (2 * sigma ** 2))
 
 
-def problem(n=(32, 32, 32), steps=20, scheme=T.IT_SECOND_ORDER_CENTERED, dtype=np.float64, ce=5):
+def walls(kind):
+    """(flow_bc, phi_bc): a closed free-slip box with an insulated scalar, or
+    Rayleigh-Benard walls (no-slip z, periodic x and y) with a Dirichlet
+    scalar on z"""
+    if kind == "closed":
+        return BCSet.uniform(T.BC_FORCED_INFLOW_VARIABLE_SLIP), BCSet.uniform(T.BC_NEUMANN)
+    flow, phi = BCSet.uniform(T.BC_PERIODIC), BCSet.uniform(T.BC_PERIODIC)
+    for f in ("zpos", "zneg"):
+        setattr(flow, f, BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0, 1.0))
+        setattr(phi, f, BC(T.BC_DIRICHELET, 0.0, 0.0))
+    return flow, phi
+
+
+def problem(n=(32, 32, 32), steps=20, scheme=T.IT_SECOND_ORDER_CENTERED, dtype=np.float64, ce=5, bcs=None):
     h = tuple(1.0 / x for x in n)
+    flow, phi = bcs if bcs is not None else (None, None)
     p = A().AdvectProjectProblem(*n, *h, dt=.25 * h[0], steps=steps, scheme=scheme, dtype=dtype,
-                                 checkpoint_every=ce)
+                                 checkpoint_every=ce, flow_bc=flow, phi_bc=phi)
     c0 = .3 + .4 * uniform(3002, (3,))
     c1 = .3 + .4 * uniform(3001, (3,))
     p.set_scalars(blob(n, h, c0, .1), blob(n, h, c1, .1))
@@ -72,6 +88,48 @@ def test_gradient_matches_central_differences(gpu, scheme, flow):
         an = dot(g, dv)
         worst = max(worst, abs(fd - an) / abs(an))
     assert worst <= tol, worst
+
+
+@pytest.mark.parametrize("kind", ["closed", "rb"])
+def test_gradient_with_walls_matches_central_differences(gpu, kind):
+    """as above (centred scheme, random flow, 8 directions, eps 1e-4, FP64,
+    32^3, 1e-6) with walls: the projection's adjoint goes through the
+    gradient and divergence adjoints and a Neumann pressure solve"""
+    p = problem(bcs=walls(kind))
+    v = velocity(p)
+    p.forward(*v)
+    g = p.backward()[:3]
+    eps = 1e-4
+    worst = 0.0
+    for d in range(8):
+        dv = velocity(p, seeds=(4101 + 3 * d, 4102 + 3 * d, 4103 + 3 * d))
+        lp = p.forward(*[a + eps * b for a, b in zip(v, dv)])
+        lm = p.forward(*[a - eps * b for a, b in zip(v, dv)])
+        fd = (lp - lm) / (2 * eps)
+        an = dot(g, dv)
+        worst = max(worst, abs(fd - an) / abs(an))
+    assert worst <= 1e-6, worst
+
+
+@pytest.mark.parametrize("kind", ["periodic", "closed", "rb"])
+def test_projection_adjoint_dot_product(gpu, kind):
+    """<P a, b> = <a, P^T b> over the parameter faces, P^T from the adjoint
+    kernels and one pressure solve, to the solves' tolerance"""
+    import torch
+    p = problem(n=(16, 16, 32), steps=1, bcs=None if kind == "periodic" else walls(kind))
+    a, b = velocity(p, (5101, 5102, 5103)), velocity(p, (5104, 5105, 5106))
+    for g, y, m in zip((p.u, p.v, p.w), a, p.masks):
+        g.tensor.zero_()
+        g.tensor[m] = y[m]
+    p._project(p.proj)
+    pa = [torch.where(m, g.tensor, torch.zeros_like(g.tensor)) for g, m in zip((p.u, p.v, p.w), p.masks)]
+    for g, y, m in zip((p.au, p.av, p.aw), b, p.masks):
+        g.tensor.zero_()
+        g.tensor[m] = y[m]
+    p._project_adjoint()
+    ptb = [torch.where(m, g.tensor, torch.zeros_like(g.tensor)) for g, m in zip((p.au, p.av, p.aw), p.masks)]
+    lhs, rhs = dot(pa, b), dot(a, ptb)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs), (lhs, rhs)
 
 
 def test_checkpointing_gives_the_same_gradient(gpu):

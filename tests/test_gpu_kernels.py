@@ -511,6 +511,31 @@ def test_ghost_adjoints(gpu, dtype, case):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n", [(8, 6, 10), (7, 9, 5), (1, 1, 1), (33, 4, 70)])
+def test_divergence_gradient_adjoints(gpu, dtype, n):
+    """ocu_divergence_adjoint / ocu_gradient_adjoint bit-exact against the
+    oracle over whole buffers (every ghost, padding and face included)"""
+    u, v, w, lam = mac_grids(*n, dtype)
+    for q, x in enumerate((u, v, w, lam)):
+        x.buf[:] = (2 * uniform(141 + q, x.buf.shape) - 1).astype(dtype)
+    h = (.5, .25, 1 / 3)
+    wu, wv, ww = u.copy(), v.copy(), w.copy()
+    assert O.lib().orc_divergence_adjoint(lam.ptr(), wu.ptr(), wv.ptr(), ww.ptr(), *n, *h) == 1
+    du, dv, dw, dl = (D().DeviceGrid.from_host(x) for x in (u, v, w, lam))
+    D().divergence_adjoint(dl, du, dv, dw, *n, *h)
+    for got, want, nm in ((du, wu, "u"), (dv, wv, "v"), (dw, ww, "w")):
+        assert_bitexact(got.to_host(), want, f"divergence adjoint {nm} {n}")
+    for coef in (-1.0, .7):
+        wp = lam.copy()
+        assert O.lib().orc_gradient_adjoint(u.ptr(), v.ptr(), w.ptr(), wp.ptr(), coef, *h) == 1
+        du, dv, dw, dp = (D().DeviceGrid.from_host(x) for x in (u, v, w, lam))
+        D().gradient_adjoint(du, dv, dw, dp, coef, *h)
+        assert_bitexact(dp.to_host(), wp, f"gradient adjoint {coef} {n}")
+        for got, x in ((du, u), (dv, v), (dw, w)):  # read only
+            assert_bitexact(got.to_host(), x, "gradient adjoint input")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_laplacian_thermal(gpu, dtype):
     nx, ny, nz = 16, 12, 20
     u, v, w, t = mac_grids(nx, ny, nz, dtype)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from opencurrent_amd import _types as T
-from opencurrent_amd._types import BC, BCSet, HostGrid
+from opencurrent_amd._types import BC, BCSet, HostGrid, MGConfig
 from fields import mac_grids, uniform
 import oracle_lib as O
 
@@ -34,8 +34,8 @@ def bcs(kind):
     return s
 
 
-def velocity(seed, away_from_zero=False):
-    u, v, w, t = mac_grids(*N)
+def velocity(seed, away_from_zero=False, n=N):
+    u, v, w, t = mac_grids(*n)
     for q, g in enumerate((u, v, w)):
         x = 2 * uniform(seed + q, g.buf.shape) - 1
         if away_from_zero:
@@ -67,9 +67,9 @@ def adjoint(u, v, w, phi_grid, mu, bc, scheme):
     return out, (au, av, aw)
 
 
-def faces(g, axis):
+def faces(g, axis, n=N):
     """the velocity entries an advection reads: faces [0, n] of interior rows"""
-    nx, ny, nz = N
+    nx, ny, nz = n
     f = g.full()[1:, 1:, 1:]  # index 0 = cell 0
     return f[:nx + 1, :ny, :nz] if axis == 0 else f[:nx, :ny + 1, :nz] if axis == 1 else f[:nx, :ny, :nz + 1]
 
@@ -124,3 +124,110 @@ def test_mac_normal_adjoint_folds_periodic_duplicates():
         sl_first[q] = 0
         assert np.all(got[tuple(sl_last)] == 0)
         assert np.allclose(got[tuple(sl_first)], wnt[tuple(sl_first)] + wnt[tuple(sl_last)], rtol=0, atol=0)
+
+
+def cells_and_face_ghosts(g):
+    """interior cells plus the three low face-ghost layers, as a mask over buf"""
+    m = np.zeros(g.full().shape, bool)
+    nx, ny, nz = N
+    m[1:nx + 1, 1:ny + 1, 1:nz + 1] = True
+    m[0, 1:ny + 1, 1:nz + 1] = m[1:nx + 1, 0, 1:nz + 1] = m[1:nx + 1, 1:ny + 1, 0] = True
+    return m
+
+
+def test_divergence_adjoint_dot_product():
+    """<D u, mu> = <u, D^T mu> over the faces [0,n] of interior rows; mu's
+    ghost values are not read"""
+    u, v, w, d = velocity(71)
+    assert O.lib().orc_divergence(u.ptr(), v.ptr(), w.ptr(), d.ptr(), *N, *H) == 1
+    mu = 2 * uniform(81, N) - 1
+    lam = HostGrid.congruent(d)
+    lam.buf[:] = 5.0  # ghosts and padding must not matter
+    lam.interior()[:] = mu
+    au, av, aw, _ = mac_grids(*N)
+    assert O.lib().orc_divergence_adjoint(lam.ptr(), au.ptr(), av.ptr(), aw.ptr(), *N, *H) == 1
+    lhs = float((d.interior() * mu).sum())
+    rhs = sum(float((faces(g, q) * faces(a, q)).sum()) for q, (g, a) in enumerate(zip((u, v, w), (au, av, aw))))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+    for q, a in enumerate((au, av, aw)):  # nothing written off those faces
+        faces(a, q)[:] = 0
+        assert np.all(a.buf == 0)
+
+
+@pytest.mark.parametrize("coef", [-1.0, .75])
+def test_gradient_adjoint_dot_product(coef):
+    """<G p, lam> = <p, G^T lam>: G p is the increment gradient_sub adds to
+    the faces [0,n-1] of interior rows; p is read on the interior and the
+    low face ghosts, where G^T lam is written"""
+    u, v, w, p = mac_grids(*N)
+    p.buf[:] = 2 * uniform(91, p.buf.shape) - 1
+    assert O.lib().orc_gradient_sub(u.ptr(), v.ptr(), w.ptr(), p.ptr(), coef, *H) == 1
+    lu, lv, lw, ap = velocity(95)
+    ap.buf[:] = 0
+    assert O.lib().orc_gradient_adjoint(lu.ptr(), lv.ptr(), lw.ptr(), ap.ptr(), coef, *H) == 1
+    lhs = sum(float((g.buf * l.buf).sum()) for g, l in zip((u, v, w), (lu, lv, lw)))
+    rhs = float((p.buf * ap.buf).sum())
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+    m = cells_and_face_ghosts(ap)
+    assert np.all(ap.full()[~m] == 0)
+
+
+def flow_bcs(kind):
+    if kind == "periodic":
+        return BCSet.uniform(T.BC_PERIODIC)
+    if kind == "closed":  # free-slip box
+        return BCSet.uniform(T.BC_FORCED_INFLOW_VARIABLE_SLIP)
+    s = BCSet.uniform(T.BC_PERIODIC)  # Rayleigh-Benard: no-slip z walls, periodic x, y
+    s.zpos = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0, 1.0)
+    s.zneg = BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0, 1.0)
+    return s
+
+
+def poisson_bcs(flow):
+    """convert_bc_to_poisson_eqn (sol_project3d.cpp:26-43)"""
+    s = BCSet.uniform(T.BC_PERIODIC)
+    for f in ("xpos", "xneg", "ypos", "yneg", "zpos", "zneg"):
+        if getattr(flow, f).type == T.BC_FORCED_INFLOW_VARIABLE_SLIP:
+            setattr(s, f, BC(T.BC_NEUMANN, 0.0, 0.0))
+    return s
+
+
+def project_adjoint(lu, lv, lw, flow, tol, n, h):
+    """P^T in place: M^T, G^T (c = -1), pressure-fill^T, L^-1 (L = L^T),
+    D^T, M^T - the transpose of MAC fill, divergence, solve, gradient, MAC
+    fill taken in reverse order"""
+    O.lib().orc_mac_adjoint_normal(lu.ptr(), lv.ptr(), lw.ptr(), C.byref(flow))
+    _, _, _, lp = mac_grids(*n)
+    lp.buf[:] = 0
+    assert O.lib().orc_gradient_adjoint(lu.ptr(), lv.ptr(), lw.ptr(), lp.ptr(), -1.0, *h) == 1
+    pb = poisson_bcs(flow)
+    assert O.lib().orc_bc_adjoint_nocorners(lp.ptr(), C.byref(pb)) == 1
+    x = HostGrid.congruent(lp)
+    cfg = MGConfig.make(pb, convergence=T.CONVERGENCE_LINF)
+    res, vc = C.c_double(), C.c_int()
+    hist, hl = (C.c_double * 1)(), C.c_int()
+    assert O.lib().orc_mg_solve(x.ptr(), lp.ptr(), *h, C.byref(cfg), tol, 15, 0, C.byref(res), C.byref(vc), hist, 0,
+                                C.byref(hl)) == 1
+    assert O.lib().orc_divergence_adjoint(x.ptr(), lu.ptr(), lv.ptr(), lw.ptr(), *n, *h) == 1
+    O.lib().orc_mac_adjoint_normal(lu.ptr(), lv.ptr(), lw.ptr(), C.byref(flow))
+
+
+@pytest.mark.parametrize("kind", ["periodic", "closed", "rb"])
+def test_projection_adjoint_dot_product(kind):
+    """<P u, lam> = <u, P^T lam> over the normal faces [0,n] of interior rows,
+    to the pressure solves' tolerance, for periodic and walled domains"""
+    flow = flow_bcs(kind)
+    n, h = (16, 16, 16), (1 / 16, 1 / 16, 1 / 16)  # the multigrid needs even sizes
+    u, v, w, _ = velocity(101, n=n)
+    x0 = [faces(g, q, n).copy() for q, g in enumerate((u, v, w))]
+    p = HostGrid.congruent(u)
+    r = C.c_double()
+    assert O.lib().orc_project(u.ptr(), v.ptr(), w.ptr(), C.byref(flow), *n, *h, 1e-11, p.ptr(), C.byref(r)) == 1
+    lu, lv, lw, _ = mac_grids(*n)
+    for q, g in enumerate((lu, lv, lw)):
+        faces(g, q, n)[:] = 2 * uniform(111 + q, faces(g, q, n).shape) - 1
+    y = [faces(g, q, n).copy() for q, g in enumerate((lu, lv, lw))]
+    lhs = sum(float((faces(g, q, n) * yq).sum()) for q, (g, yq) in enumerate(zip((u, v, w), y)))
+    project_adjoint(lu, lv, lw, flow, 1e-11, n, h)
+    rhs = sum(float((faces(g, q, n) * xq).sum()) for q, (g, xq) in enumerate(zip((lu, lv, lw), x0)))
+    assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1.0), (lhs, rhs)

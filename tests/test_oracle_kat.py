@@ -405,3 +405,14 @@ def test_advection_conserves_periodic_mass_and_constants(scheme):
     L().orc_project(u.ptr(), v.ptr(), w.ptr(), C.byref(bc), nx, ny, nz, .5, .5, .5, 1e-13, p.ptr(), C.byref(r))
     L().orc_passive_advect(u.ptr(), v.ptr(), w.ptr(), c.ptr(), d.ptr(), nx, ny, nz, .5, .5, .5, scheme)
     assert np.abs(d.interior()).max() < 1e-11
+
+
+def test_rayleigh_critical_number_matches_reference_res16():
+    """The oracle's NS step (advection, diffusion, thermal force, time-step
+    rule, projection) reproduces the reference's measured Ra_cr of free-slip
+    Rayleigh-Benard convection at resolution 16, 659.693989
+    (tests/rayleigh.cpp:338), within the measurement's own resolution
+    (5e-3; the discretisation error that number carries is 2.18)."""
+    import rayleigh as R
+    ra = R.critical_ra(R.OracleRun, 16, T.DIR_ZPOS, False, 657.0, 660.0, log=lambda s: None)
+    assert abs(ra - R.RA_FREE_SLIP[16]) <= 5e-3, ra
