@@ -416,3 +416,49 @@ def test_rayleigh_critical_number_matches_reference_res16():
     import rayleigh as R
     ra = R.critical_ra(R.OracleRun, 16, T.DIR_ZPOS, False, 657.0, 660.0, log=lambda s: None)
     assert abs(ra - R.RA_FREE_SLIP[16]) <= 5e-3, ra
+
+
+@pytest.mark.parametrize("ts", [T.TS_FORWARD_EULER, T.TS_ADAMS_BASHFORD2])
+def test_ns_time_stepping_of_a_diffusing_eigenmode(ts):
+    """With zero velocity, no buoyancy and periodic faces the NS temperature
+    obeys T' = kappa L T; a separable Fourier mode of the 7-point Laplacian
+    (eigenvalue -sum 4/h^2 sin^2(pi m/n)) is multiplied each step by the
+    amplification of the scheme: forward Euler a_{n+1} = (1 + dt mu) a_n;
+    Adams-Bashforth 2 with ab = -dt^2 / (2 dt_last) (first step Euler,
+    eqn_incompressns3d.cu:355-390) a_{n+1} = a_n + (dt - ab) mu a_n +
+    ab mu a_{n-1}.  Unequal steps exercise dt_last."""
+    from opencurrent_amd._lib import NSParams
+    n, h, kappa = (16, 8, 16), (1 / 16, 1 / 8, 1 / 16), .7
+    m = (1, 3, 2)
+    p = NSParams()
+    p.nx, p.ny, p.nz = n
+    p.hx, p.hy, p.hz = h
+    p.max_divergence = 1e-10
+    p.temp_bc = BCSet.uniform(T.BC_PERIODIC)
+    p.flow_bc = BCSet.uniform(T.BC_PERIODIC)
+    p.vertical_direction = T.DIR_ZPOS
+    p.gravity, p.bouyancy = 0.0, 0.0
+    p.thermal_diffusion, p.viscosity = kappa, 1.0
+    p.advection_scheme, p.time_step, p.cfl_factor = T.IT_FIRST_ORDER_UPWIND, ts, .5
+    u, v, w, t = mac_grids(*n)
+    x = [2 * math.pi * mq * (np.arange(nq) + .5) / nq for mq, nq in zip(m, n)]
+    t.interior()[:] = np.cos(x[0])[:, None, None] * np.sin(x[1])[None, :, None] * np.cos(x[2] + .3)[None, None, :]
+    t0 = t.interior().copy()
+    mu = -kappa * sum(4 / hq ** 2 * math.sin(math.pi * mq / nq) ** 2 for hq, mq, nq in zip(h, m, n))
+    ho = O.lib().orc_ns_create(T.OCU_F64, C.byref(p), u.ptr(), v.ptr(), w.ptr(), t.ptr())
+    assert ho
+    try:
+        a_prev, a, last = None, 1.0, 0.0
+        for dt in (2e-4, 2e-4, 3e-4, 1e-4, 2.5e-4):
+            assert O.lib().orc_ns_advance_one_step(ho, dt) == 1
+            if ts == T.TS_ADAMS_BASHFORD2 and last > 0:
+                ab = -dt * dt / (2 * last)
+                a_prev, a = a, a + (dt - ab) * mu * a + ab * mu * a_prev
+            else:
+                a_prev, a = a, a + dt * mu * a
+            last = dt
+            got = HostGrid.congruent(t)
+            assert O.lib().orc_ns_get(ho, 3, got.ptr()) == 1
+            assert np.abs(got.interior() - a * t0).max() <= 1e-13, (dt, a)
+    finally:
+        O.lib().orc_ns_destroy(ho)
