@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--no-extra", action="store_true", help="skip the C1/C3/C4 side measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C1/C3/C4/C5 side measurements")
     return ap.parse_args()
 
 
@@ -213,6 +213,97 @@ def time_adjoint(n=128, steps=20):
             "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2), "loss": loss}
 
 
+def _device_used():
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    return total - free
+
+
+def time_lockex(n=(512, 512, 256), steps=10):
+    """config C5: lock exchange in a closed no-slip box of 512 x 512 x 256
+    cells (hx = hy = 1, hz = 0.5, so the multigrid runs with (fx,fy,fz) =
+    (.25,.25,1)), heavy fluid for x < L/2, gravity along y, nu = kappa = 0.1;
+    FP64 equations projected by FP32 V-cycles in FP64 refinement, dt =
+    min(stable step, 0.1).  Then the advection-projection gradient model on
+    the same grid and walls: 10 steps forward and backward, FP64.  Reports
+    the device memory each holds (measured) against the 288 GB of HBM."""
+    import torch
+    import fields
+    from opencurrent_amd import _types as T
+    from opencurrent_amd import adjoint, solvers
+    from opencurrent_amd._lib import NSParams
+    nx, ny, nz = n
+    p = NSParams()
+    p.nx, p.ny, p.nz = n
+    p.hx, p.hy, p.hz = 1.0, 1.0, .5
+    p.max_divergence = 1e-8
+    walls = T.BCSet.uniform(T.BC_FORCED_INFLOW_VARIABLE_SLIP)
+    for f in ("xpos", "xneg", "ypos", "yneg", "zpos", "zneg"):
+        setattr(walls, f, T.BC(T.BC_FORCED_INFLOW_VARIABLE_SLIP, 0.0, 1.0))
+    p.flow_bc = walls
+    p.temp_bc = T.BCSet.uniform(T.BC_NEUMANN)
+    p.vertical_direction = T.DIR_YPOS
+    p.gravity, p.bouyancy, p.viscosity, p.thermal_diffusion = -1.0, 1.0, .1, .1
+    p.advection_scheme, p.time_step, p.cfl_factor = T.IT_FIRST_ORDER_UPWIND, T.TS_FORWARD_EULER, 1.0
+    p.mixed_projection = 1
+    u, v, w, t = fields.mac_grids(nx, ny, nz)
+    noise = 1e-3 * (2 * fields.uniform(5001, (ny, nz)) - 1)
+    t.interior()[:] = np.where(np.arange(nx) < nx // 2, -1.0, 1.0)[:, None, None]
+    t.interior()[nx // 2 - 1:nx // 2 + 1] += noise[None]
+    torch.cuda.synchronize()
+    base = _device_used()
+    ns = solvers.IncompressibleNS3D(p, u, v, w, t)
+    del u, v, w, t
+    torch.cuda.synchronize()
+    ns_bytes = _device_used() - base
+    ns.advance_one_step(min(ns.get_max_stable_timestep(), .1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if not ns.advance_one_step(min(ns.get_max_stable_timestep(), .1)):
+            raise RuntimeError("lock-exchange step failed")
+    torch.cuda.synchronize()
+    ns_s = (time.perf_counter() - t0) / steps
+    del ns
+    torch.cuda.empty_cache()
+
+    flow = walls
+    base = _device_used()
+    ap = adjoint.AdvectProjectProblem(nx, ny, nz, 1.0, 1.0, .5, dt=.25, steps=steps, dtype=np.float64,
+                                      checkpoint_every=5, flow_bc=flow, phi_bc=T.BCSet.uniform(T.BC_NEUMANN))
+    x = (np.arange(nx) + .5) / nx
+    y = (np.arange(ny) + .5) / ny
+    z = (np.arange(nz) + .5) / nz
+    ap.set_scalars(np.tanh(8 * (x - .5))[:, None, None] * np.ones((1, ny, nz)),
+                   np.tanh(8 * (x - .45))[:, None, None] + .1 * np.sin(2 * np.pi * y)[None, :, None]
+                   * np.cos(np.pi * z)[None, None, :])
+    vel = []
+    for q, g in enumerate((ap.u, ap.v, ap.w)):
+        tt = torch.as_tensor(2 * fields.uniform(1001 + q, tuple(g.tensor.shape)) - 1, dtype=g.tensor.dtype,
+                             device=g.tensor.device)
+        tt[~ap.masks[q]] = 0
+        vel.append(tt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ap.forward(*vel)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ap.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    adj_bytes = _device_used() - base
+    del ap, vel
+    torch.cuda.empty_cache()
+    cells = nx * ny * nz
+    return {"workload": f"lock exchange {nx}x{ny}x{nz}, hx=hy=1, hz=0.5, no-slip walls, FP64 NS with the "
+                        f"FP32-refined projection to 1e-8 Linf; advection-projection gradient over {steps} steps, "
+                        f"FP64, checkpoints every 5",
+            "ns_ms_per_step": round(ns_s * 1e3, 2), "ns_mcells_s": round(cells / ns_s / 1e6, 1),
+            "ns_device_gb": round(ns_bytes / 1e9, 2),
+            "adjoint_forward_ms": round((t1 - t0) * 1e3, 1), "adjoint_backward_ms": round((t2 - t1) * 1e3, 1),
+            "adjoint_device_gb": round(adj_bytes / 1e9, 2), "hbm_gb": 288}
+
+
 def cpu_baseline(n, rel_tol, budget_s=25.0):
     """The oracle's FP64 multigrid (reference FMG schedule, OpenMP over
     i-planes) on the host cores, same problem and target, repeated while the
@@ -358,9 +449,11 @@ def main():
                        "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes}
 
     if not args.no_extra:
-        out["c1_fp64_mg_64"] = time_fp64_mg_64(hip, stream)
-        out["c3_ns_step"] = time_ns_step()
-        out["c4_adjoint"] = time_adjoint()
+        for key, fn in (("c1_fp64_mg_64", lambda: time_fp64_mg_64(hip, stream)), ("c3_ns_step", time_ns_step),
+                        ("c4_adjoint", time_adjoint), ("c5_lockex", time_lockex)):
+            t0 = time.perf_counter()
+            out[key] = fn()
+            print(f"[bench] {key} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.rel_tol)
