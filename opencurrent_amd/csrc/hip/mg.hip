@@ -408,42 +408,28 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
     return *reinterpret_cast<const VT *>(brow[r] + (int64_t)min(max(p, 0), nx - 1) * b.xs);
   };
 
-  // per row: U0 planes p-1 (a), p (cc), p+1 (n) and the prefetched p+2 (nn);
-  // U1 planes p-2, p-1, p; b planes p-1, p and the prefetched p+1
-  VT a[R], cc[R], n[R], nn[R], u1m2[R], u1m1[R], u1[R], bprev[R], bcur[R], bn[R];
-  T ga[R], gc[R], gn[R], gnn[R];
-#ifdef OCU_FUSED_DEEP
-  VT n3[R], bn2[R];
-  T gn3[R];
-#endif
+  // per row: U0 planes p (cc) and p+1 (n, loaded during iteration p); U1
+  // planes p-2, p-1, p; b planes p-1 and p (loaded during iteration p).  The
+  // first colour's x-neighbours on plane p-1 are cells of the other colour,
+  // which U1 holds unchanged, so U0 of plane p-1 is not kept: u1m1 stands in
+  // for it (and starts as U0 of plane i0-2).  Registers stay under 80 so two
+  // blocks share a CU and each hides the other's barriers.
+  VT cc[R], n[R], u1m2[R], u1m1[R], u1[R], bprev[R], bcur[R];
+  T ga[R], gc[R], gn[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    load(r, i0 - 2, a[r], ga[r]);
+    load(r, i0 - 2, u1m1[r], ga[r]);
     load(r, i0 - 1, cc[r], gc[r]);
-    load(r, i0, n[r], gn[r]);
-    bn[r] = loadb(r, i0 - 1);
-#ifdef OCU_FUSED_DEEP
-    load(r, i0 + 1, nn[r], gnn[r]);
-    bn2[r] = loadb(r, i0);
-#endif
 #pragma unroll
-    for (int e = 0; e < V; e++) u1m2[r].v[e] = u1m1[r].v[e] = bprev[r].v[e] = (T)0;
+    for (int e = 0; e < V; e++) u1m2[r].v[e] = bprev[r].v[e] = (T)0;
   }
 
   for (int p = i0 - 1; p <= i1; ++p) {
     const bool p_in = p >= 0 && p < nx;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-#ifdef OCU_FUSED_DEEP
-      load(r, p + 3, n3[r], gn3[r]);
-      bcur[r] = bn[r];
-      bn[r] = bn2[r];
-      bn2[r] = loadb(r, p + 2);
-#else
-      load(r, p + 2, nn[r], gnn[r]);
-      bcur[r] = bn[r];
-      bn[r] = loadb(r, p + 1);
-#endif
+      load(r, p + 1, n[r], gn[r]);
+      bcur[r] = loadb(r, p);
       sU0[t0 + r][lane] = cc[r];
     }
     __syncthreads();
@@ -459,7 +445,7 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
         const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[t0 - 1][lane];
         const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[t0 + R][lane];
         relax_row<T, V>(c, diag_xy<T>(c, p, j, nx, ny), (first_color ^ p ^ j) & 1, k0, kupd, nz, cc[r], lo, hi, ym,
-                        yp, a[r], n[r], bcur[r], u1[r]);
+                        yp, u1m1[r], n[r], bcur[r], u1[r]);
       }
       sU1[p & 1][t0 + r][lane] = u1[r];
     }
@@ -492,12 +478,8 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      a[r] = cc[r]; ga[r] = gc[r];
+      ga[r] = gc[r];
       cc[r] = n[r]; gc[r] = gn[r];
-      n[r] = nn[r]; gn[r] = gnn[r];
-#ifdef OCU_FUSED_DEEP
-      nn[r] = n3[r]; gnn[r] = gn3[r];
-#endif
       u1m2[r] = u1m1[r]; u1m1[r] = u1[r];
       bprev[r] = bcur[r];
     }
@@ -524,7 +506,7 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
 // for FP32 rows and FP64 rows of four cells per lane; the macros exist for
 // tuning builds
 #ifndef OCU_ROWS_F32
-#define OCU_ROWS_F32 2, 10
+#define OCU_ROWS_F32 1, 12
 #endif
 #ifndef OCU_ROWS_F64
 #define OCU_ROWS_F64 1, 12
