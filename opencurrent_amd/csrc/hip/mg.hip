@@ -5,6 +5,7 @@
 // lanes always walks 64 consecutive k of one (i,j) row.  All arithmetic
 // keeps the reference's operation order; the library is built with
 // -ffp-contract=off so the results match the CPU restatement bit for bit.
+#include <algorithm>
 #include <cstring>
 
 #include "ghost_l1.h"
@@ -610,6 +611,97 @@ __global__ __launch_bounds__(256) void k_res_restrict(DGrid<T> u, DGrid<T> b, DG
       for (int w = 0; w < 4; w++) { a += sh[0][w]; m = fmax(m, sh[1][w]); }
       parts[2 * blockIdx.x] = a;
       parts[2 * blockIdx.x + 1] = m;
+    }
+  }
+}
+
+// Residual + restriction with whole rows: each wave owns one coarse row jc,
+// i.e. the fine rows 2jc and 2jc+1 held as V fine cells per lane (V/2
+// coarse cells), and marches along the fine planes of its i-chunk.  The
+// fine planes P-1, P, P+1 of the two rows stay in registers, the j-halo rows
+// 2jc-1 and 2jc+2 of plane P are loaded beside them (L2 hits: they are
+// other waves' rows), k-neighbours come by DPP with the z ghosts of the row
+// in g (lane 0: k = -1; the lane after the row, or lane 63: k = nz).  No LDS,
+// no barriers; operation orders are those of res_pt and restrict_sum.
+template <typename T>
+__device__ __forceinline__ T res_val(const ResC<T> &c, T b, T zm, T zp, T ym, T yp, T xm, T xp, T uc) {
+  return b - (c.fz * (zm + zp) + c.fy * (ym + yp) + c.fx * (xm + xp) - c.diag * uc);
+}
+
+template <typename T, int V, int NW>
+__global__ __launch_bounds__(64 * NW) void k_res_restrict_rows(DGrid<T> u, DGrid<T> b, DGrid<T> bc, ResC<T> c,
+                                                                int tiles_j, int ichunk, int ntiles) {
+  typedef Vec<T, V> VT;
+  typedef Vec<T, V / 2> VC;
+  const int per = gridDim.x / FXCD;
+  const int logical = (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD);
+  if (logical >= ntiles) return;  // uniform per block
+  const int tj = logical % tiles_j, ti = logical / tiles_j;
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int jc = tj * NW + w;
+  if (jc >= bc.ny) return;  // uniform per wave; no barriers follow
+  const int nz = u.nz;
+  const int k0 = lane * V;
+  const bool active = k0 < nz;
+  const int kc = active ? k0 : 0;
+  const int hi_lane = (nz == 64 * V) ? 63 : nz / V;
+  const int gk = lane == 0 ? -1 : nz;
+  const bool g_in_data = lane == hi_lane && nz < 64 * V;
+  const int ic0 = ti * ichunk, ic1 = min(bc.nx, ic0 + ichunk);
+  // fine rows 2jc-1 .. 2jc+2 (rows -1 and ny are ghost rows)
+  const T *urow[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) urow[q] = u.p + (int64_t)(2 * jc - 1 + q) * u.ys;
+  const T *brow[2] = {b.p + (int64_t)(2 * jc) * b.ys + kc, b.p + (int64_t)(2 * jc + 1) * b.ys + kc};
+  auto row = [&](int q, int P) { return *reinterpret_cast<const VT *>(urow[q] + (int64_t)P * u.xs + kc); };
+  auto rowg = [&](int q, int P, VT &d, T &g) {
+    const T *r = urow[q] + (int64_t)P * u.xs;
+    d = *reinterpret_cast<const VT *>(r + kc);
+    g = r[gk];
+    if (g_in_data) d.v[0] = g;
+  };
+
+  VT prev[2], cur[2], nxt[2], rr0[2];
+  T gc[2], gn[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    prev[q] = row(q + 1, 2 * ic0 - 1);
+    rowg(q + 1, 2 * ic0, cur[q], gc[q]);
+  }
+  for (int P = 2 * ic0; P < 2 * ic1; ++P) {
+    VT h0 = row(0, P), h3 = row(3, P), bb[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      rowg(q + 1, P + 1, nxt[q], gn[q]);
+      bb[q] = *reinterpret_cast<const VT *>(brow[q] + (int64_t)P * b.xs);
+    }
+    VT rr[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const T lo = from_lower(cur[q].v[V - 1], gc[q]), hi = from_upper(cur[q].v[0], gc[q]);
+      const VT &ym = q == 0 ? h0 : cur[0];
+      const VT &yp = q == 0 ? cur[1] : h3;
+#pragma unroll
+      for (int e = 0; e < V; e++)
+        rr[q].v[e] = res_val(c, bb[q].v[e], e > 0 ? cur[q].v[e - 1] : lo, e < V - 1 ? cur[q].v[e + 1] : hi, ym.v[e],
+                             yp.v[e], prev[q].v[e], nxt[q].v[e], cur[q].v[e]);
+    }
+    if (P & 1) {
+      VC o;
+#pragma unroll
+      for (int m = 0; m < V / 2; m++)
+        o.v[m] = restrict_sum(rr0[0].v[2 * m], rr0[0].v[2 * m + 1], rr0[1].v[2 * m], rr0[1].v[2 * m + 1],
+                              rr[0].v[2 * m], rr[0].v[2 * m + 1], rr[1].v[2 * m], rr[1].v[2 * m + 1]);
+      if (active) *reinterpret_cast<VC *>(bc.p + (int64_t)(P >> 1) * bc.xs + (int64_t)jc * bc.ys + lane * (V / 2)) = o;
+    } else {
+      rr0[0] = rr[0];
+      rr0[1] = rr[1];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      prev[q] = cur[q];
+      cur[q] = nxt[q];
+      gc[q] = gn[q];
     }
   }
 }
@@ -1388,6 +1480,41 @@ int launch_bottom(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_p
 
 }  // namespace
 
+// fine cells per lane of k_res_restrict_rows (whole rows, whole coarse
+// cells per lane, aligned vectors), or 0 when the tiled kernel must run
+int res_row_width(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc) {
+  const int nz = u->nz;
+  const int V = nz <= 128 ? 2 : nz <= 256 ? 4 : 0;
+  if (!V || nz % V || u->gx < 1 || u->gy < 1 || u->gz < 1) return 0;
+  const size_t w = u->dtype == OCU_F64 ? 8 : 4;
+  auto aligned = [&](const ocu_grid3d *g, int v) {
+    return g->pnz % v == 0 && ((int64_t)g->pny * g->pnz) % v == 0 &&
+           ((uintptr_t)g->base + (uintptr_t)g->shift * w) % (w * v) == 0;
+  };
+  return aligned(u, V) && aligned(b, V) && aligned(bc, V / 2) ? V : 0;
+}
+
+template <int V, typename T>
+int launch_res_restrict_rows_t(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
+                               double fy, double fz, hipStream_t stream) {
+  constexpr int NW = 8;
+  const int tiles_j = (bc->ny + NW - 1) / NW;
+  // about 4096 waves in all: enough to fill the chip, long marches
+  const int ichunk = std::max(1, std::min(bc->nx, (int)(((int64_t)bc->nx * bc->ny) / 4096)));
+  const int ntiles = tiles_j * ((bc->nx + ichunk - 1) / ichunk);
+  const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+  k_res_restrict_rows<T, V, NW><<<nblocks, dim3(64, NW), 0, stream>>>(dgrid<T>(u), dgrid<T>(b), dgrid<T>(bc),
+                                                                      res_c<T>(h, fx, fy, fz), tiles_j, ichunk,
+                                                                      ntiles);
+  OCU_HIP_LAUNCHED("ocu_mg_residual_restrict");
+}
+template <int V>
+int launch_res_restrict_rows(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
+                             double fy, double fz, hipStream_t stream) {
+  return u->dtype == OCU_F64 ? launch_res_restrict_rows_t<V, double>(u, b, bc, h, fx, fy, fz, stream)
+                             : launch_res_restrict_rows_t<V, float>(u, b, bc, h, fx, fy, fz, stream);
+}
+
 extern "C" {
 
 size_t ocu_mg_bottom_lds_bytes(const ocu_grid3d *u, int nlev, int coarse_mode) {
@@ -1512,6 +1639,13 @@ int ocu_mg_residual_restrict(const ocu_grid3d *u, const ocu_grid3d *b, const ocu
   }
   const int cnx = b_coarse->nx, cny = b_coarse->ny, cnz = b_coarse->nz;
   if ((int64_t)cnx * cny * cnz == 0) return OCU_OK;
+  if (!norm_ws && !r) {
+    switch (res_row_width(u, b, b_coarse)) {
+      case 2: return launch_res_restrict_rows<2>(u, b, b_coarse, h, fx, fy, fz, S(stream));
+      case 4: return launch_res_restrict_rows<4>(u, b, b_coarse, h, fx, fy, fz, S(stream));
+      default: break;
+    }
+  }
   int tiles_k = (cnz + TK - 1) / TK, tiles_j = (cny + TJ - 1) / TJ;
   int ntiles = tiles_k * tiles_j * cnx;
   dim3 block = stencil_block();

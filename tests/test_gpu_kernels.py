@@ -300,7 +300,8 @@ def test_relax_fused_rejects_periodic(gpu):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("size", [(16, 16, 16), (32, 8, 24), (64, 64, 64)])
+@pytest.mark.parametrize("size", [(16, 16, 16), (32, 8, 24), (64, 64, 64), (16, 10, 256), (8, 6, 200),
+                                  (12, 12, 132), (2, 2, 2), (4, 34, 130)])
 def test_residual_restrict(gpu, dtype, size):
     bc = BC_CASES["mixed"]
     h = 1 / 32
