@@ -792,6 +792,117 @@ __global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int q
   }
 }
 
+// Prolongation with whole rows: each wave owns one fine row J (V fine cells
+// per lane, V/2 coarse cells) and marches along the fine planes I of its
+// chunk.  The coarse rows it reads - rows jc and jc+sy (sy = +-1 toward the
+// fine row) of the coarse planes ic-1, ic, ic+1 - stay in registers as a
+// sliding window, with their z ghosts in g for the DPP shifts; the fine row
+// is read, corrected and written once.  Same operations as k_prolong,
+// including its corner choice and the level-1 ghost writes.
+template <typename T, int V>
+struct CRow {
+  Vec<T, V> v;
+  T g;
+};
+
+template <typename T, int V, int NW>
+__global__ __launch_bounds__(64 * NW) void k_prolong_rows(DGrid<T> uc, DGrid<T> uf, int quirk, GhostL1<T> gw,
+                                                          int wghost, int tiles_j, int ichunk, int ntiles) {
+  constexpr int VC = V / 2;
+  typedef Vec<T, V> VT;
+  typedef CRow<T, VC> CR;
+  const int per = gridDim.x / FXCD;
+  const int logical = (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD);
+  if (logical >= ntiles) return;  // uniform per block
+  const int tj = logical % tiles_j, ti = logical / tiles_j;
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int J = tj * NW + w;
+  if (J >= uf.ny) return;  // uniform per wave; no barriers in this kernel
+  const int nzc = uc.nz;
+  const int k0 = lane * V;
+  const bool active = k0 < uf.nz;
+  const int kc = active ? lane * VC : 0;
+  const int hi_lane = (uf.nz == 64 * V) ? 63 : uf.nz / V;
+  const int gk = lane == 0 ? -1 : nzc;
+  const bool g_in_data = lane == hi_lane && uf.nz < 64 * V;
+  const int jc = J >> 1, dy = J & 1;
+  const int I0 = 2 * (ti * ichunk), I1 = min(uf.nx, I0 + 2 * ichunk);
+  // coarse rows jc (r = 0) and jc + (dy ? 1 : -1) (r = 1)
+  const T *crow[2] = {uc.p + (int64_t)jc * uc.ys, uc.p + (int64_t)(jc + (dy ? 1 : -1)) * uc.ys};
+  auto load = [&](int r, int ic, CR &d) {
+    const T *p = crow[r] + (int64_t)ic * uc.xs;
+    d.v = *reinterpret_cast<const Vec<T, VC> *>(p + kc);
+    d.g = p[gk];
+    if (g_in_data) d.v.v[0] = d.g;
+  };
+  // window: coarse planes ic-1 (m), ic (c), ic+1 (p)
+  CR m[2], c[2], p[2];
+  int ic = I0 >> 1;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    load(r, ic - 1, m[r]);
+    load(r, ic, c[r]);
+    load(r, ic + 1, p[r]);
+  }
+  for (int I = I0; I < I1; ++I) {
+    const int dx = I & 1;
+    if (I > I0 && !dx) {  // entering coarse plane ic+1
+      ++ic;
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        m[r] = c[r];
+        c[r] = p[r];
+        load(r, ic + 1, p[r]);
+      }
+    }
+    const CR *X = dx ? p : m;  // the coarse plane toward the fine plane
+    // coarse values at kc-1 (toward dz = 0) and kc+VC (toward dz = 1) of
+    // each row: lane neighbours by DPP, the row's ghost at the ends
+    T zl[2][2], zh[2][2];  // [plane: 0 = c, 1 = X][row]
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      zl[0][r] = from_lower(c[r].v.v[VC - 1], c[r].g);
+      zh[0][r] = from_upper(c[r].v.v[0], c[r].g);
+      zl[1][r] = from_lower(X[r].v.v[VC - 1], X[r].g);
+      zh[1][r] = from_upper(X[r].v.v[0], X[r].g);
+    }
+    const bool qrow = quirk && dx == 0 && dy == 1;  // its dz = 1 children take the corner from plane ic+1
+    T qzh = 0;
+    if (qrow) qzh = from_upper(p[1].v.v[0], p[1].g);
+    T *frow = uf.p + (int64_t)I * uf.xs + (int64_t)J * uf.ys;
+    VT f = *reinterpret_cast<const VT *>(frow + (active ? k0 : 0));
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+      const int mm = e >> 1, dz = e & 1;
+      auto zs = [&](const CR &row, T lo, T hi) {  // coarse value toward the child in z
+        return dz ? (mm + 1 < VC ? row.v.v[mm + 1] : hi) : (mm > 0 ? row.v.v[mm - 1] : lo);
+      };
+      const T C = c[0].v.v[mm], Xv = X[0].v.v[mm], Y = c[1].v.v[mm];
+      const T Z = zs(c[0], zl[0][0], zh[0][0]);
+      const T XY = X[1].v.v[mm];
+      const T XZ = zs(X[0], zl[1][0], zh[1][0]);
+      const T YZ = zs(c[1], zl[0][1], zh[0][1]);
+      const T XYZ = (qrow && dz) ? (mm + 1 < VC ? p[1].v.v[mm + 1] : qzh) : zs(X[1], zl[1][1], zh[1][1]);
+      f.v[e] = f.v[e] + PW<T>::c0 * C + PW<T>::c1 * (Xv + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+    }
+    if (active) {
+      *reinterpret_cast<VT *>(frow + k0) = f;
+      if (wghost) {
+        const bool xl = I == 0, xh = I == uf.nx - 1, yl = J == 0, yh = J == uf.ny - 1;
+        write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, 0);
+        if (yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, -1);
+        if (yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, 1);
+        if (xl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, 0);
+        if (xh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, 0);
+        if (xl && yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, -1);
+        if (xl && yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, 1);
+        if (xh && yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, -1);
+        if (xh && yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, 1);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // coarsest level in one workgroup (LDS resident)
 // ------------------------------------------------------------------------
@@ -1388,6 +1499,55 @@ int relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3
               : launch_fused<float, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream));
 }
 
+// fine cells per lane of k_res_restrict_rows (whole rows, whole coarse
+// cells per lane, aligned vectors), or 0 when the tiled kernel must run
+int res_row_width(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc) {
+  const int nz = u->nz;
+  const int V = nz <= 128 ? 2 : nz <= 256 ? 4 : 0;
+  if (!V || nz % V || u->gx < 1 || u->gy < 1 || u->gz < 1) return 0;
+  const size_t w = u->dtype == OCU_F64 ? 8 : 4;
+  auto aligned = [&](const ocu_grid3d *g, int v) {
+    return g->pnz % v == 0 && ((int64_t)g->pny * g->pnz) % v == 0 &&
+           ((uintptr_t)g->base + (uintptr_t)g->shift * w) % (w * v) == 0;
+  };
+  return aligned(u, V) && aligned(b, V) && aligned(bc, V / 2) ? V : 0;
+}
+
+template <int V, typename T>
+int launch_res_restrict_rows_t(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
+                               double fy, double fz, hipStream_t stream) {
+  constexpr int NW = 8;
+  const int tiles_j = (bc->ny + NW - 1) / NW;
+  // about 4096 waves in all: enough to fill the chip, long marches
+  const int ichunk = std::max(1, std::min(bc->nx, (int)(((int64_t)bc->nx * bc->ny) / 4096)));
+  const int ntiles = tiles_j * ((bc->nx + ichunk - 1) / ichunk);
+  const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+  k_res_restrict_rows<T, V, NW><<<nblocks, dim3(64, NW), 0, stream>>>(dgrid<T>(u), dgrid<T>(b), dgrid<T>(bc),
+                                                                      res_c<T>(h, fx, fy, fz), tiles_j, ichunk,
+                                                                      ntiles);
+  OCU_HIP_LAUNCHED("ocu_mg_residual_restrict");
+}
+template <int V>
+int launch_res_restrict_rows(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
+                             double fy, double fz, hipStream_t stream) {
+  return u->dtype == OCU_F64 ? launch_res_restrict_rows_t<V, double>(u, b, bc, h, fx, fy, fz, stream)
+                             : launch_res_restrict_rows_t<V, float>(u, b, bc, h, fx, fy, fz, stream);
+}
+
+template <typename T, int V>
+int launch_prolong_rows(const ocu_grid3d *uc, const ocu_grid3d *uf, int quirk, const GhostL1<T> &gw, int wghost,
+                        hipStream_t stream) {
+  constexpr int NW = 8;
+  const int tiles_j = (uf->ny + NW - 1) / NW;
+  // coarse planes per wave: about 4096 waves in all
+  const int ichunk = std::max(1, std::min(uc->nx, (int)(((int64_t)uc->nx * uf->ny) / 4096)));
+  const int ntiles = tiles_j * ((uc->nx + ichunk - 1) / ichunk);
+  const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+  k_prolong_rows<T, V, NW><<<nblocks, dim3(64, NW), 0, stream>>>(dgrid<T>(uc), dgrid<T>(uf), quirk, gw, wghost,
+                                                                 tiles_j, ichunk, ntiles);
+  OCU_HIP_LAUNCHED("ocu_mg_prolong");
+}
+
 int prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, const ocu_bc_set *bc, double hx,
             double hy, double hz, void *stream) {
   if (int e = check_grid(u_coarse, "ocu_mg_prolong(coarse)")) return e;
@@ -1410,6 +1570,22 @@ int prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, con
   if ((int64_t)u_coarse->nx * u_coarse->ny * u_coarse->nz == 0) return OCU_OK;
   static const ocu_bc_set none = {};
   const ocu_bc_set *fb = bc ? bc : &none;
+  if (u_fine->nx == 2 * u_coarse->nx && u_fine->ny == 2 * u_coarse->ny && u_fine->nz == 2 * u_coarse->nz) {
+    const bool f64 = u_fine->dtype == OCU_F64;
+    switch (res_row_width(u_fine, u_fine, u_coarse)) {
+      case 2:
+        return f64 ? launch_prolong_rows<double, 2>(u_coarse, u_fine, quirk, ghost_l1<double>(fb, hx, hy, hz), wghost,
+                                                     S(stream))
+                   : launch_prolong_rows<float, 2>(u_coarse, u_fine, quirk, ghost_l1<float>(fb, hx, hy, hz), wghost,
+                                                    S(stream));
+      case 4:
+        return f64 ? launch_prolong_rows<double, 4>(u_coarse, u_fine, quirk, ghost_l1<double>(fb, hx, hy, hz), wghost,
+                                                     S(stream))
+                   : launch_prolong_rows<float, 4>(u_coarse, u_fine, quirk, ghost_l1<float>(fb, hx, hy, hz), wghost,
+                                                    S(stream));
+      default: break;
+    }
+  }
   dim3 grid = stencil_grid(u_coarse->nx, u_coarse->ny, u_coarse->nz);
   if (u_fine->dtype == OCU_F64)
     k_prolong<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u_coarse), dgrid<double>(u_fine), quirk,
@@ -1479,41 +1655,6 @@ int launch_bottom(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_p
 }
 
 }  // namespace
-
-// fine cells per lane of k_res_restrict_rows (whole rows, whole coarse
-// cells per lane, aligned vectors), or 0 when the tiled kernel must run
-int res_row_width(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc) {
-  const int nz = u->nz;
-  const int V = nz <= 128 ? 2 : nz <= 256 ? 4 : 0;
-  if (!V || nz % V || u->gx < 1 || u->gy < 1 || u->gz < 1) return 0;
-  const size_t w = u->dtype == OCU_F64 ? 8 : 4;
-  auto aligned = [&](const ocu_grid3d *g, int v) {
-    return g->pnz % v == 0 && ((int64_t)g->pny * g->pnz) % v == 0 &&
-           ((uintptr_t)g->base + (uintptr_t)g->shift * w) % (w * v) == 0;
-  };
-  return aligned(u, V) && aligned(b, V) && aligned(bc, V / 2) ? V : 0;
-}
-
-template <int V, typename T>
-int launch_res_restrict_rows_t(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
-                               double fy, double fz, hipStream_t stream) {
-  constexpr int NW = 8;
-  const int tiles_j = (bc->ny + NW - 1) / NW;
-  // about 4096 waves in all: enough to fill the chip, long marches
-  const int ichunk = std::max(1, std::min(bc->nx, (int)(((int64_t)bc->nx * bc->ny) / 4096)));
-  const int ntiles = tiles_j * ((bc->nx + ichunk - 1) / ichunk);
-  const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
-  k_res_restrict_rows<T, V, NW><<<nblocks, dim3(64, NW), 0, stream>>>(dgrid<T>(u), dgrid<T>(b), dgrid<T>(bc),
-                                                                      res_c<T>(h, fx, fy, fz), tiles_j, ichunk,
-                                                                      ntiles);
-  OCU_HIP_LAUNCHED("ocu_mg_residual_restrict");
-}
-template <int V>
-int launch_res_restrict_rows(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
-                             double fy, double fz, hipStream_t stream) {
-  return u->dtype == OCU_F64 ? launch_res_restrict_rows_t<V, double>(u, b, bc, h, fx, fy, fz, stream)
-                             : launch_res_restrict_rows_t<V, float>(u, b, bc, h, fx, fy, fz, stream);
-}
 
 extern "C" {
 

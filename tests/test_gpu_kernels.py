@@ -261,7 +261,8 @@ def assert_ghost_layer_bitexact(got, want, what):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("case", ["neumann0", "dirichlet", "neumann_v"])
-@pytest.mark.parametrize("cn", [(8, 6, 10), (1, 2, 3), (16, 16, 32)])
+@pytest.mark.parametrize("cn", [(8, 6, 10), (1, 2, 3), (16, 16, 32), (3, 5, 128), (4, 3, 100), (2, 9, 66),
+                                (5, 4, 31)])
 def test_prolong_bc(gpu, dtype, case, cn):
     bc = BC_CASES[case]
     hx, hy, hz = .25, .125, .5
@@ -337,8 +338,8 @@ def test_residual_restrict(gpu, dtype, size):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("quirk", [0, 1])
-def test_prolong(gpu, dtype, quirk):
-    cn = (8, 6, 10)
+@pytest.mark.parametrize("cn", [(8, 6, 10), (3, 5, 128), (4, 3, 100), (1, 1, 1)])
+def test_prolong(gpu, dtype, quirk, cn):
     bc = BC_CASES["mixed"]
     uc = filled(*cn, dtype, 61, bc, .1)
     uf = random_grid(*(2 * s for s in cn), dtype, 62)
