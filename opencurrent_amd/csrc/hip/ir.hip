@@ -2,7 +2,10 @@
 // scaled cast to FP32, the FP64 correction, and interior sums.  No reference
 // counterpart: OpenCurrent's mixed solver (sol_mgpressuremixed3ddevd.cpp:
 // 79-100) runs an FP32 FMG and then a full FP64 FMG instead.
+#include <algorithm>
+
 #include "ocu_hip_common.h"
+#include "rows.h"
 
 using namespace ocu_hip;
 
@@ -46,6 +49,90 @@ __global__ __launch_bounds__(256) void k_ir_residual(DGrid<double> x, DGrid<doub
     parts[3 * blockIdx.x] = a;
     parts[3 * blockIdx.x + 1] = m;
     parts[3 * blockIdx.x + 2] = s;
+  }
+}
+
+// The same residual with whole rows: each wave owns a row j (V cells per
+// lane) and marches along the planes of its i-chunk with planes i-1, i, i+1
+// of its row in registers; rows j-1 and j+1 of plane i are read beside them
+// (L2 hits: other waves' rows) and k-neighbours come by DPP with the row's z
+// ghosts in g.  Exactly NORM_PARTS blocks (idle ones write zero partials);
+// tiles are dealt to the XCDs in contiguous runs.
+constexpr int IRR_NW = 4;
+template <int V>
+__global__ __launch_bounds__(64 * IRR_NW) void k_ir_residual_rows(DGrid<double> x, DGrid<double> b, double bshift,
+                                                                 DGrid<float> r32, double scale, ResC64 c,
+                                                                 int tiles_j, int ichunk, int ntiles,
+                                                                 double *parts) {
+  typedef Vec<double, V> VT;
+  typedef Vec<float, V> VF;
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int nz = x.nz;
+  const int k0 = lane * V;
+  const bool active = k0 < nz;
+  const int kc = active ? k0 : 0;
+  const int hi_lane = (nz == 64 * V) ? 63 : nz / V;
+  const int gk = lane == 0 ? -1 : nz;
+  const bool g_in_data = lane == hi_lane && nz < 64 * V;
+  double s2 = 0, mx = 0, sm = 0;
+  const int per = gridDim.x / 8;
+  const int first = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  for (int t = first; t < ntiles; t += gridDim.x) {
+    const int tj = t % tiles_j, ti = t / tiles_j;
+    const int j = tj * IRR_NW + w;
+    if (j >= x.ny) continue;  // per wave; the only barrier is after the loop
+    const int i0 = ti * ichunk, i1 = min(x.nx, i0 + ichunk);
+    const double *xr = x.p + (int64_t)j * x.ys;
+    const double *br = b.p + (int64_t)j * b.ys + kc;
+    float *rr = r32.p + (int64_t)j * r32.ys + k0;
+    auto row = [&](int i, int dj) {
+      return *reinterpret_cast<const VT *>(xr + (int64_t)i * x.xs + (int64_t)dj * x.ys + kc);
+    };
+    auto rowg = [&](int i, VT &d, double &g) {
+      const double *p = xr + (int64_t)i * x.xs;
+      d = *reinterpret_cast<const VT *>(p + kc);
+      g = p[gk];
+      if (g_in_data) d.v[0] = g;
+    };
+    VT prev = row(i0 - 1, 0), cur, nxt;
+    double gc, gn;
+    rowg(i0, cur, gc);
+    for (int i = i0; i < i1; ++i) {
+      rowg(i + 1, nxt, gn);
+      const VT ym = row(i, -1), yp = row(i, 1);
+      const VT bb = *reinterpret_cast<const VT *>(br + (int64_t)i * b.xs);
+      const double lo = from_lower(cur.v[V - 1], gc), hi = from_upper(cur.v[0], gc);
+      VF o;
+#pragma unroll
+      for (int e = 0; e < V; e++) {
+        const double zm = e > 0 ? cur.v[e - 1] : lo, zp = e < V - 1 ? cur.v[e + 1] : hi;
+        const double r = (bb.v[e] - bshift) - (c.fz * (zm + zp) + c.fy * (ym.v[e] + yp.v[e]) +
+                                               c.fx * (prev.v[e] + nxt.v[e]) - c.diag * cur.v[e]);
+        o.v[e] = (float)(r * scale);
+        if (active) {
+          s2 += r * r;
+          mx = fmax(mx, fabs(r));
+          sm += r;
+        }
+      }
+      if (active) *reinterpret_cast<VF *>(rr + (int64_t)i * r32.xs) = o;
+      prev = cur;
+      cur = nxt;
+      gc = gn;
+    }
+  }
+  __shared__ double sh[3][IRR_NW];
+  s2 = wave_sum(s2);
+  mx = wave_max(mx);
+  sm = wave_sum(sm);
+  if (lane == 0) { sh[0][w] = s2; sh[1][w] = mx; sh[2][w] = sm; }
+  __syncthreads();
+  if (lane == 0 && w == 0) {
+    double a = 0, m = 0, q = 0;
+    for (int v = 0; v < IRR_NW; v++) { a += sh[0][v]; m = fmax(m, sh[1][v]); q += sh[2][v]; }
+    parts[3 * blockIdx.x] = a;
+    parts[3 * blockIdx.x + 1] = m;
+    parts[3 * blockIdx.x + 2] = q;
   }
 }
 
@@ -126,9 +213,27 @@ int ocu_ir_residual(const ocu_grid3d *x, const ocu_grid3d *b, double bshift, con
                   x->nz == r32->nz,
               "ocu_ir_residual: dimension mismatch");
   OCU_REQUIRE(x->gx >= 1 && x->gy >= 1 && x->gz >= 1, "ocu_ir_residual: x needs ghosts");
+  ResC64 c{fx / (h * h), fy / (h * h), fz / (h * h), 2 * (fx + fy + fz) / (h * h)};
+  const int nz = x->nz;
+  const int V = nz <= 64 ? 1 : nz <= 128 ? 2 : nz <= 256 ? 4 : 0;
+  if (V && nz % V == 0 && row_aligned(x, V) && row_aligned(b, V) && row_aligned(r32, V) &&
+      (int64_t)x->nx * x->ny * nz > 0) {
+    const int tiles_j = (x->ny + IRR_NW - 1) / IRR_NW;
+    // one tile per block where the grid allows
+    const int ichunk = std::max(1, (int)(((int64_t)x->nx * tiles_j + NORM_PARTS - 1) / NORM_PARTS));
+    const int ntiles = tiles_j * ((x->nx + ichunk - 1) / ichunk);
+#define IRR(VV)                                                                                                      \
+  k_ir_residual_rows<VV><<<NORM_PARTS, dim3(64, IRR_NW), 0, S(stream)>>>(dgrid<double>(x), dgrid<double>(b), bshift, \
+                                                                        dgrid<float>(r32), scale, c, tiles_j,      \
+                                                                        ichunk, ntiles, ws)
+    if (V == 1) IRR(1);
+    else if (V == 2) IRR(2);
+    else IRR(4);
+#undef IRR
+    OCU_HIP_LAUNCHED("ocu_ir_residual");
+  }
   int tiles_k = (x->nz + TK - 1) / TK, tiles_j = (x->ny + TJ - 1) / TJ;
   int ntiles = tiles_k * tiles_j * x->nx;
-  ResC64 c{fx / (h * h), fy / (h * h), fz / (h * h), 2 * (fx + fy + fz) / (h * h)};
   k_ir_residual<<<NORM_PARTS, stencil_block(), 0, S(stream)>>>(dgrid<double>(x), dgrid<double>(b), bshift,
                                                                 dgrid<float>(r32), scale, c, tiles_k, tiles_j, ntiles, ws);
   OCU_HIP_LAUNCHED("ocu_ir_residual");

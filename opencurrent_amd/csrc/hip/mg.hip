@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "ghost_l1.h"
+#include "rows.h"
 #include "ocu_hip_common.h"
 
 using namespace ocu_hip;
@@ -256,39 +257,6 @@ __global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T>
 // ring of two rows each side) and marches along i like the tiled kernel:
 // first colour of plane p on the ring-1 rows from U0, second colour of plane
 // p-1 on the output rows from U1, j-neighbours through LDS.
-
-template <typename T, int V>
-struct alignas(sizeof(T) * V) Vec {
-  T v[V];
-};
-
-// DPP wave_shr:1 / wave_shl:1; a lane whose source is off the wave keeps old
-__device__ __forceinline__ int dpp_lower_i(int src, int old) {
-  return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xf, 0xf, false);
-}
-__device__ __forceinline__ int dpp_upper_i(int src, int old) {
-  return __builtin_amdgcn_update_dpp(old, src, 0x130, 0xf, 0xf, false);
-}
-// value of lane l-1 (lane 0: old)
-__device__ __forceinline__ float from_lower(float src, float old) {
-  return __int_as_float(dpp_lower_i(__float_as_int(src), __float_as_int(old)));
-}
-__device__ __forceinline__ double from_lower(double src, double old) {
-  long long s = __double_as_longlong(src), o = __double_as_longlong(old);
-  int lo = dpp_lower_i((int)s, (int)o);
-  int hi = dpp_lower_i((int)(s >> 32), (int)(o >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-// value of lane l+1 (lane 63: old)
-__device__ __forceinline__ float from_upper(float src, float old) {
-  return __int_as_float(dpp_upper_i(__float_as_int(src), __float_as_int(old)));
-}
-__device__ __forceinline__ double from_upper(double src, double old) {
-  long long s = __double_as_longlong(src), o = __double_as_longlong(old);
-  int lo = dpp_upper_i((int)s, (int)o);
-  int hi = dpp_upper_i((int)(s >> 32), (int)(o >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 // One ghost row of o derived from the row (q,j) of the cells this lane wrote
 // (k0 .. k0+V-1, values v): the row at offset (sx,sy) gets the x/y face

@@ -556,3 +556,41 @@ def test_laplacian_thermal(gpu, dtype):
         gd2, gt = D().DeviceGrid.from_host(d), D().DeviceGrid.from_host(t)
         D().thermal_force(gd2, gt, nx, ny, nz, vd, 7.1e4)
         assert_bitexact(gd2.to_host(), want2, f"thermal {vd}")
+
+
+@pytest.mark.parametrize("size", [(16, 12, 20), (8, 6, 256), (5, 7, 200), (3, 4, 130), (4, 9, 64), (2, 3, 1)])
+def test_ir_residual(gpu, size):
+    """ocu_ir_residual: r = (b - bshift) - A x in FP64 with the reference
+    residual's operand order, r32 = (float)(r * scale) bit for bit (numpy
+    performs the same IEEE double operations in the same order), and the
+    (sum r^2, max |r|, sum r) partials reduced by ocu_norm3_finalize."""
+    import torch
+    from opencurrent_amd import _lib
+    h, fx, fy, fz, bshift, scale = 1 / 32, 1.0, .64, .25, .0123, 2.0 ** 7
+    x = random_grid(*size, np.float64, 151)
+    b = random_grid(*size, np.float64, 152)
+    r32 = HostGrid(*size, dtype=np.float32)
+    X = x.full()
+    n = size
+    cx, cy, cz, cd = fx / (h * h), fy / (h * h), fz / (h * h), 2 * (fx + fy + fz) / (h * h)
+    xc = X[1:n[0] + 1, 1:n[1] + 1, 1:n[2] + 1]
+    zm, zp = X[1:n[0] + 1, 1:n[1] + 1, 0:n[2]], X[1:n[0] + 1, 1:n[1] + 1, 2:n[2] + 2]
+    ym, yp = X[1:n[0] + 1, 0:n[1], 1:n[2] + 1], X[1:n[0] + 1, 2:n[1] + 2, 1:n[2] + 1]
+    xm, xp = X[0:n[0], 1:n[1] + 1, 1:n[2] + 1], X[2:n[0] + 2, 1:n[1] + 1, 1:n[2] + 1]
+    r = (b.interior() - bshift) - (cz * (zm + zp) + cy * (ym + yp) + cx * (xm + xp) - cd * xc)
+    gx, gb, gr = (D().DeviceGrid.from_host(g) for g in (x, b, r32))
+    hip = _lib.hip()
+    ws = torch.zeros(3 * 2048 + 3, dtype=torch.float64, device="cuda")
+    out = ws[3 * 2048:]
+    _lib.check(hip.ocu_ir_residual(gx.ptr(), gb.ptr(), bshift, gr.ptr(), scale, h, fx, fy, fz,
+                                   C.cast(ws.data_ptr(), _lib.DP), None), "ir_residual")
+    _lib.check(hip.ocu_norm3_finalize(C.cast(ws.data_ptr(), _lib.DP), C.cast(out.data_ptr(), _lib.DP), None),
+               "norm3_finalize")
+    torch.cuda.synchronize()
+    got = gr.to_host().interior()
+    want = (r * scale).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    s2, mx, sm = (float(v) for v in out.cpu())
+    assert mx == np.abs(r).max()
+    assert abs(s2 - (r * r).sum()) <= 1e-12 * (r * r).sum()
+    assert abs(sm - r.sum()) <= 1e-12 * np.abs(r).sum()
