@@ -30,6 +30,11 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "MG time to 1e-10 rel. residual + NS step Mcells/s, FP32-refined vs FP64"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the level-0 fused sweep at 256^3, measured by two
+# rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) and scaled by what the
+# same counters read for a plain copy of known size; recorded with the
+# profile summary profiles/level0_kernels_256_hbm_traffic.md
+SWEEP_TRAFFIC_256 = 241_400_000
 
 
 def parse():
@@ -449,7 +454,11 @@ def main():
     kms, kbytes = time_fused_sweep(n, args.kernel_reps)
     achieved = kbytes / (kms / 1e3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "frac": round(achieved / HBM_PEAK_GBS, 4),
+                       "traffic": SWEEP_TRAFFIC_256 if n == 256 else None,
+                       "traffic_source": "rocprofv3 FETCH_SIZE x 1.999 + WRITE_SIZE x 1.000 per launch, scaled "
+                                         "on a plain copy (profiles/level0_kernels_256_hbm_traffic.md)"
+                       if n == 256 else None,
                        "kernel": "k_relax_fused_rows<float,4,1,12> (level-0 red+black sweep with ghost writes)",
                        "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes}
 
