@@ -170,6 +170,8 @@ def time_ns_step(steps=10, warmup=2, n=128):
         u, v, w, t = fields.mac_grids(n, n, n)
         z = (np.arange(n) + .5) / n
         t.interior()[:] = (1 - z)[None, None, :] + 1e-3 * (2 * fields.uniform(2001, t.interior().shape) - 1)
+        if not mixed:
+            p_fp64, init = p, (u, v, w, t)
         ns = solvers.IncompressibleNS3D(p, u, v, w, t)
         for _ in range(warmup):
             ns.advance_one_step(ns.get_max_stable_timestep())
@@ -185,7 +187,30 @@ def time_ns_step(steps=10, warmup=2, n=128):
         del ns
     out["workload"] = f"Rayleigh-Benard {n}^3, Ra 1e5, Pr 0.71, upwind, CFL 0.5, FE, projection to 1e-8 Linf"
     out["speedup_fp32_refined"] = round(out["fp64"]["ms_per_step"] / out["fp32_refined"]["ms_per_step"], 3)
+    out["cpu_oracle_fp64"] = cpu_ns_step(n, p_fp64, init)
     return out
+
+
+def cpu_ns_step(n, p, init, steps=3):
+    """the oracle's FP64 NS step (the reference's step restated, OpenMP) on
+    the same problem, on the host cores: a few steps after one warm-up"""
+    import oracle_lib as O
+    from opencurrent_amd._types import dtype_code
+    u, v, w, t = init
+    h = O.lib().orc_ns_create(dtype_code(np.float64), C.byref(p), u.ptr(), v.ptr(), w.ptr(), t.ptr())
+    if not h:
+        return None
+    try:
+        O.lib().orc_ns_advance_one_step(h, O.lib().orc_ns_max_stable_timestep(h))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            O.lib().orc_ns_advance_one_step(h, O.lib().orc_ns_max_stable_timestep(h))
+        dt = (time.perf_counter() - t0) / steps
+    finally:
+        O.lib().orc_ns_destroy(h)
+    return {"ms_per_step": round(dt * 1e3, 1), "mcells_s": round(n ** 3 / dt / 1e6, 2),
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            "sample": f"{steps} steps after one warm-up"}
 
 
 def time_adjoint(n=128, steps=20):

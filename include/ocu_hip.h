@@ -119,6 +119,16 @@ int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color,
  * periodic (the reference refreshes ghosts between colours only for periodic
  * faces, sol_mgpressure3ddev.cu:227-241); returns OCU_ERR_UNSUPPORTED if a
  * face of `bc` is periodic. */
+/* ocu_mg_relax (OCU_RELAX_DEVICE) that also writes the level-1 ghosts of
+ * the cells it updates, so that relax-colour + apply_3d_boundary_conditions_
+ * level1(u, bc, hx, hy, hz) (the reference's fill between colours for
+ * periodic faces, sol_mgpressure3ddev.cu:227-241) is one launch.  Equal to
+ * the pair bit for bit when u's ghosts were filled before the call.  Faces
+ * Neumann, Dirichlet or periodic in pairs with an even number of cells along
+ * a periodic axis (else OCU_ERR_UNSUPPORTED).  No reference counterpart as
+ * one call. */
+int ocu_mg_relax_bc(const ocu_grid3d *u, const ocu_grid3d *b, int color, const ocu_mg_level_params *p,
+                    const ocu_bc_set *bc, double hx, double hy, double hz, void *stream);
 int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                        const ocu_mg_level_params *p, const ocu_bc_set *bc, int first_color,
                        void *stream);

@@ -72,6 +72,7 @@ HIP_SIGS = {
     "ocu_bc_fill_mac_level1": (I, [GP, GP, GP, BP, P]),
     "ocu_mg_relax": (I, [GP, GP, I, MP, I, P]),
     "ocu_mg_relax_fused": (I, [GP, GP, GP, MP, BP, I, P]),
+    "ocu_mg_relax_bc": (I, [GP, GP, I, MP, BP, D, D, D, P]),
     "ocu_mg_relax_fused_bc": (I, [GP, GP, GP, MP, BP, D, D, D, I, I, P]),
     "ocu_mg_residual": (I, [GP, GP, GP, D, D, D, D, P]),
     "ocu_mg_restrict": (I, [GP, GP, P]),

@@ -67,6 +67,73 @@ struct GhostL1 {
   }
 };
 
+// Faces may also be periodic in pairs: the level-1 fill copies across a
+// periodic axis (ghost -1 from cell n-1, ghost n from cell 0) and transforms
+// across the others, x then y then z, so every ghost still derives from one
+// interior cell.  write() stores every ghost that derives from cell (i,j,k).
+template <typename T>
+struct GhostP {
+  GhostL1<T> t;
+  int per[3];  // x, y, z axis periodic
+  __device__ void write(const DGrid<T> &g, int i, int j, int k, T v) const {
+    const int n[3] = {g.nx, g.ny, g.nz}, c[3] = {i, j, k};
+    int off[3][3], face[3][3], cnt[3];
+    bool edge = false;
+    for (int a = 0; a < 3; a++) {
+      off[a][0] = 0;
+      face[a][0] = -1;
+      cnt[a] = 1;
+      if (c[a] == 0) {  // the low ghost (or, periodic, the high one) derives from here
+        off[a][cnt[a]] = per[a] ? n[a] : -1;
+        face[a][cnt[a]++] = per[a] ? -1 : 2 * a + 1;
+      }
+      if (c[a] == n[a] - 1) {
+        off[a][cnt[a]] = per[a] ? -n[a] : 1;
+        face[a][cnt[a]++] = per[a] ? -1 : 2 * a;
+      }
+      edge = edge || cnt[a] > 1;
+    }
+    if (!edge) return;
+    for (int ax = 0; ax < cnt[0]; ax++)
+      for (int ay = 0; ay < cnt[1]; ay++)
+        for (int az = 0; az < cnt[2]; az++) {
+          if (!ax && !ay && !az) continue;
+          T w = v;
+          if (face[0][ax] >= 0) w = t.xf(face[0][ax], w);
+          if (face[1][ay] >= 0) w = t.xf(face[1][ay], w);
+          if (face[2][az] >= 0) w = t.xf(face[2][az], w);
+          g.at(i + off[0][ax], j + off[1][ay], k + off[2][az]) = w;
+        }
+  }
+};
+
+// GhostP applies to the level-1 fill of bc when every face is Neumann,
+// Dirichlet or periodic with its opposite face
+inline bool gfaces_paired(const ocu_bc_set *bc) {
+  const ocu_bc *lo[3] = {&bc->xneg, &bc->yneg, &bc->zneg}, *hi[3] = {&bc->xpos, &bc->ypos, &bc->zpos};
+  for (int a = 0; a < 3; a++) {
+    const bool pl = lo[a]->type == OCU_BC_PERIODIC, ph = hi[a]->type == OCU_BC_PERIODIC;
+    if (pl != ph) return false;
+    if (pl) continue;
+    for (const ocu_bc *b : {lo[a], hi[a]})
+      if (b->type != OCU_BC_NEUMANN && b->type != OCU_BC_DIRICHELET) return false;
+  }
+  return true;
+}
+
+template <typename T>
+inline GhostL1<T> ghost_l1(const ocu_bc_set *bc, double hx, double hy, double hz);
+
+template <typename T>
+inline GhostP<T> ghost_p(const ocu_bc_set *bc, double hx, double hy, double hz) {
+  GhostP<T> g;
+  g.t = ghost_l1<T>(bc, hx, hy, hz);
+  g.per[0] = bc->xpos.type == OCU_BC_PERIODIC;
+  g.per[1] = bc->ypos.type == OCU_BC_PERIODIC;
+  g.per[2] = bc->zpos.type == OCU_BC_PERIODIC;
+  return g;
+}
+
 template <typename T>
 inline GhostL1<T> ghost_l1(const ocu_bc_set *bc, double hx, double hy, double hz) {
   GhostL1<T> g;

@@ -124,9 +124,15 @@ __device__ __forceinline__ T relax_pt_host(const HostC<T> &c, T uc, T zm, T zp, 
 // One thread per cell of the (k,j) tile; cells of the other colour idle so
 // every row is read with full 64-lane coalescing.  The device variants only
 // update k < 2*(nz/2) (the reference launches nz/2 threads along k).
+// wg: also write the level-1 ghosts that derive from each updated cell
+// (GhostP), which equals running the level-1 fill after the colour when the
+// ghosts were filled before it and every periodic extent is even: a
+// non-periodic face ghost is read only by its own source cell, and across an
+// even periodic axis the copies of cells 0 and n-1 are read by cells of the
+// other colour.
 template <typename T>
 __global__ __launch_bounds__(256) void k_relax_dev(DGrid<T> u, DGrid<T> b, typename DevCFor<T>::type c,
-                                                   int color) {
+                                                   int color, GhostP<T> gp, int wg) {
   int k = blockIdx.x * TK + threadIdx.x;
   int j = blockIdx.y * TJ + threadIdx.y;
   int i = blockIdx.z;
@@ -136,8 +142,10 @@ __global__ __launch_bounds__(256) void k_relax_dev(DGrid<T> u, DGrid<T> b, typen
   T *p = u.p;
   T uc = p[id];
   T d = diag_dev<T>(c, i, j, k, u.nx, u.ny, u.nz);
-  p[id] = relax_pt(c, uc, p[id - 1], p[id + 1], p[id - u.ys], p[id + u.ys], p[id - u.xs], p[id + u.xs],
-                   b.p[b.idx(i, j, k)], d);
+  const T v = relax_pt(c, uc, p[id - 1], p[id + 1], p[id - u.ys], p[id + u.ys], p[id - u.xs], p[id + u.xs],
+                       b.p[b.idx(i, j, k)], d);
+  p[id] = v;
+  if (wg) gp.write(u, i, j, k, v);
 }
 
 template <typename T>
@@ -1675,9 +1683,11 @@ int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color, const ocu_
   dim3 grid = stencil_grid(u->nx, u->ny, u->nz);
   if (variant == OCU_RELAX_DEVICE) {
     if (u->dtype == OCU_F64)
-      k_relax_dev<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), dev_c64(p), color);
+      k_relax_dev<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), dev_c64(p), color,
+                                                                   GhostP<double>{}, 0);
     else
-      k_relax_dev<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), dev_c32(p), color);
+      k_relax_dev<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), dev_c32(p), color,
+                                                                  GhostP<float>{}, 0);
   } else if (variant == OCU_RELAX_HOST) {
     if (u->dtype == OCU_F64)
       k_relax_host<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), host_c<double>(p), color);
@@ -1688,6 +1698,35 @@ int ocu_mg_relax(const ocu_grid3d *u, const ocu_grid3d *b, int color, const ocu_
     return OCU_ERR_INVALID;
   }
   OCU_HIP_LAUNCHED("ocu_mg_relax");
+}
+
+int ocu_mg_relax_bc(const ocu_grid3d *u, const ocu_grid3d *b, int color, const ocu_mg_level_params *p,
+                    const ocu_bc_set *bc, double hx, double hy, double hz, void *stream) {
+  const char *what = "ocu_mg_relax_bc";
+  if (int e = check_pair(u, b, what)) return e;
+  OCU_REQUIRE(p && bc, "%s: null argument", what);
+  OCU_REQUIRE(color == 0 || color == 1, "%s: bad color %d", what, color);
+  if (!gfaces_paired(bc)) {
+    set_error("%s: ghost writes need Neumann or Dirichlet faces or periodic face pairs", what);
+    return OCU_ERR_UNSUPPORTED;
+  }
+  const int n[3] = {u->nx, u->ny, u->nz};
+  const int per[3] = {bc->xpos.type == OCU_BC_PERIODIC, bc->ypos.type == OCU_BC_PERIODIC,
+                      bc->zpos.type == OCU_BC_PERIODIC};
+  for (int a = 0; a < 3; a++)
+    if (per[a] && n[a] % 2) {
+      set_error("%s: a periodic axis needs an even number of cells (%d)", what, n[a]);
+      return OCU_ERR_UNSUPPORTED;
+    }
+  if ((int64_t)u->nx * u->ny * u->nz == 0) return OCU_OK;
+  dim3 grid = stencil_grid(u->nx, u->ny, u->nz);
+  if (u->dtype == OCU_F64)
+    k_relax_dev<double><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<double>(u), dgrid<double>(b), dev_c64(p), color,
+                                                                 ghost_p<double>(bc, hx, hy, hz), 1);
+  else
+    k_relax_dev<float><<<grid, stencil_block(), 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), dev_c32(p), color,
+                                                                ghost_p<float>(bc, hx, hy, hz), 1);
+  OCU_HIP_LAUNCHED(what);
 }
 
 int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,

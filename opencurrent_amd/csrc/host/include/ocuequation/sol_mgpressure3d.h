@@ -134,6 +134,10 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   virtual void clear_zero(int level);
 
   void relax_colour(int level, int color);
+  // one colour that also writes the level-1 ghosts it changes (no fill
+  // launch); relax_bc_ok: the level's faces allow it
+  void relax_colour_bc(int level, int color);
+  bool relax_bc_ok(int level) const;
   void relax_coarsest(int level, int iterations, RelaxOrder order);
   bool defer_ok(int level) const;
   void materialize(int level);

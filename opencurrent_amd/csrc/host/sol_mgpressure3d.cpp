@@ -445,6 +445,39 @@ void Sol_MultigridPressure3DDevice<T>::relax_colour(int level, int color) {
             "Sol_MultigridPressure3DDevice::relax");
 }
 
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::relax_colour_bc(int level, int color) {
+  materialize(level);
+  ocu_mg_level_params p;
+  level_params(level, &p);
+  ocu_bc_set cbc = to_c_bc(get_bc_at_level(level));
+  ocu_grid3d u = get_u(level).desc(), b = get_b(level).desc();
+  ok_status(ocu_mg_relax_bc(&u, &b, color, &p, &cbc, hx(level), hy(level), hz(level),
+                            ThreadManager::get_compute_stream()),
+            "Sol_MultigridPressure3DDevice::relax");
+}
+
+// the colour kernel can write the fill's ghosts itself when every face is
+// Neumann or Dirichlet, or periodic with its opposite face across an even
+// number of cells (the conditions of ocu_mg_relax_bc)
+template <typename T>
+bool Sol_MultigridPressure3DDevice<T>::relax_bc_ok(int level) const {
+  const BoundaryConditionSet b = get_bc_at_level(level);
+  const BoundaryCondition *lo[3] = {&b.xneg, &b.yneg, &b.zneg}, *hi[3] = {&b.xpos, &b.ypos, &b.zpos};
+  const int n[3] = {nx(level), ny(level), nz(level)};
+  for (int a = 0; a < 3; a++) {
+    const bool pl = lo[a]->type == BC_PERIODIC, ph = hi[a]->type == BC_PERIODIC;
+    if (pl != ph) return false;
+    if (pl) {
+      if (n[a] % 2) return false;
+      continue;
+    }
+    for (const BoundaryCondition *c : {lo[a], hi[a]})
+      if (c->type != BC_NEUMANN && c->type != BC_DIRICHELET) return false;
+  }
+  return true;
+}
+
 // relax_on_host (sol_mgpressure3ddev.cu:134-188) on the GPU: one LDS
 // workgroup, host arithmetic, no-corner fill after each colour, level-1 fill
 // at the end
@@ -503,10 +536,15 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
     }
     for (; it < iterations; it++) {
       int c0 = first_colour(it);
-      relax_colour(level, c0);
-      relax_colour(level, !c0);
-      apply_boundary_conditions(level);
+      relax_colour_bc(level, c0);
+      relax_colour_bc(level, !c0);
     }
+  } else if (relax_bc_ok(level)) {
+    // each colour launch writes the ghosts of the cells it updates, which is
+    // what the reference's fill between colours (periodic faces) or after
+    // each iteration computes; the ghosts were filled before the call
+    for (int it = 0; it < iterations; it++)
+      for (int rb = 0; rb < 2; rb++) relax_colour_bc(level, (first_colour(it) == 1) ? !rb : rb);
   } else {
     materialize(level);
     for (int it = 0; it < iterations; it++) {

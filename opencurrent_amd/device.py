@@ -124,6 +124,11 @@ def mg_relax(u, b, color, params, variant):
     _lib.check(_lib.hip().ocu_mg_relax(u.ptr(), b.ptr(), color, C.byref(params), variant, None), "mg_relax")
 
 
+def mg_relax_bc(u, b, color, params, bc, hx, hy, hz):
+    _lib.check(_lib.hip().ocu_mg_relax_bc(u.ptr(), b.ptr(), color, C.byref(params), C.byref(bc), hx, hy, hz, None),
+               "mg_relax_bc")
+
+
 def mg_relax_fused(u_in, u_out, b, params, bc, first_color=0):
     return _lib.check(_lib.hip().ocu_mg_relax_fused(u_in.ptr(), u_out.ptr(), b.ptr(), C.byref(params), C.byref(bc),
                                                     first_color, None), "mg_relax_fused")

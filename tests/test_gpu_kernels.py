@@ -594,3 +594,42 @@ def test_ir_residual(gpu, size):
     assert mx == np.abs(r).max()
     assert abs(s2 - (r * r).sum()) <= 1e-12 * (r * r).sum()
     assert abs(sm - r.sum()) <= 1e-12 * np.abs(r).sum()
+
+
+RELAX_BC_CASES = {
+    "mixed": BC_CASES["mixed"],  # periodic x, Dirichlet y, Neumann z (values nonzero)
+    "periodic": BC_CASES["periodic"],
+    "neumann_v": BC_CASES["neumann_v"],
+    "dirichlet": BC_CASES["dirichlet"],
+    "rb": bcset(T.BC_PERIODIC, T.BC_PERIODIC, T.BC_PERIODIC, T.BC_PERIODIC, (T.BC_NEUMANN, 0.0), (T.BC_NEUMANN, 0.0)),
+}
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", list(RELAX_BC_CASES))
+@pytest.mark.parametrize("size", [(16, 12, 20), (8, 6, 10), (2, 4, 2), (6, 2, 34)])
+def test_relax_bc_equals_colour_then_fill(gpu, dtype, case, size):
+    """ocu_mg_relax_bc == one colour of ocu_mg_relax followed by the level-1
+    fill, over the whole buffer, for both colours in turn"""
+    bc = RELAX_BC_CASES[case]
+    hx, hy, hz = .25, .125, .5
+    u = random_grid(*size, dtype, 171)
+    O.lib().orc_bc_fill(0, u.ptr(), C.byref(bc), hx, hy, hz)
+    b = random_grid(*size, dtype, 172)
+    p = O.level_params(bc, 1, hx, hy, hz)
+    want = u.copy()
+    gu, gb = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(b)
+    for color in (1, 0):
+        O.lib().orc_mg_relax(want.ptr(), b.ptr(), color, C.byref(p), T.RELAX_DEVICE)
+        O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), hx, hy, hz)
+        D().mg_relax_bc(gu, gb, color, p, bc, hx, hy, hz)
+        assert_bitexact(gu.to_host(), want, f"relax_bc {case} {size} colour {color}")
+
+
+def test_relax_bc_rejects_odd_periodic_and_none_faces(gpu):
+    from opencurrent_amd._lib import OcuError
+    p = O.level_params(BC_CASES["periodic"], 0, .1, .1, .1)
+    for size, bc in (((7, 8, 8), BC_CASES["periodic"]), ((8, 8, 8), BC_CASES["none_faces"])):
+        u, b = D().DeviceGrid(*size), D().DeviceGrid(*size)
+        with pytest.raises(OcuError):
+            D().mg_relax_bc(u, b, 0, p, bc, .1, .1, .1)
