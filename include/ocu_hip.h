@@ -244,6 +244,20 @@ int ocu_self_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *
 int ocu_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
                        const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
                        double hx, double hy, double hz, int scheme, void *stream);
+/* The two entry points above (same references, same argument meaning) with
+ * one more scheme, OCU_IT_SECOND_ORDER_UPWIND, which has no reference
+ * counterpart.  Bit a of periodic_axes (x=1, y=2, z=4) marks axis a
+ * periodic: that scheme's two-cell upstream stencil then wraps across it
+ * instead of falling back to first order at the boundary faces.  The
+ * periodic ghost layer must be filled on entry, as for the other schemes. */
+int ocu_self_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                       const ocu_grid3d *du, const ocu_grid3d *dv, const ocu_grid3d *dw, int nx,
+                       int ny, int nz, double hx, double hy, double hz, int scheme,
+                       int periodic_axes, void *stream);
+int ocu_passive_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                          const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
+                          double hx, double hy, double hz, int scheme, int periodic_axes,
+                          void *stream);
 /* ---- adjoints (no reference counterpart; for gradients of losses through
  * the advection-projection path) ----
  * Adjoint of ocu_passive_advect with the velocity's upwind selection fixed:

@@ -63,13 +63,18 @@ enum ocu_convergence {
   OCU_CONVERGENCE_NONE_CALC_L2LINF = 0x13
 };
 
-/* Interpolation schemes (ocuutil/interpolation.h:26-31).  The value 4 is a
- * scheme with no reference counterpart (second-order upwind, 2-ghost). */
+/* Interpolation schemes (ocuutil/interpolation.h:26-31).  The value 4 has
+ * no reference counterpart: second-order (linear) upwind, the face value
+ * extrapolated from the two upstream cells, x_up + (x_up - x_upup)/2; across
+ * a periodic axis the stencil wraps, and at a face whose second upstream
+ * cell lies outside the one-cell ghost layer of a non-periodic axis it is
+ * first-order upwind. */
 enum ocu_interp {
   OCU_IT_ERROR = 0,
   OCU_IT_FIRST_ORDER_UPWIND = 1,
   OCU_IT_SECOND_ORDER_CENTERED = 2,
-  OCU_IT_THIRD_ORDER_VAN_LEER_FLUX_LIMITED = 3
+  OCU_IT_THIRD_ORDER_VAN_LEER_FLUX_LIMITED = 3,
+  OCU_IT_SECOND_ORDER_UPWIND = 4
 };
 
 /* Direction flags (ocuutil/direction.h:24-38). */

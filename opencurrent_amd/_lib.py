@@ -100,6 +100,8 @@ HIP_SIGS = {
     "ocu_gradient_sub": (I, [GP, GP, GP, GP, D, D, D, D, P]),
     "ocu_self_advect": (I, [GP, GP, GP, GP, GP, GP, I, I, I, D, D, D, I, P]),
     "ocu_passive_advect": (I, [GP, GP, GP, GP, GP, I, I, I, D, D, D, I, P]),
+    "ocu_self_advect_ex": (I, [GP, GP, GP, GP, GP, GP, I, I, I, D, D, D, I, I, P]),
+    "ocu_passive_advect_ex": (I, [GP, GP, GP, GP, GP, I, I, I, D, D, D, I, I, P]),
     "ocu_laplacian_acc": (I, [GP, GP, I, I, I, D, D, D, D, P]),
     "ocu_thermal_force": (I, [GP, GP, I, I, I, I, D, P]),
 }

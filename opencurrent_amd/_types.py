@@ -25,7 +25,7 @@ CONVERGENCE_LINF = 0x0A
 CONVERGENCE_NONE = 0x10
 CONVERGENCE_NONE_CALC_L2LINF = 0x13
 
-IT_FIRST_ORDER_UPWIND, IT_SECOND_ORDER_CENTERED = 1, 2
+IT_FIRST_ORDER_UPWIND, IT_SECOND_ORDER_CENTERED, IT_SECOND_ORDER_UPWIND = 1, 2, 4
 
 DIR_XPOS, DIR_XNEG, DIR_YPOS, DIR_YNEG, DIR_ZPOS, DIR_ZNEG = 0x05, 0x06, 0x09, 0x0A, 0x11, 0x12
 

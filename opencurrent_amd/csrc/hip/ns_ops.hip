@@ -120,6 +120,101 @@ __global__ __launch_bounds__(256) void k_passive_advect(const T *__restrict__ u,
   d[id] = (-du * ih.x) - (dv * ih.y) - (dw * ih.z);
 }
 
+// ---- second-order (linear) upwind, OCU_IT_SECOND_ORDER_UPWIND (no reference
+// counterpart) ----
+//
+// The face value between xL and xR is extrapolated from the two upstream
+// samples, xL + .5*(xL - xLL) for vel >= 0 and xR + .5*(xR - xRR) for vel < 0
+// (vel == 0 takes the low side, as the reference's upwind does).  Only the
+// second upstream sample can leave a ghost layer: along axis a the layer of
+// a cell-centred or tangential array spans [-1, n_a], that of the face array
+// normal to a [-1, n_a + 1].  Beyond it the sample wraps (periodic axis) or
+// is missing, and the face falls back to first order.
+template <typename T>
+__device__ __forceinline__ T up2(T vel, T xLL, bool hLL, T xL, T xR, T xRR, bool hRR) {
+  if (vel < 0) return hRR ? xR + ((T).5) * (xR - xRR) : xR;
+  return hLL ? xL + ((T).5) * (xL - xLL) : xL;
+}
+
+// Offsets (in elements) of the samples two cells below and above cell c.
+struct Far2 {
+  int64_t lo, hi;
+  bool hlo, hhi;
+};
+
+__device__ __forceinline__ Far2 far2(int c, int n, int top, int64_t s, bool per) {
+  Far2 f;
+  const int wlo = (((c - 2) % n) + n) % n, whi = (c + 2) % n;
+  f.hlo = c >= 1 || per;
+  f.hhi = c + 2 <= top || per;
+  f.lo = (c >= 1 ? (int64_t)-2 : (int64_t)(wlo - c)) * s;
+  f.hi = (c + 2 <= top ? (int64_t)2 : (int64_t)(whi - c)) * s;
+  return f;
+}
+
+// (vh * face_hi) - (vl * face_lo) for the array X along the axis of stride s
+template <typename T>
+__device__ __forceinline__ T dflux2(const T *__restrict__ X, int64_t id, int64_t s, const Far2 &f, T vh, T vl) {
+  const T x0 = X[id], xm = X[id - s], xp = X[id + s];
+  const T xpp = f.hhi ? X[id + f.hi] : (T)0;
+  const T xmm = f.hlo ? X[id + f.lo] : (T)0;
+  return (vh * up2(vh, xm, true, x0, xp, xpp, f.hhi)) - (vl * up2(vl, xmm, f.hlo, xm, x0, xp, true));
+}
+
+// self-advection with second-order upwind face values: the face velocities
+// and flux terms of k_self_advect
+template <typename T>
+__global__ __launch_bounds__(256) void k_self_advect_up2(const T *__restrict__ U, const T *__restrict__ V,
+                                                         const T *__restrict__ W, T *__restrict__ DU,
+                                                         T *__restrict__ DV, T *__restrict__ DW, Inv<T> ih,
+                                                         int64_t xs, int64_t ys, int nx, int ny, int nz, int pmask) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  const int64_t pi = id + xs, pj = id + ys, pk = id + 1, mi = id - xs, mj = id - ys, mk = id - 1;
+  const bool px = pmask & 1, py = pmask & 2, pz = pmask & 4;
+  // tangential (t) and normal (n) ghost extents per axis
+  const Far2 fxt = far2(i, nx, nx, xs, px), fxn = far2(i, nx, nx + 1, xs, px);
+  const Far2 fyt = far2(j, ny, ny, ys, py), fyn = far2(j, ny, ny + 1, ys, py);
+  const Far2 fzt = far2(k, nz, nz, 1, pz), fzn = far2(k, nz, nz + 1, 1, pz);
+  const T half = (T).5;
+  const T u0 = U[id], v0 = V[id], w0 = W[id];
+
+  const T duu = dflux2(U, id, xs, fxn, half * (u0 + U[pi]), half * (u0 + U[mi]));
+  const T dvu = dflux2(U, id, ys, fyt, half * (V[pj] + V[pj - xs]), half * (v0 + V[mi]));
+  const T dwu = dflux2(U, id, (int64_t)1, fzt, half * (W[pk] + W[pk - xs]), half * (w0 + W[mi]));
+  DU[id] = -duu * ih.x - dvu * ih.y - dwu * ih.z;
+
+  const T duv = dflux2(V, id, xs, fxt, half * (U[pi] + U[pi - ys]), half * (u0 + U[mj]));
+  const T dvv = dflux2(V, id, ys, fyn, half * (v0 + V[pj]), half * (v0 + V[mj]));
+  const T dwv = dflux2(V, id, (int64_t)1, fzt, half * (W[pk] + W[pk - ys]), half * (w0 + W[mj]));
+  DV[id] = -duv * ih.x - dvv * ih.y - dwv * ih.z;
+
+  const T duw = dflux2(W, id, xs, fxt, half * (U[pi] + U[pi - 1]), half * (u0 + U[mk]));
+  const T dvw = dflux2(W, id, ys, fyt, half * (V[pj] + V[pj - 1]), half * (v0 + V[mk]));
+  const T dww = dflux2(W, id, (int64_t)1, fzn, half * (w0 + W[pk]), half * (w0 + W[mk]));
+  DW[id] = -duw * ih.x - dvw * ih.y - dww * ih.z;
+}
+
+// passive advection with second-order upwind face values
+template <typename T>
+__global__ __launch_bounds__(256) void k_passive_advect_up2(const T *__restrict__ u, const T *__restrict__ v,
+                                                            const T *__restrict__ w, const T *__restrict__ phi,
+                                                            T *__restrict__ d, Inv<T> ih, int64_t xs, int64_t ys,
+                                                            int nx, int ny, int nz, int pmask) {
+  int k = blockIdx.x * TK + threadIdx.x;
+  int j = blockIdx.y * TJ + threadIdx.y;
+  int i = blockIdx.z;
+  if (j >= ny || k >= nz) return;
+  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
+  const T du = dflux2(phi, id, xs, far2(i, nx, nx, xs, pmask & 1), u[id + xs], u[id]);
+  const T dv = dflux2(phi, id, ys, far2(j, ny, ny, ys, pmask & 2), v[id + ys], v[id]);
+  const T dw = dflux2(phi, id, (int64_t)1, far2(k, nz, nz, 1, pmask & 4), w[id + 1], w[id]);
+  d[id] = (-du * ih.x) - (dv * ih.y) - (dw * ih.z);
+}
+
 // ---- adjoints (no reference counterpart) ----
 //
 // passive advection is dphi_c = -sum_axes (F_{c+1} - F_c)/h with face fluxes
@@ -392,16 +487,37 @@ int ocu_gradient_sub(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d 
 int ocu_self_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *du,
                     const ocu_grid3d *dv, const ocu_grid3d *dw, int nx, int ny, int nz, double hx, double hy,
                     double hz, int scheme, void *stream) {
+  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED,
+              "Sol_SelfAdvection3DDevice::solve - invalid interpolation type %d", scheme);
+  return ocu_self_advect_ex(u, v, w, du, dv, dw, nx, ny, nz, hx, hy, hz, scheme, 0, stream);
+}
+
+int ocu_self_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *du,
+                       const ocu_grid3d *dv, const ocu_grid3d *dw, int nx, int ny, int nz, double hx, double hy,
+                       double hz, int scheme, int periodic_axes, void *stream) {
   const char *what = "Sol_SelfAdvection3DDevice::solve";
   if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
   if (int e = check_same(u, du, what)) return e;
   if (int e = check_same(u, dv, what)) return e;
   if (int e = check_same(u, dw, what)) return e;
-  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED,
+  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED ||
+                  scheme == OCU_IT_SECOND_ORDER_UPWIND,
               "%s - invalid interpolation type %d", what, scheme);
+  OCU_REQUIRE(periodic_axes >= 0 && periodic_axes <= 7, "%s - invalid periodic axes mask %d", what, periodic_axes);
   if ((int64_t)nx * ny * nz == 0) return OCU_OK;
   int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
   dim3 g = stencil_grid(nx, ny, nz);
+  if (scheme == OCU_IT_SECOND_ORDER_UPWIND) {
+    if (u->dtype == OCU_F64)
+      k_self_advect_up2<double><<<g, stencil_block(), 0, S(stream)>>>(
+          at0<double>(u), at0<double>(v), at0<double>(w), at0<double>(du), at0<double>(dv), at0<double>(dw),
+          inv<double>(hx, hy, hz), xs, ys, nx, ny, nz, periodic_axes);
+    else
+      k_self_advect_up2<float><<<g, stencil_block(), 0, S(stream)>>>(
+          at0<float>(u), at0<float>(v), at0<float>(w), at0<float>(du), at0<float>(dv), at0<float>(dw),
+          inv<float>(hx, hy, hz), xs, ys, nx, ny, nz, periodic_axes);
+    OCU_HIP_LAUNCHED("ocu_self_advect");
+  }
   if (u->dtype == OCU_F64)
     k_self_advect<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(u), at0<double>(v), at0<double>(w),
                                                                 at0<double>(du), at0<double>(dv), at0<double>(dw),
@@ -416,15 +532,36 @@ int ocu_self_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *
 int ocu_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *phi,
                        const ocu_grid3d *dphi, int nx, int ny, int nz, double hx, double hy, double hz, int scheme,
                        void *stream) {
+  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED,
+              "Sol_PassiveAdvection3DDevice::solve - invalid interpolation type %d", scheme);
+  return ocu_passive_advect_ex(u, v, w, phi, dphi, nx, ny, nz, hx, hy, hz, scheme, 0, stream);
+}
+
+int ocu_passive_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *phi,
+                          const ocu_grid3d *dphi, int nx, int ny, int nz, double hx, double hy, double hz,
+                          int scheme, int periodic_axes, void *stream) {
   const char *what = "Sol_PassiveAdvection3DDevice::solve";
   if (int e = check_mac(u, v, w, nx, ny, nz, what)) return e;
   if (int e = check_same(u, phi, what)) return e;
   if (int e = check_same(u, dphi, what)) return e;
-  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED,
+  OCU_REQUIRE(scheme == OCU_IT_FIRST_ORDER_UPWIND || scheme == OCU_IT_SECOND_ORDER_CENTERED ||
+                  scheme == OCU_IT_SECOND_ORDER_UPWIND,
               "%s - invalid interpolation type %d", what, scheme);
+  OCU_REQUIRE(periodic_axes >= 0 && periodic_axes <= 7, "%s - invalid periodic axes mask %d", what, periodic_axes);
   if ((int64_t)nx * ny * nz == 0) return OCU_OK;
   int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
   dim3 g = stencil_grid(nx, ny, nz);
+  if (scheme == OCU_IT_SECOND_ORDER_UPWIND) {
+    if (u->dtype == OCU_F64)
+      k_passive_advect_up2<double><<<g, stencil_block(), 0, S(stream)>>>(
+          at0<double>(u), at0<double>(v), at0<double>(w), at0<double>(phi), at0<double>(dphi),
+          inv<double>(hx, hy, hz), xs, ys, nx, ny, nz, periodic_axes);
+    else
+      k_passive_advect_up2<float><<<g, stencil_block(), 0, S(stream)>>>(
+          at0<float>(u), at0<float>(v), at0<float>(w), at0<float>(phi), at0<float>(dphi),
+          inv<float>(hx, hy, hz), xs, ys, nx, ny, nz, periodic_axes);
+    OCU_HIP_LAUNCHED("ocu_passive_advect");
+  }
   if (u->dtype == OCU_F64)
     k_passive_advect<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(u), at0<double>(v), at0<double>(w),
                                                                    at0<double>(phi), at0<double>(dphi),

@@ -115,6 +115,8 @@ bool Eqn_IncompressibleNS3D<T>::set_parameters(const Eqn_IncompressibleNS3DParam
   _projection_solver.bc = params.flow_bc;
   _advection_solver.interp_type = params.advection_scheme;
   _thermal_solver.interp_type = params.advection_scheme;
+  _advection_solver.periodic_axes = params.flow_bc.periodic_axes();
+  _thermal_solver.periodic_axes = params.temp_bc.periodic_axes();
 
   if (!_thermal_solver.initialize_storage(this->_nx, this->_ny, this->_nz, this->_hx, this->_hy, this->_hz, &_u, &_v,
                                           &_w, &_temp, &_deriv_tempdt)) {
@@ -290,6 +292,7 @@ template <typename T>
 bool Eqn_ScalarAdvection3D<T>::set_parameters(const Eqn_ScalarAdvection3DParams<T> &params) {
   bc = params.bc;
   _advection.interp_type = params.advection_scheme;
+  _advection.periodic_axes = params.bc.periodic_axes();
   if (!u.init_congruent(params.u) || !v.init_congruent(params.v) || !w.init_congruent(params.w)) {
     printf("[ERROR] Eqn_ScalarAdvection3D::set_parameters - failed to initialize u,v,w grids\n");
     return false;

@@ -21,6 +21,9 @@ class Sol_PassiveAdvection3DDevice : public Solver {
   Grid3DDevice<T> *phi;
   Grid3DDevice<T> *deriv_phidt;
   InterpolationType interp_type;
+  // no reference counterpart: axes (x=1, y=2, z=4) across which the
+  // IT_SECOND_ORDER_UPWIND stencil wraps; the other schemes ignore it
+  int periodic_axes;
 
   bool solve();
   bool initialize_storage(int nx, int ny, int nz, double hx, double hy, double hz, Grid3DDevice<T> *u_val,

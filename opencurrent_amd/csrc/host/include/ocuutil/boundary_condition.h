@@ -68,6 +68,15 @@ struct BoundaryConditionSet {
            zpos.check_type(t1, t2, t3, t4) && zneg.check_type(t1, t2, t3, t4);
   }
 
+  // no reference counterpart: bit a (x=1, y=2, z=4) set when both faces of
+  // axis a are periodic
+  int periodic_axes() const {
+    auto pp = [](const BoundaryCondition &a, const BoundaryCondition &b) {
+      return a.type == BC_PERIODIC && b.type == BC_PERIODIC;
+    };
+    return (pp(xneg, xpos) ? 1 : 0) | (pp(yneg, ypos) ? 2 : 0) | (pp(zneg, zpos) ? 4 : 0);
+  }
+
   void make_homogeneous() {
     BoundaryCondition *all[6] = {&xpos, &xneg, &ypos, &yneg, &zpos, &zneg};
     for (auto b : all) { b->value = 0; b->aux_value = 0; }

@@ -8,7 +8,9 @@ enum InterpolationType {
   IT_ERROR = 0,
   IT_FIRST_ORDER_UPWIND,
   IT_SECOND_ORDER_CENTERED,
-  IT_THIRD_ORDER_VAN_LEER_FLUX_LIMITED
+  IT_THIRD_ORDER_VAN_LEER_FLUX_LIMITED,
+  // no reference counterpart: second-order (linear) upwind, see ocu_types.h
+  IT_SECOND_ORDER_UPWIND
 };
 
 }  // namespace ocu

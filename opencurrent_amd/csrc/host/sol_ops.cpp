@@ -117,18 +117,20 @@ Sol_SelfAdvection3DDevice<T>::Sol_SelfAdvection3DDevice() {
   u = v = w = 0;
   deriv_udt = deriv_vdt = deriv_wdt = 0;
   interp_type = IT_FIRST_ORDER_UPWIND;
+  periodic_axes = 0;
 }
 
 template <typename T>
 bool Sol_SelfAdvection3DDevice<T>::solve() {
-  if (interp_type != IT_FIRST_ORDER_UPWIND && interp_type != IT_SECOND_ORDER_CENTERED) {
+  if (interp_type != IT_FIRST_ORDER_UPWIND && interp_type != IT_SECOND_ORDER_CENTERED &&
+      interp_type != IT_SECOND_ORDER_UPWIND) {
     printf("[ERROR] Sol_SelfAdvection3DDevice::solve - invalid interpolation type %d\n", interp_type);
     return false;
   }
   ocu_grid3d du = u->desc(), dv = v->desc(), dw = w->desc();
   ocu_grid3d a = deriv_udt->desc(), b = deriv_vdt->desc(), c = deriv_wdt->desc();
-  return ok_status(ocu_self_advect(&du, &dv, &dw, &a, &b, &c, _nx, _ny, _nz, _hx, _hy, _hz, (int)interp_type,
-                                   ThreadManager::get_compute_stream()),
+  return ok_status(ocu_self_advect_ex(&du, &dv, &dw, &a, &b, &c, _nx, _ny, _nz, _hx, _hy, _hz, (int)interp_type,
+                                      periodic_axes, ThreadManager::get_compute_stream()),
                    "Sol_SelfAdvection3DDevice::solve");
 }
 
@@ -184,17 +186,19 @@ Sol_PassiveAdvection3DDevice<T>::Sol_PassiveAdvection3DDevice() {
   phi = 0;
   deriv_phidt = 0;
   interp_type = IT_FIRST_ORDER_UPWIND;
+  periodic_axes = 0;
 }
 
 template <typename T>
 bool Sol_PassiveAdvection3DDevice<T>::solve() {
-  if (interp_type != IT_FIRST_ORDER_UPWIND && interp_type != IT_SECOND_ORDER_CENTERED) {
+  if (interp_type != IT_FIRST_ORDER_UPWIND && interp_type != IT_SECOND_ORDER_CENTERED &&
+      interp_type != IT_SECOND_ORDER_UPWIND) {
     printf("[ERROR] Sol_PassiveAdvection3DDevice::solve() - invalid interpolation type %d\n", interp_type);
     return false;
   }
   ocu_grid3d du = u->desc(), dv = v->desc(), dw = w->desc(), dp = phi->desc(), dd = deriv_phidt->desc();
-  return ok_status(ocu_passive_advect(&du, &dv, &dw, &dp, &dd, _nx, _ny, _nz, _hx, _hy, _hz, (int)interp_type,
-                                      ThreadManager::get_compute_stream()),
+  return ok_status(ocu_passive_advect_ex(&du, &dv, &dw, &dp, &dd, _nx, _ny, _nz, _hx, _hy, _hz, (int)interp_type,
+                                         periodic_axes, ThreadManager::get_compute_stream()),
                    "Sol_PassiveAdvection3DDevice::solve");
 }
 

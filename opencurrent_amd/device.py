@@ -214,14 +214,25 @@ def gradient_sub(u, v, w, p, coef, hx, hy, hz):
                "gradient_sub")
 
 
-def self_advect(u, v, w, du, dv, dw, nx, ny, nz, hx, hy, hz, scheme):
-    _lib.check(_lib.hip().ocu_self_advect(u.ptr(), v.ptr(), w.ptr(), du.ptr(), dv.ptr(), dw.ptr(), nx, ny, nz,
-                                          hx, hy, hz, scheme, None), "self_advect")
+def self_advect(u, v, w, du, dv, dw, nx, ny, nz, hx, hy, hz, scheme, periodic_axes=None):
+    """periodic_axes None: the reference's entry point (schemes 1 and 2);
+    a bitmask (x=1, y=2, z=4): the extended one, which also takes
+    IT_SECOND_ORDER_UPWIND."""
+    if periodic_axes is None:
+        _lib.check(_lib.hip().ocu_self_advect(u.ptr(), v.ptr(), w.ptr(), du.ptr(), dv.ptr(), dw.ptr(), nx, ny, nz,
+                                              hx, hy, hz, scheme, None), "self_advect")
+    else:
+        _lib.check(_lib.hip().ocu_self_advect_ex(u.ptr(), v.ptr(), w.ptr(), du.ptr(), dv.ptr(), dw.ptr(), nx, ny,
+                                                 nz, hx, hy, hz, scheme, periodic_axes, None), "self_advect")
 
 
-def passive_advect(u, v, w, phi, dphi, nx, ny, nz, hx, hy, hz, scheme):
-    _lib.check(_lib.hip().ocu_passive_advect(u.ptr(), v.ptr(), w.ptr(), phi.ptr(), dphi.ptr(), nx, ny, nz,
-                                             hx, hy, hz, scheme, None), "passive_advect")
+def passive_advect(u, v, w, phi, dphi, nx, ny, nz, hx, hy, hz, scheme, periodic_axes=None):
+    if periodic_axes is None:
+        _lib.check(_lib.hip().ocu_passive_advect(u.ptr(), v.ptr(), w.ptr(), phi.ptr(), dphi.ptr(), nx, ny, nz,
+                                                 hx, hy, hz, scheme, None), "passive_advect")
+    else:
+        _lib.check(_lib.hip().ocu_passive_advect_ex(u.ptr(), v.ptr(), w.ptr(), phi.ptr(), dphi.ptr(), nx, ny, nz,
+                                                    hx, hy, hz, scheme, periodic_axes, None), "passive_advect")
 
 
 def passive_advect_adjoint(u, v, w, phi, lam, lam_out, au, av, aw, scale, nx, ny, nz, hx, hy, hz, scheme):

@@ -178,6 +178,24 @@ int orc_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3
   return 1;
 }
 
+int orc_self_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *du,
+                       const ocu_grid3d *dv, const ocu_grid3d *dw, int nx, int ny, int nz,
+                       double hx, double hy, double hz, int scheme, int periodic_axes) {
+  if (u->dtype == OCU_F64)
+    self_advect<double>(*u, *v, *w, *du, *dv, *dw, nx, ny, nz, hx, hy, hz, scheme, periodic_axes);
+  else self_advect<float>(*u, *v, *w, *du, *dv, *dw, nx, ny, nz, hx, hy, hz, scheme, periodic_axes);
+  return 1;
+}
+
+int orc_passive_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                          const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
+                          double hx, double hy, double hz, int scheme, int periodic_axes) {
+  if (u->dtype == OCU_F64)
+    passive_advect<double>(*u, *v, *w, *phi, *dphi, nx, ny, nz, hx, hy, hz, scheme, periodic_axes);
+  else passive_advect<float>(*u, *v, *w, *phi, *dphi, nx, ny, nz, hx, hy, hz, scheme, periodic_axes);
+  return 1;
+}
+
 int orc_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
                       double hx, double hy, double hz, double coefficient) {
   if (phi->dtype == OCU_F64) laplacian_acc<double>(*phi, *dphi, nx, ny, nz, hx, hy, hz, coefficient);

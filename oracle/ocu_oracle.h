@@ -61,6 +61,14 @@ int orc_self_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *
 int orc_passive_advect(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
                        const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
                        double hx, double hy, double hz, int scheme);
+/* The two above with OCU_IT_SECOND_ORDER_UPWIND allowed; periodic_axes bit a
+ * (x=1, y=2, z=4) lets that scheme's stencil wrap along axis a. */
+int orc_self_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *du,
+                       const ocu_grid3d *dv, const ocu_grid3d *dw, int nx, int ny, int nz,
+                       double hx, double hy, double hz, int scheme, int periodic_axes);
+int orc_passive_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                          const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
+                          double hx, double hy, double hz, int scheme, int periodic_axes);
 int orc_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int ny, int nz,
                       double hx, double hy, double hz, double coefficient);
 int orc_thermal_force(const ocu_grid3d *d, const ocu_grid3d *t, int nx, int ny, int nz,

@@ -20,6 +20,38 @@ inline T interp(int scheme, T u, T x0, T x1) {
   return u < 0 ? x1 : x0;
 }
 
+/* Second-order (linear) upwind, OCU_IT_SECOND_ORDER_UPWIND (no reference
+ * counterpart): the face value between xL and xR extrapolated from the two
+ * upstream cells, xL + .5*(xL - xLL) for vel >= 0, xR + .5*(xR - xRR) for
+ * vel < 0 (the reference's upwind picks xL at vel == 0); a missing second
+ * upstream cell (h* false) gives the first-order value. */
+template <typename T>
+inline T up2(T vel, T xLL, bool hLL, T xL, T xR, T xRR, bool hRR) {
+  if (vel < 0) return hRR ? xR + ((T).5) * (xR - xRR) : xR;
+  return hLL ? xL + ((T).5) * (xL - xLL) : xL;
+}
+
+/* The cell c+d (d = +-2) along an axis of a grid whose ghost layer spans
+ * [-1, hi]: present in the layer, or wrapped by the period P when the axis
+ * is periodic, or missing.  Returns the offset to use (in cells). */
+inline bool far_cell(int c, int d, int hi, int P, bool per, int &off) {
+  const int cc = c + d;
+  if (cc >= -1 && cc <= hi) {
+    off = d;
+    return true;
+  }
+  if (!per) return false;
+  off = ((cc % P) + P) % P - c;
+  return true;
+}
+
+/* Bit a set when both faces of axis a are periodic: the axes along which
+ * the second-order upwind stencil wraps. */
+inline int periodic_axes(const ocu_bc_set &bc) {
+  auto pp = [](const ocu_bc &a, const ocu_bc &b) { return a.type == OCU_BC_PERIODIC && b.type == OCU_BC_PERIODIC; };
+  return (pp(bc.xneg, bc.xpos) ? 1 : 0) | (pp(bc.yneg, bc.ypos) ? 2 : 0) | (pp(bc.zneg, bc.zpos) ? 4 : 0);
+}
+
 /* Sol_Divergence3DDevice_calculate_divergence, sol_divergence3ddev.cu:23-45;
  * 1/h rounded to T by the launcher (:80-84). */
 template <typename T>
@@ -59,9 +91,17 @@ void gradient_sub(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &
  * naive kernel :29-114 has the same arithmetic).  Writes du,dv,dw on
  * i<nx, j<ny, k<nz only. */
 template <typename T>
+void self_advect_up2(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg, const ocu_grid3d &dug,
+                     const ocu_grid3d &dvg, const ocu_grid3d &dwg, int nx, int ny, int nz, double hx, double hy,
+                     double hz, int pmask);
+template <typename T>
 void self_advect(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
                  const ocu_grid3d &dug, const ocu_grid3d &dvg, const ocu_grid3d &dwg,
-                 int nx, int ny, int nz, double hx, double hy, double hz, int scheme) {
+                 int nx, int ny, int nz, double hx, double hy, double hz, int scheme, int pmask = 0) {
+  if (scheme == OCU_IT_SECOND_ORDER_UPWIND) {
+    self_advect_up2<T>(ug, vg, wg, dug, dvg, dwg, nx, ny, nz, hx, hy, hz, pmask);
+    return;
+  }
   View<T> U(ug), V(vg), W(wg), DU(dug), DV(dvg), DW(dwg);
   const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
   const T half = (T).5;
@@ -118,13 +158,94 @@ void self_advect(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &w
       }
 }
 
+/* Self-advection with OCU_IT_SECOND_ORDER_UPWIND (no reference counterpart):
+ * the flux-form terms and face velocities of self_advect above, each face
+ * value taken by up2 from the two upstream samples of the advected component
+ * along the flux axis.  The component's ghost layer along axis a spans
+ * [-1, n_a] or, for the component normal to a, [-1, n_a + 1]; a second
+ * upstream sample beyond it wraps when bit a of pmask is set and is
+ * otherwise missing (first order at that face). */
+template <typename T>
+void self_advect_up2(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg, const ocu_grid3d &dug,
+                     const ocu_grid3d &dvg, const ocu_grid3d &dwg, int nx, int ny, int nz, double hx, double hy,
+                     double hz, int pmask) {
+  View<T> U(ug), V(vg), W(wg), DU(dug), DV(dvg), DW(dwg);
+  const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+  const T half = (T).5;
+  const int n[3] = {nx, ny, nz};
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const int c[3] = {i, j, k};
+        auto u = [&](int a, int b, int cc) { return U.at(i + a, j + b, k + cc); };
+        auto v = [&](int a, int b, int cc) { return V.at(i + a, j + b, k + cc); };
+        auto w = [&](int a, int b, int cc) { return W.at(i + a, j + b, k + cc); };
+        // (vh * face_hi) - (vl * face_lo) for component X (normal axis xa)
+        // along axis a
+        auto dflux = [&](const View<T> &X, int xa, int a, T vh, T vl) {
+          auto at = [&](int dd) { return X.at(i + (a == 0) * dd, j + (a == 1) * dd, k + (a == 2) * dd); };
+          const bool per = (pmask >> a) & 1;
+          const int hi = n[a] + (xa == a);
+          int ohi = 0, olo = 0;
+          const bool hhi = far_cell(c[a], 2, hi, n[a], per, ohi);
+          const bool hlo = far_cell(c[a], -2, hi, n[a], per, olo);
+          const T x0 = at(0), xm = at(-1), xp = at(1);
+          const T xpp = hhi ? at(ohi) : (T)0, xmm = hlo ? at(olo) : (T)0;
+          return (vh * up2<T>(vh, xm, true, x0, xp, xpp, hhi)) - (vl * up2<T>(vl, xmm, hlo, xm, x0, xp, true));
+        };
+        const T u0 = u(0, 0, 0), v0 = v(0, 0, 0), w0 = w(0, 0, 0);
+        const T duu = dflux(U, 0, 0, half * (u0 + u(1, 0, 0)), half * (u0 + u(-1, 0, 0)));
+        const T dvu = dflux(U, 0, 1, half * (v(0, 1, 0) + v(-1, 1, 0)), half * (v0 + v(-1, 0, 0)));
+        const T dwu = dflux(U, 0, 2, half * (w(0, 0, 1) + w(-1, 0, 1)), half * (w0 + w(-1, 0, 0)));
+        DU.at(i, j, k) = -duu * ix - dvu * iy - dwu * iz;
+        const T duv = dflux(V, 1, 0, half * (u(1, 0, 0) + u(1, -1, 0)), half * (u0 + u(0, -1, 0)));
+        const T dvv = dflux(V, 1, 1, half * (v0 + v(0, 1, 0)), half * (v0 + v(0, -1, 0)));
+        const T dwv = dflux(V, 1, 2, half * (w(0, 0, 1) + w(0, -1, 1)), half * (w0 + w(0, -1, 0)));
+        DV.at(i, j, k) = -duv * ix - dvv * iy - dwv * iz;
+        const T duw = dflux(W, 2, 0, half * (u(1, 0, 0) + u(1, 0, -1)), half * (u0 + u(0, 0, -1)));
+        const T dvw = dflux(W, 2, 1, half * (v(0, 1, 0) + v(0, 1, -1)), half * (v0 + v(0, 0, -1)));
+        const T dww = dflux(W, 2, 2, half * (w0 + w(0, 0, 1)), half * (w0 + w(0, 0, -1)));
+        DW.at(i, j, k) = -duw * ix - dvw * iy - dww * iz;
+      }
+}
+
 /* Sol_PassiveAdvection3D_apply_interp, sol_passiveadvection3ddev.cu:31-78 */
 template <typename T>
 void passive_advect(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
                     const ocu_grid3d &phig, const ocu_grid3d &dg, int nx, int ny, int nz,
-                    double hx, double hy, double hz, int scheme) {
+                    double hx, double hy, double hz, int scheme, int pmask = 0) {
   View<T> u(ug), v(vg), w(wg), phi(phig), d(dg);
   const T ix = (T)(1 / hx), iy = (T)(1 / hy), iz = (T)(1 / hz);
+  if (scheme == OCU_IT_SECOND_ORDER_UPWIND) {
+    // pmask bit a: axis a periodic.  Each face flux takes its two upstream
+    // cells; only the second one can leave the ghost layer.
+    const int n[3] = {nx, ny, nz};
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          const int c[3] = {i, j, k};
+          auto at = [&](int a, int dd) {
+            return phi.at(i + (a == 0) * dd, j + (a == 1) * dd, k + (a == 2) * dd);
+          };
+          T fl[3];
+          for (int a = 0; a < 3; a++) {
+            const T vh = a == 0 ? u.at(i + 1, j, k) : a == 1 ? v.at(i, j + 1, k) : w.at(i, j, k + 1);
+            const T vl = a == 0 ? u.at(i, j, k) : a == 1 ? v.at(i, j, k) : w.at(i, j, k);
+            const bool per = (pmask >> a) & 1;
+            int ohi = 0, olo = 0;
+            const bool hhi = far_cell(c[a], 2, n[a], n[a], per, ohi);   // high face: xRR = c+2
+            const bool hlo = far_cell(c[a], -2, n[a], n[a], per, olo);  // low face: xLL = c-2
+            const T p = at(a, 0);
+            const T fh = vh * up2<T>(vh, at(a, -1), true, p, at(a, 1), hhi ? at(a, ohi) : (T)0, hhi);
+            const T flo = vl * up2<T>(vl, hlo ? at(a, olo) : (T)0, hlo, at(a, -1), p, at(a, 1), true);
+            fl[a] = fh - flo;
+          }
+          d.at(i, j, k) = (-fl[0] * ix) - (fl[1] * iy) - (fl[2] * iz);
+        }
+    return;
+  }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < nx; i++)
     for (int j = 0; j < ny; j++)
@@ -494,14 +615,16 @@ struct NS {
     bool ok = true;
     num_steps++;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    self_advect<T>(u.desc, v.desc, w.desc, du.desc, dv.desc, dw.desc, nx, ny, nz, p.hx, p.hy, p.hz, p.advection_scheme);
+    self_advect<T>(u.desc, v.desc, w.desc, du.desc, dv.desc, dw.desc, nx, ny, nz, p.hx, p.hy, p.hz, p.advection_scheme,
+                   periodic_axes(p.flow_bc));
     if (viscosity > 0) {
       laplacian_acc<T>(u.desc, du.desc, nx + 1, ny, nz, p.hx, p.hy, p.hz, viscosity);
       laplacian_acc<T>(v.desc, dv.desc, nx, ny + 1, nz, p.hx, p.hy, p.hz, viscosity);
       laplacian_acc<T>(w.desc, dw.desc, nx, ny, nz + 1, p.hx, p.hy, p.hz, viscosity);
     }
     add_thermal_force();
-    passive_advect<T>(u.desc, v.desc, w.desc, temp.desc, dtemp.desc, nx, ny, nz, p.hx, p.hy, p.hz, p.advection_scheme);
+    passive_advect<T>(u.desc, v.desc, w.desc, temp.desc, dtemp.desc, nx, ny, nz, p.hx, p.hy, p.hz, p.advection_scheme,
+                      periodic_axes(p.temp_bc));
     if (thermal_diffusion > 0)
       laplacian_acc<T>(temp.desc, dtemp.desc, nx, ny, nz, p.hx, p.hy, p.hz, thermal_diffusion);
     T ab_coeff = -dt * dt / (2 * lastdt);
@@ -550,7 +673,7 @@ template <typename T>
 bool scalar_advect_step(const ocu_grid3d &ug, const ocu_grid3d &vg, const ocu_grid3d &wg,
                         const ocu_grid3d &phig, const ocu_grid3d &dphig, const ocu_bc_set &bc,
                         int nx, int ny, int nz, double hx, double hy, double hz, int scheme, double dt) {
-  passive_advect<T>(ug, vg, wg, phig, dphig, nx, ny, nz, hx, hy, hz, scheme);
+  passive_advect<T>(ug, vg, wg, phig, dphig, nx, ny, nz, hx, hy, hz, scheme, periodic_axes(bc));
   bool ok = lincomb2<T>(phig, (T)1, phig, (T)dt, dphig);
   ok = bc_fill_nocorners_device<T>(phig, bc, hx, hy, hz) && ok;
   return ok;

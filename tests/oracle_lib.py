@@ -41,6 +41,8 @@ SIGS = {
     "orc_gradient_sub": (I, [GP, GP, GP, GP, D, D, D, D]),
     "orc_self_advect": (I, [GP, GP, GP, GP, GP, GP, I, I, I, D, D, D, I]),
     "orc_passive_advect": (I, [GP, GP, GP, GP, GP, I, I, I, D, D, D, I]),
+    "orc_self_advect_ex": (I, [GP, GP, GP, GP, GP, GP, I, I, I, D, D, D, I, I]),
+    "orc_passive_advect_ex": (I, [GP, GP, GP, GP, GP, I, I, I, D, D, D, I, I]),
     "orc_passive_advect_adjoint": (I, [GP, GP, GP, GP, GP, GP, GP, GP, GP, D, I, I, I, D, D, D, I]),
     "orc_bc_adjoint_nocorners": (I, [GP, BP]),
     "orc_mac_adjoint_normal": (I, [GP, GP, GP, BP]),

@@ -464,6 +464,48 @@ def test_advection(gpu, dtype, scheme, size):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("pmask", [0, 3, 7, 4])
+@pytest.mark.parametrize("size", NS_SIZES + [(2, 3, 1), (1, 5, 2)])
+def test_advection_second_order_upwind(gpu, dtype, pmask, size):
+    """second-order upwind self- and passive advection (random values in
+    every ghost and padding cell, so a wrong far-sample index shows) against
+    the oracle, bit for bit"""
+    nx, ny, nz = size
+    u, v, w, t = mac_grids(nx, ny, nz, dtype)
+    for q, g in enumerate((u, v, w, t)):
+        g.buf[:] = (2 * uniform(140 + q, g.buf.shape) - 1).astype(dtype)
+    hx, hy, hz = .5, .25, .125
+    up2 = T.IT_SECOND_ORDER_UPWIND
+    d = [HostGrid.congruent(g) for g in (u, v, w, t)]
+    O.lib().orc_self_advect_ex(u.ptr(), v.ptr(), w.ptr(), d[0].ptr(), d[1].ptr(), d[2].ptr(), nx, ny, nz, hx, hy,
+                               hz, up2, pmask)
+    O.lib().orc_passive_advect_ex(u.ptr(), v.ptr(), w.ptr(), t.ptr(), d[3].ptr(), nx, ny, nz, hx, hy, hz, up2, pmask)
+    du, dv, dw, dt = (D().DeviceGrid.from_host(x) for x in (u, v, w, t))
+    g = [D().DeviceGrid.from_host(HostGrid.congruent(x)) for x in (u, v, w, t)]
+    D().self_advect(du, dv, dw, g[0], g[1], g[2], nx, ny, nz, hx, hy, hz, up2, pmask)
+    D().passive_advect(du, dv, dw, dt, g[3], nx, ny, nz, hx, hy, hz, up2, pmask)
+    for q, n in enumerate(("du", "dv", "dw", "dphi")):
+        assert_bitexact(g[q].to_host(), d[q], f"advection {n}")
+
+
+def test_second_order_upwind_entry_checks(gpu):
+    """the reference's entry points keep their two schemes; the extended
+    ones take three and a 3-bit axes mask"""
+    from opencurrent_amd._lib import OcuError
+    u, v, w, t = (D().DeviceGrid.from_host(x) for x in mac_grids(6, 5, 4))
+    dt = D().DeviceGrid.congruent(t)
+    up2 = T.IT_SECOND_ORDER_UPWIND
+    with pytest.raises(OcuError):
+        D().passive_advect(u, v, w, t, dt, 6, 5, 4, 1, 1, 1, up2)
+    with pytest.raises(OcuError):
+        D().self_advect(u, v, w, dt, dt, dt, 6, 5, 4, 1, 1, 1, up2)
+    with pytest.raises(OcuError):
+        D().passive_advect(u, v, w, t, dt, 6, 5, 4, 1, 1, 1, up2, 8)
+    with pytest.raises(OcuError):
+        D().self_advect(u, v, w, dt, dt, dt, 6, 5, 4, 1, 1, 1, 3, 0)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("scheme", [T.IT_FIRST_ORDER_UPWIND, T.IT_SECOND_ORDER_CENTERED])
 @pytest.mark.parametrize("size", NS_SIZES)
 def test_advection_adjoint(gpu, dtype, scheme, size):

@@ -358,7 +358,7 @@ def rb_initial(n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("scheme", [T.IT_FIRST_ORDER_UPWIND, T.IT_SECOND_ORDER_CENTERED])
+@pytest.mark.parametrize("scheme", [T.IT_FIRST_ORDER_UPWIND, T.IT_SECOND_ORDER_CENTERED, T.IT_SECOND_ORDER_UPWIND])
 @pytest.mark.parametrize("ts", [T.TS_FORWARD_EULER, T.TS_ADAMS_BASHFORD2])
 def test_ns_steps_match_oracle(gpu, dtype, scheme, ts):
     """Rayleigh-Benard steps (rayleightiming.cpp set-up at 16^3): the GPU
