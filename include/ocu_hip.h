@@ -208,17 +208,32 @@ int ocu_mg_coarse_relax(const ocu_grid3d *u, const ocu_grid3d *b, int iterations
 /* ---------------- mixed-precision iterative refinement -------------------- */
 
 /* r = (b - bshift) - A x in FP64 (x, b FP64), r32 = (float)(r*scale) (FP32,
- * congruent); partial (sum r^2, max|r|, sum r) per block to ws
- * (OCU_NORM_PARTS triples) for ocu_norm3_finalize. */
+ * congruent); partial (sum r^2, max|r|, sum r, sum x) per block to ws
+ * (OCU_NORM_PARTS quadruples) for ocu_norm4_finalize. */
 int ocu_ir_residual(const ocu_grid3d *x, const ocu_grid3d *b, double bshift,
                     const ocu_grid3d *r32, double scale, double h, double fx, double fy,
                     double fz, double *ws, void *stream);
+/* out[0..2] / out[0..3]: the triples / quadruples of ws reduced (slot 1 by
+ * max, the others by sum) */
 int ocu_norm3_finalize(const double *ws, double *out, void *stream);
+int ocu_norm4_finalize(const double *ws, double *out, void *stream);
+/* shift[0] = (xsum + inv_scale * sum of e over the interior) / (nx*ny*nz),
+ * the interior mean of x + e*inv_scale when xsum is the interior sum of x
+ * (e FP32; ws: OCU_NORM_PARTS doubles; shift: device memory). */
+int ocu_ir_shift(const ocu_grid3d *e, double inv_scale, double xsum, double *ws, double *shift,
+                 void *stream);
 /* the same triples for an FP64 grid itself (sum g^2, max|g|, sum g over
  * the interior): the three statistics of the right-hand side in one pass */
 int ocu_grid_stats3(const ocu_grid3d *g, double *ws, void *stream);
 /* x += (double)e * inv_scale over the interior (x FP64, e FP32) */
 int ocu_ir_correct(const ocu_grid3d *x, const ocu_grid3d *e, double inv_scale, void *stream);
+/* x = (x + (double)e * inv_scale) - shift[0] (shift: device memory, or null
+ * for none) over the interior, and with faces != 0 also over the six
+ * face-ghost layers (not edges or corners).  Face mode is the ghost fill
+ * itself when every face of x and e is a copy (homogeneous Neumann or
+ * periodic) and both ghost layers are filled on entry. */
+int ocu_ir_correct_ex(const ocu_grid3d *x, const ocu_grid3d *e, double inv_scale,
+                      const double *shift, int faces, void *stream);
 /* out[0] = interior sum of g (double) via ws (OCU_NORM_PARTS doubles) */
 int ocu_grid_sum(const ocu_grid3d *g, double *ws, double *out, void *stream);
 

@@ -94,6 +94,9 @@ HIP_SIGS = {
     "ocu_ir_residual": (I, [GP, GP, D, GP, D, D, D, D, D, DP, P]),
     "ocu_norm3_finalize": (I, [DP, DP, P]),
     "ocu_ir_correct": (I, [GP, GP, D, P]),
+    "ocu_ir_correct_ex": (I, [GP, GP, D, DP, I, P]),
+    "ocu_ir_shift": (I, [GP, D, D, DP, DP, P]),
+    "ocu_norm4_finalize": (I, [DP, DP, P]),
     "ocu_grid_sum": (I, [GP, DP, DP, P]),
     "ocu_norm_parts": (I, []),
     "ocu_divergence": (I, [GP, GP, GP, GP, I, I, I, D, D, D, P]),

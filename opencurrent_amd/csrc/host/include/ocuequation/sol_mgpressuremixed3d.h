@@ -22,7 +22,7 @@ class Sol_MultigridPressureMixed3DDeviceD : public Sol_MultigridPressure3DBase {
   Grid3DDevice<double> _x;                     // FP64 solution
   Grid3DDevice<double> *_rhs;                  // caller's FP64 right-hand side
   Grid3DDevice<float> _r32;                    // scaled FP32 residual (level-0 rhs of _mg_f)
-  double *_ws;                                 // device: NORM_PARTS triples + results
+  double *_ws;                                 // device: NORM_PARTS quadruples, NORM_PARTS sums, the shift
   double *_host;                               // pinned: results
   double *_host_dev;                           // the same memory as the device sees it
   bool _singular;
@@ -35,7 +35,8 @@ class Sol_MultigridPressureMixed3DDeviceD : public Sol_MultigridPressure3DBase {
   virtual void clear_zero(int level);
   virtual bool do_fmg(double tolerance, int max_iter, double &result_l2, double &result_linf);
 
-  bool fp64_residual(double bshift, double scale, double &sumsq, double &maxabs, double &sum);
+  bool fp64_residual(double bshift, double scale, double &sumsq, double &maxabs, double &sum, double &xsum);
+  bool copy_faces() const;
   bool interior_mean(const Grid3DDevice<double> &g, double &mean);
   void vcycle_f32(int fine_level);
   void fmg_f32(int cycles0);

@@ -73,7 +73,7 @@ bool Sol_MultigridPressureMixed3DDeviceD::initialize_storage(int nx_val, int ny_
                 bc.ypos.check_type(BC_DIRICHELET) || bc.yneg.check_type(BC_DIRICHELET) ||
                 bc.zpos.check_type(BC_DIRICHELET) || bc.zneg.check_type(BC_DIRICHELET));
   void *w = 0, *h = 0;
-  if (ocu_malloc(&w, sizeof(double) * (3 * ocu_norm_parts() + 8)) != OCU_OK ||
+  if (ocu_malloc(&w, sizeof(double) * (5 * ocu_norm_parts() + 8)) != OCU_OK ||
       ocu_host_malloc(&h, sizeof(double) * 8) != OCU_OK) {
     printf("[ERROR] Sol_MultigridPressureMixed3DDeviceD::initialize_storage - workspace allocation failed\n");
     return false;
@@ -90,18 +90,29 @@ bool Sol_MultigridPressureMixed3DDeviceD::initialize_storage(int nx_val, int ny_
 }
 
 // r = (b - bshift) - A x in FP64, written to _r32 scaled by `scale`; returns
-// the FP64 norms of r (blocking: the refinement loop needs them)
+// the FP64 norms of r and the interior sum of x (blocking: the refinement
+// loop needs them)
 bool Sol_MultigridPressureMixed3DDeviceD::fp64_residual(double bshift, double scale, double &sumsq, double &maxabs,
-                                                        double &sum) {
+                                                        double &sum, double &xsum) {
   void *s = ThreadManager::get_compute_stream();
   ocu_grid3d x = _x.desc(), b = _rhs->desc(), r = _r32.desc();
   bool ok = ok_status(ocu_ir_residual(&x, &b, bshift, &r, scale, get_h(0), _fx, _fy, _fz, _ws, s), "ir_residual");
-  ok = ok && ok_status(ocu_norm3_finalize(_ws, _host_dev, s), "ir_residual");  // into pinned host memory
+  ok = ok && ok_status(ocu_norm4_finalize(_ws, _host_dev, s), "ir_residual");  // into pinned host memory
   ok = ok && ok_status(ocu_stream_synchronize(s), "ir_residual");
   sumsq = _host[0];
   maxabs = _host[1];
   sum = _host[2];
+  xsum = _host[3];
   return ok;
+}
+
+// every face of x (and so of the homogeneous correction) is a copy of an
+// interior cell: the correction can update the face ghosts itself
+bool Sol_MultigridPressureMixed3DDeviceD::copy_faces() const {
+  const BoundaryCondition *all[6] = {&bc.xpos, &bc.xneg, &bc.ypos, &bc.yneg, &bc.zpos, &bc.zneg};
+  for (auto b : all)
+    if (!(b->type == BC_PERIODIC || (b->type == BC_NEUMANN && b->value == 0))) return false;
+  return nu1 >= 1;
 }
 
 bool Sol_MultigridPressureMixed3DDeviceD::interior_mean(const Grid3DDevice<double> &g, double &mean) {
@@ -188,15 +199,17 @@ bool Sol_MultigridPressureMixed3DDeviceD::do_fmg(double tolerance, int max_iter,
   BoundaryConditionSet bc0 = bc;
   check_ok(apply_3d_boundary_conditions_level1(_x, bc0, hx(0), hy(0), hz(0)));
 
-  double prev_max = bmax, prev_res = 0;
+  double prev_max = bmax, prev_res = 0, xsum = 0;
   int prev_cycles = 0;
   bool converged = false;
+  const bool faces = copy_faces();
+  double *shift = _singular ? _ws + 5 * ocu_norm_parts() : nullptr;
   for (int it = 0;; it++) {
     int ex = 0;
     frexp(prev_max > 0 ? prev_max : 1.0, &ex);
     double scale = ldexp(1.0, -ex), inv_scale = ldexp(1.0, ex);
     double s2 = 0, mx = 0, sm = 0;
-    if (!fp64_residual(bm, scale, s2, mx, sm)) {
+    if (!fp64_residual(bm, scale, s2, mx, sm, xsum)) {
       add_error();
       break;
     }
@@ -226,12 +239,16 @@ bool Sol_MultigridPressureMixed3DDeviceD::do_fmg(double tolerance, int max_iter,
     prev_res = res;
     _mg_f.relax(0, nu1, RO_RED_BLACK);
     ocu_grid3d x = _x.desc(), e = _mg_f.pressure().desc();
-    ok_status(ocu_ir_correct(&x, &e, inv_scale, s), "ir_correct");
-    check_ok(apply_3d_boundary_conditions_level1(_x, bc0, hx(0), hy(0), hz(0)));
+    // singular problems: the correction also removes the interior mean of
+    // x + e/s (from the residual pass's sum of x and one pass over e), so x
+    // stays mean-free and needs no final pass
+    if (shift) ok_status(ocu_ir_shift(&e, inv_scale, xsum, _ws + 4 * ocu_norm_parts(), shift, s), "ir_shift");
+    ok_status(ocu_ir_correct_ex(&x, &e, inv_scale, shift, faces ? 1 : 0, s), "ir_correct");
+    if (!faces) check_ok(apply_3d_boundary_conditions_level1(_x, bc0, hx(0), hy(0), hz(0)));
     if (_mg_f.any_error()) add_error();
   }
   _mg_f.settle();
-  if (_singular) {
+  if (_singular && ir_iterations == 0) {
     double xm = 0;
     check_ok(interior_mean(_x, xm));
     ocu_grid3d x = _x.desc();

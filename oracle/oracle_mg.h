@@ -643,8 +643,10 @@ inline bool is_singular(const ocu_bc_set &bc) {
  *     else:                  cycles V-cycles
  *     (each V-cycle: relax(l, nu1) + restrict going down, coarsest relaxed in
  *      FP64, prolong + relax(l, nu2) going up for l > 0), then relax(0, nu1)
- *     x += (double)e32 / s ; BC(x)
- *   if singular: x -= mean(x); BC(x)
+ *     singular: shift = (sum x + sum e32 / s) / N, the mean of x + e32/s
+ *     x = (x + (double)e32 / s) - shift (singular) or x + (double)e32 / s ;
+ *     BC(x)
+ *   if singular and no correction was made: x -= mean(x); BC(x)
  */
 /* V-cycles for refinement iteration it >= 1 when the previous iteration
  * ran prev_cycles V-cycles and took the residual from prev_res to res: the
@@ -783,6 +785,7 @@ struct MixedIR {
       std::frexp(prev_max > 0 ? prev_max : 1.0, &ex);
       double scale = std::ldexp(1.0, -ex), inv = std::ldexp(1.0, ex);
       residual_shifted_f64(x, b, bm, r, x, mg.get_h(0), mg.fx, mg.fy, mg.fz);
+      const double xsum = mean_interior(x) * ((double)nx * ny * nz);
       double s2 = 0, mx = 0;
       for (double q : r) { s2 += q * q; mx = std::max(mx, std::fabs(q)); }
       res = (convergence & OCU_CONVERGENCE_CRITERIA_LINF) ? mx
@@ -809,12 +812,23 @@ struct MixedIR {
       }
       prev_res = res;
       mg.relax(0, nu1, MGSolver<float>::RO_RED_BLACK);
+      double shift = 0;
+      if (singular) {
+        double esum = 0;
+        for (int i = 0; i < nx; i++)
+          for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) esum += (double)ev.at(i, j, k);
+        shift = (xsum + inv * esum) / ((double)nx * ny * nz);
+      }
       for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++)
-          for (int k = 0; k < nz; k++) xv.at(i, j, k) = xv.at(i, j, k) + (double)ev.at(i, j, k) * inv;
+          for (int k = 0; k < nz; k++) {
+            const double v = xv.at(i, j, k) + (double)ev.at(i, j, k) * inv;
+            xv.at(i, j, k) = singular ? v - shift : v;
+          }
       bc_fill_level1<double>(x, bc0, mg.hx(0), mg.hy(0), mg.hz(0));
     }
-    if (singular) {
+    if (singular && ir_iterations == 0) {
       double xm = mean_interior(x);
       for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++)

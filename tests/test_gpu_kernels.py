@@ -605,7 +605,7 @@ def test_ir_residual(gpu, size):
     """ocu_ir_residual: r = (b - bshift) - A x in FP64 with the reference
     residual's operand order, r32 = (float)(r * scale) bit for bit (numpy
     performs the same IEEE double operations in the same order), and the
-    (sum r^2, max |r|, sum r) partials reduced by ocu_norm3_finalize."""
+    (sum r^2, max |r|, sum r, sum x) partials reduced by ocu_norm4_finalize."""
     import torch
     from opencurrent_amd import _lib
     h, fx, fy, fz, bshift, scale = 1 / 32, 1.0, .64, .25, .0123, 2.0 ** 7
@@ -622,20 +622,85 @@ def test_ir_residual(gpu, size):
     r = (b.interior() - bshift) - (cz * (zm + zp) + cy * (ym + yp) + cx * (xm + xp) - cd * xc)
     gx, gb, gr = (D().DeviceGrid.from_host(g) for g in (x, b, r32))
     hip = _lib.hip()
-    ws = torch.zeros(3 * 2048 + 3, dtype=torch.float64, device="cuda")
-    out = ws[3 * 2048:]
+    ws = torch.zeros(4 * 2048 + 4, dtype=torch.float64, device="cuda")
+    out = ws[4 * 2048:]
     _lib.check(hip.ocu_ir_residual(gx.ptr(), gb.ptr(), bshift, gr.ptr(), scale, h, fx, fy, fz,
                                    C.cast(ws.data_ptr(), _lib.DP), None), "ir_residual")
-    _lib.check(hip.ocu_norm3_finalize(C.cast(ws.data_ptr(), _lib.DP), C.cast(out.data_ptr(), _lib.DP), None),
-               "norm3_finalize")
+    _lib.check(hip.ocu_norm4_finalize(C.cast(ws.data_ptr(), _lib.DP), C.cast(out.data_ptr(), _lib.DP), None),
+               "norm4_finalize")
     torch.cuda.synchronize()
     got = gr.to_host().interior()
     want = (r * scale).astype(np.float32)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
-    s2, mx, sm = (float(v) for v in out.cpu())
+    s2, mx, sm, sx = (float(v) for v in out.cpu())
     assert mx == np.abs(r).max()
     assert abs(s2 - (r * r).sum()) <= 1e-12 * (r * r).sum()
     assert abs(sm - r.sum()) <= 1e-12 * np.abs(r).sum()
+    assert abs(sx - xc.sum()) <= 1e-12 * np.abs(xc).sum()
+
+
+@pytest.mark.parametrize("size", [(16, 12, 20), (8, 6, 256), (5, 7, 200), (3, 4, 130), (4, 9, 64), (2, 3, 1),
+                                  (6, 5, 300)])
+def test_grid_stats3(gpu, size):
+    """(sum g^2, max|g|, sum g) of an FP64 interior, row and tiled forms"""
+    import torch
+    from opencurrent_amd import _lib
+    g = random_grid(*size, np.float64, 161)
+    gg = D().DeviceGrid.from_host(g)
+    ws = torch.zeros(3 * 2048 + 3, dtype=torch.float64, device="cuda")
+    out = ws[3 * 2048:]
+    hip = _lib.hip()
+    _lib.check(hip.ocu_grid_stats3(gg.ptr(), C.cast(ws.data_ptr(), _lib.DP), None), "stats3")
+    _lib.check(hip.ocu_norm3_finalize(C.cast(ws.data_ptr(), _lib.DP), C.cast(out.data_ptr(), _lib.DP), None),
+               "norm3_finalize")
+    s2, mx, sm = (float(v) for v in out.cpu())
+    x = g.interior()
+    assert mx == np.abs(x).max()
+    assert abs(s2 - (x * x).sum()) <= 1e-12 * (x * x).sum()
+    assert abs(sm - x.sum()) <= 1e-12 * np.abs(x).sum()
+
+
+@pytest.mark.parametrize("size", [(16, 12, 20), (8, 6, 256), (5, 7, 200), (2, 3, 1), (6, 5, 300)])
+def test_ir_shift_and_correct(gpu, size):
+    """ocu_ir_shift gives the interior mean of x + e*inv_scale from the sum of
+    x and one pass over e; ocu_ir_correct_ex applies (x + e*inv) - shift
+    bit for bit over the interior and, in face mode, the six face-ghost
+    layers, leaving edges, corners and padding alone."""
+    import torch
+    from opencurrent_amd import _lib
+    hip = _lib.hip()
+    x = random_grid(*size, np.float64, 171)
+    e = HostGrid(*size, dtype=np.float32)
+    e.buf[:] = (2 * uniform(172, e.buf.shape) - 1).astype(np.float32)
+    inv = 2.0 ** -9
+    xi, ei = x.interior(), e.interior()
+    ws = torch.zeros(2048 + 1, dtype=torch.float64, device="cuda")
+    shift_dev = ws[2048:]
+    ge = D().DeviceGrid.from_host(e)
+    _lib.check(hip.ocu_ir_shift(ge.ptr(), inv, float(xi.sum()), C.cast(ws.data_ptr(), _lib.DP),
+                                C.cast(shift_dev.data_ptr(), _lib.DP), None), "ir_shift")
+    shift = float(shift_dev.cpu()[0])
+    n = float(np.prod(size))
+    want_shift = (xi.sum() + inv * ei.astype(np.float64).sum()) / n
+    assert abs(shift - want_shift) <= 1e-14 * (np.abs(xi).sum() + inv * np.abs(ei).sum()) / n
+    X, E = x.full(), e.full()
+    for faces in (0, 1):
+        gx = D().DeviceGrid.from_host(x)
+        sp = C.cast(shift_dev.data_ptr(), _lib.DP) if faces else None
+        _lib.check(hip.ocu_ir_correct_ex(gx.ptr(), ge.ptr(), inv, sp, faces, None), "ir_correct_ex")
+        got = gx.to_host().full()
+        want = X.copy()
+        nx, ny, nz = size
+        ii, jj, kk = np.meshgrid(np.arange(X.shape[0]) - 1, np.arange(X.shape[1]) - 1, np.arange(X.shape[2]) - 1,
+                                 indexing="ij")
+        ghost = ((ii < 0) | (ii >= nx)).astype(int) + ((jj < 0) | (jj >= ny)) + ((kk < 0) | (kk >= nz))
+        inbox = (ii <= nx) & (jj <= ny) & (kk <= nz)
+        sel = (ghost == 0) if not faces else ((ghost <= 1) & inbox)
+        v = X + E.astype(np.float64) * inv
+        if faces:
+            v = v - shift
+        want[sel] = v[sel]
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), faces
 
 
 RELAX_BC_CASES = {
