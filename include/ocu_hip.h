@@ -144,6 +144,22 @@ int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const oc
 int ocu_mg_relax_fused_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                           const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx,
                           double hy, double hz, int first_color, int flags, void *stream);
+/* ocu_mg_prolong_bc(u_coarse, u_in, quirk, bc, hx, hy, hz) followed by
+ * ocu_mg_relax_fused_bc(u_in, u_out, b, ..., flags) in one launch, bit for
+ * bit, except that u_in is left as it was (the sweep reads u_in plus the
+ * prolonged coarse correction and its ghost transforms).  flags: only
+ * OCU_FUSED_WRITE_GHOSTS.  Needs fine = 2 x coarse, 129..256 cells per row
+ * with aligned rows (ocu_mg_relax_fused_prolong_ok), Neumann or Dirichlet
+ * faces, the coarse ghost layer filled; else OCU_ERR_UNSUPPORTED.  No
+ * reference counterpart as one call. */
+int ocu_mg_relax_fused_prolong_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out,
+                                  const ocu_grid3d *b, const ocu_grid3d *u_coarse, int quirk,
+                                  const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx,
+                                  double hy, double hz, int first_color, int flags, void *stream);
+/* 1 when ocu_mg_relax_fused_prolong_bc can run for this fine/coarse pair
+ * and these faces (u stands for u_in, u_out and b), else 0 */
+int ocu_mg_relax_fused_prolong_ok(const ocu_grid3d *u, const ocu_grid3d *u_coarse,
+                                  const ocu_bc_set *bc);
 /* r = b - A u over the interior; calculate_residual (devd.cu:94-114,
  * devf.cu:68-90; constants as in the launchers devd.cu:1372-1405). */
 int ocu_mg_residual(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r, double h,

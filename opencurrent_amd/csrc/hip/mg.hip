@@ -7,6 +7,7 @@
 // -ffp-contract=off so the results match the CPU restatement bit for bit.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "ghost_l1.h"
 #include "rows.h"
@@ -251,6 +252,79 @@ __global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T>
   }
 }
 
+// prolongation weights 27/64, 9/64, 3/64, 1/64 (devd.cu:118-223)
+template <typename T> struct PW;
+template <> struct PW<double> {
+  static constexpr double c0 = 0.421875, c1 = 0.140625, c2 = 0.046875, c3 = 0.015625;
+};
+template <> struct PW<float> {
+  static constexpr float c0 = 0.421875f, c1 = 0.140625f, c2 = 0.046875f, c3 = 0.015625f;
+};
+
+// a coarse row segment of VC cells per lane with its z ghost (lane 0:
+// k = -1, the lane past the row: k = nzc)
+template <typename T, int V>
+struct CRow {
+  Vec<T, V> v;
+  T g;
+};
+
+// The prolongation of one fine row (I, J) added to f (V fine cells per lane,
+// V >= 2): the four coarse rows it reads (planes ic and ic +- 1 toward the
+// fine plane, rows jc and jc +- 1 toward the fine row), and for the corner
+// quirk the row of plane ic+1, are loaded here.  The operations and their
+// order are those of k_prolong_rows, so the sum is the same bit for bit.
+template <typename T, int V>
+__device__ __forceinline__ void prolong_row_add(const DGrid<T> &uc, int I, int J, int quirk, int lane, bool active,
+                                                int hi_lane, int nzf, Vec<T, V> &f) {
+  constexpr int VC = V / 2;
+  typedef CRow<T, VC> CR;
+  const int kc = active ? lane * VC : 0;
+  const int gk = lane == 0 ? -1 : uc.nz;
+  const bool g_in_data = lane == hi_lane && nzf < 64 * V;
+  const int jc = J >> 1, dy = J & 1, ic = I >> 1, dx = I & 1;
+  const int jc2 = jc + (dy ? 1 : -1), icx = ic + (dx ? 1 : -1);
+  auto ld = [&](int i, int j, CR &d) {
+    const T *p = uc.p + (int64_t)i * uc.xs + (int64_t)j * uc.ys;
+    d.v = *reinterpret_cast<const Vec<T, VC> *>(p + kc);
+    d.g = p[gk];
+    if (g_in_data) d.v.v[0] = d.g;
+  };
+  CR c[2], X[2], q1;
+  ld(ic, jc, c[0]);
+  ld(ic, jc2, c[1]);
+  ld(icx, jc, X[0]);
+  ld(icx, jc2, X[1]);
+  const bool qrow = quirk && dx == 0 && dy == 1;  // its dz = 1 children take the corner from plane ic+1
+  T qzh = 0;
+  if (qrow) {
+    ld(ic + 1, jc2, q1);
+    qzh = from_upper(q1.v.v[0], q1.g);
+  }
+  T zl[2][2], zh[2][2];  // [plane: 0 = c, 1 = X][row]
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    zl[0][r] = from_lower(c[r].v.v[VC - 1], c[r].g);
+    zh[0][r] = from_upper(c[r].v.v[0], c[r].g);
+    zl[1][r] = from_lower(X[r].v.v[VC - 1], X[r].g);
+    zh[1][r] = from_upper(X[r].v.v[0], X[r].g);
+  }
+#pragma unroll
+  for (int e = 0; e < V; e++) {
+    const int mm = e >> 1, dz = e & 1;
+    auto zs = [&](const CR &row, T lo, T hi) {
+      return dz ? (mm + 1 < VC ? row.v.v[mm + 1] : hi) : (mm > 0 ? row.v.v[mm - 1] : lo);
+    };
+    const T C = c[0].v.v[mm], Xv = X[0].v.v[mm], Y = c[1].v.v[mm];
+    const T Z = zs(c[0], zl[0][0], zh[0][0]);
+    const T XY = X[1].v.v[mm];
+    const T XZ = zs(X[0], zl[1][0], zh[1][0]);
+    const T YZ = zs(c[1], zl[0][1], zh[0][1]);
+    const T XYZ = (qrow && dz) ? (mm + 1 < VC ? q1.v.v[mm + 1] : qzh) : zs(X[1], zl[1][1], zh[1][1]);
+    f.v[e] = f.v[e] + PW<T>::c0 * C + PW<T>::c1 * (Xv + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+  }
+}
+
 // ------------------------------------------------------------------------
 // fused red+black iteration, whole-row variant
 // ------------------------------------------------------------------------
@@ -325,13 +399,18 @@ __device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, i
 // are the halo, so a block writes R*NW-4 rows.
 //
 // ZERO: u_in is the zero grid with its level-1 ghosts (not read; the values
-// are the ghost transforms of 0).  wghost: also write the level-1 ghost layer
-// of u_out from each boundary cell (Neumann/Dirichlet faces only).
-template <typename T, int V, int R, int NW, bool ZERO>
+// are the ghost transforms of 0).  PRO: the sweep's input is u_in plus the
+// prolongation of uc (fine = 2 x coarse, V >= 2), with the level-1 ghosts
+// that prolongation writes: each row is read from u_in, the coarse rows'
+// contribution is added in registers, and ghost rows, planes and z ends are
+// the face transforms of the prolonged boundary cells; u_in is not written.
+// wghost: also write the level-1 ghost layer of u_out from each boundary
+// cell.  ZERO, PRO and wghost need Neumann/Dirichlet faces.
+template <typename T, int V, int R, int NW, bool ZERO, bool PRO>
 __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
                                                                typename DevCFor<T>::type c, int first_color,
                                                                int tiles_j, int ichunk, int ntiles,
-                                                               GhostL1<T> gw, int wghost) {
+                                                               GhostL1<T> gw, int wghost, DGrid<T> uc, int quirk) {
   typedef Vec<T, V> VT;
   constexpr int NR = R * NW;
   __shared__ VT sU0[NR][64];
@@ -370,10 +449,25 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
   auto tile = [&](int r) { return t0 + r >= 2 && t0 + r <= NR - 3; };
 
   auto load = [&](int r, int p, VT &d, T &g) {
-    if (ZERO) {  // homogeneous faces: every ghost of the zero grid is +0.0
+    if constexpr (ZERO) {  // homogeneous faces: every ghost of the zero grid is +0.0
 #pragma unroll
       for (int e = 0; e < V; e++) d.v[e] = (T)0;
       g = (T)0;
+    } else if constexpr (PRO) {
+      // the row as the plain load would clamp it, its interior source row
+      // and the ghost offset between them
+      const int jj = min(max(jb + r, -1), ny), pp = min(max(p, -1), nx);
+      const int jr = min(max(jj, 0), ny - 1), pr = min(max(pp, 0), nx - 1);
+      VT f = *reinterpret_cast<const VT *>(uin.p + (int64_t)pr * uin.xs + (int64_t)jr * uin.ys + kc);
+      prolong_row_add<T, V>(uc, pr, jr, quirk, lane, active, hi_lane, nz, f);
+      if (pp != pr || jj != jr) {
+#pragma unroll
+        for (int e = 0; e < V; e++) f.v[e] = gw.chain(pp - pr, jj - jr, 0, f.v[e]);
+      }
+      const T top = (nz == 64 * V) ? f.v[V - 1] : from_lower(f.v[V - 1], (T)0);
+      g = lane == 0 ? gw.xf(5, f.v[0]) : gw.xf(4, top);
+      d = f;
+      if (g_in_data) d.v[0] = g;
     } else {
       const T *row = urow[r] + (int64_t)min(max(p, -1), nx) * uin.xs;
       d = *reinterpret_cast<const VT *>(row + kc);
@@ -735,13 +829,6 @@ __global__ void k_norm_finalize(const double *parts, int n, double *out) {
 // prolongation + correction (devd.cu:118-223, devf.cu:94-197)
 // ------------------------------------------------------------------------
 
-template <typename T> struct PW;
-template <> struct PW<double> {
-  static constexpr double c0 = 0.421875, c1 = 0.140625, c2 = 0.046875, c3 = 0.015625;
-};
-template <> struct PW<float> {
-  static constexpr float c0 = 0.421875f, c1 = 0.140625f, c2 = 0.046875f, c3 = 0.015625f;
-};
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int quirk, GhostL1<T> gw, int wghost) {
@@ -775,11 +862,6 @@ __global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int q
 // sliding window, with their z ghosts in g for the DPP shifts; the fine row
 // is read, corrected and written once.  Same operations as k_prolong,
 // including its corner choice and the level-1 ghost writes.
-template <typename T, int V>
-struct CRow {
-  Vec<T, V> v;
-  T g;
-};
 
 template <typename T, int V, int NW>
 __global__ __launch_bounds__(64 * NW) void k_prolong_rows(DGrid<T> uc, DGrid<T> uf, int quirk, GhostL1<T> gw,
@@ -1354,19 +1436,28 @@ int row_vec_width(const ocu_grid3d *u, const ocu_grid3d *o, const ocu_grid3d *b)
 }
 
 // Concurrent blocks of a kernel on the current device (CUs x blocks per CU),
-// cached per device and kernel.
+// cached per device, kernel and block size.
 template <typename K>
 int device_slots(K kern, int threads) {
-  static int cache[64];
+  struct Entry {
+    const void *k;
+    int dev, threads, slots;
+  };
+  static std::mutex mu;
+  static Entry cache[128];
+  static int used = 0;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cache[dev]) {
-    int cus = 0, per = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per <= 0) per = 1;
-    cache[dev] = cus * per;
-  }
-  return cache[dev];
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+  const void *key = reinterpret_cast<const void *>(kern);
+  std::lock_guard<std::mutex> lock(mu);
+  for (int q = 0; q < used; q++)
+    if (cache[q].k == key && cache[q].dev == dev && cache[q].threads == threads) return cache[q].slots;
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per <= 0) per = 1;
+  const int slots = cus * per;
+  if (used < 128) cache[used++] = Entry{key, dev, threads, slots};
+  return slots;
 }
 
 // i-planes per block of the whole-row kernel.  A block reads ichunk+3
@@ -1390,20 +1481,21 @@ int fused_ichunk(int nx, int tiles_j, int slots) {
   return best;
 }
 
-template <typename T, int V, bool ZERO>
+template <typename T, int V, bool ZERO, bool PRO = false>
 int launch_fused_rows(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                       const typename DevCFor<T>::type &c, int first_color, const GhostL1<T> &gw, int wghost,
-                      hipStream_t stream) {
+                      hipStream_t stream, const ocu_grid3d *uc = nullptr, int quirk = 0) {
   typedef RowsShape<T, V> RS;
   constexpr int NR = RS::R * RS::NW;
-  auto kern = k_relax_fused_rows<T, V, RS::R, RS::NW, ZERO>;
+  auto kern = k_relax_fused_rows<T, V, RS::R, RS::NW, ZERO, PRO>;
   const int nx = u_in->nx, ny = u_in->ny;
   const int tiles_j = (ny + NR - 5) / (NR - 4);
   const int ichunk = fused_ichunk(nx, tiles_j, device_slots(kern, 64 * RS::NW));
   const int ntiles = tiles_j * ((nx + ichunk - 1) / ichunk);
   const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
   kern<<<nblocks, dim3(64, RS::NW), 0, stream>>>(dgrid<T>(u_in), dgrid<T>(u_out), dgrid<T>(b), c, first_color,
-                                                 tiles_j, ichunk, ntiles, gw, wghost);
+                                                 tiles_j, ichunk, ntiles, gw, wghost,
+                                                 PRO ? dgrid<T>(uc) : dgrid<T>(u_in), quirk);
   OCU_HIP_LAUNCHED("ocu_mg_relax_fused");
 }
 
@@ -1473,6 +1565,22 @@ int relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3
   const GhostL1<float> gw = ghost_l1<float>(bc, hx, hy, hz);
   return zero ? launch_fused<float, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream))
               : launch_fused<float, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream));
+}
+
+// The whole-row width of the prolongation-fused sweep (4), or 0 when it
+// cannot run: fine exactly twice the coarse grid, rows of four cells per
+// lane (129..256 cells; on shorter rows the separate prolongation is the
+// faster of the two), coarse rows aligned for half-width vectors, faces that
+// the ghost transforms cover, and a coarse ghost layer.
+int fused_prolong_width(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                        const ocu_grid3d *uc, const ocu_bc_set *bc) {
+  if (!uc || !bc || any_periodic(bc) || !gfaces_local(bc)) return 0;
+  if (uc->dtype != u_in->dtype || u_in->nx != 2 * uc->nx || u_in->ny != 2 * uc->ny || u_in->nz != 2 * uc->nz)
+    return 0;
+  if (uc->gx < 1 || uc->gy < 1 || uc->gz < 1) return 0;
+  const int V = row_vec_width(u_in, u_out, b);
+  if (V != 4 || !row_aligned(uc, V / 2)) return 0;
+  return V;
 }
 
 // fine cells per lane of k_res_restrict_rows (whole rows, whole coarse
@@ -1738,6 +1846,49 @@ int ocu_mg_relax_fused_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const
                           const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx, double hy, double hz,
                           int first_color, int flags, void *stream) {
   return relax_fused(u_in, u_out, b, p, bc, hx, hy, hz, first_color, flags, stream);
+}
+
+int ocu_mg_relax_fused_prolong_ok(const ocu_grid3d *u, const ocu_grid3d *u_coarse, const ocu_bc_set *bc) {
+  if (!u || !u_coarse || check_grid(u, "ocu_mg_relax_fused_prolong_ok") ||
+      check_grid(u_coarse, "ocu_mg_relax_fused_prolong_ok"))
+    return 0;
+  return fused_prolong_width(u, u, u, u_coarse, bc) ? 1 : 0;
+}
+
+int ocu_mg_relax_fused_prolong_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                                  const ocu_grid3d *u_coarse, int quirk, const ocu_mg_level_params *p,
+                                  const ocu_bc_set *bc, double hx, double hy, double hz, int first_color, int flags,
+                                  void *stream) {
+  const char *what = "ocu_mg_relax_fused_prolong";
+  if (int e = check_pair(u_in, b, what)) return e;
+  if (int e = check_grid(u_out, "ocu_mg_relax_fused_prolong(out)")) return e;
+  if (int e = check_grid(u_coarse, "ocu_mg_relax_fused_prolong(coarse)")) return e;
+  OCU_REQUIRE(layout_match(u_in, u_out) && u_in->dtype == u_out->dtype && u_in->nx == u_out->nx &&
+                  u_in->ny == u_out->ny && u_in->nz == u_out->nz,
+              "%s: u_out must be congruent with u_in", what);
+  OCU_REQUIRE(u_out->gx >= 1 && u_out->gy >= 1 && u_out->gz >= 1, "%s: u_out needs one ghost layer", what);
+  OCU_REQUIRE((flags & ~OCU_FUSED_WRITE_GHOSTS) == 0, "%s: bad flags %d", what, flags);
+  OCU_REQUIRE(u_in->base != u_out->base && u_coarse->base != u_out->base, "%s: u_out must not alias an input",
+              what);
+  OCU_REQUIRE(p && bc, "%s: null params", what);
+  OCU_REQUIRE(first_color == 0 || first_color == 1, "%s: bad first_color %d", what, first_color);
+  const int V = fused_prolong_width(u_in, u_out, b, u_coarse, bc);
+  if (!V) {
+    set_error("%s: needs fine = 2 x coarse, whole rows of 4 cells per lane, aligned coarse rows and "
+              "Neumann or Dirichlet faces", what);
+    return OCU_ERR_UNSUPPORTED;
+  }
+  const int wghost = (flags & OCU_FUSED_WRITE_GHOSTS) ? 1 : 0;
+  if (u_in->dtype == OCU_F64) {
+    const DevC64 c = dev_c64(p);
+    const GhostL1<double> gw = ghost_l1<double>(bc, hx, hy, hz);
+    return launch_fused_rows<double, 4, false, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream),
+                                                     u_coarse, quirk);
+  }
+  const DevC32 c = dev_c32(p);
+  const GhostL1<float> gw = ghost_l1<float>(bc, hx, hy, hz);
+  return launch_fused_rows<float, 4, false, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream), u_coarse,
+                                                  quirk);
 }
 
 int ocu_mg_residual(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *r, double h, double fx,

@@ -118,6 +118,11 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   // written; the next fused relaxation starts from zero without reading u
   static const int kMaxLevels = 40;
   bool _u_zero[kMaxLevels];
+  // _u_pro[l]: the prolongation of level l+1 into level l (with its level-1
+  // fill) is pending: the next fused relaxation of level l adds it while it
+  // reads u, and anything else that reads level l, or writes level l+1,
+  // runs it first
+  bool _u_pro[kMaxLevels];
 
   Grid3DDevice<T> &get_u(int level) { return *_u_grid[level]; }
   Grid3DDevice<T> &get_r(int level) { return *_r_grid[level]; }
@@ -141,6 +146,11 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   void relax_coarsest(int level, int iterations, RelaxOrder order);
   bool defer_ok(int level) const;
   void materialize(int level);
+  void run_prolong(int coarse_level, int fine_level);
+  // level l is about to change: a pending prolongation that reads it runs first
+  void flush_reader(int level) {
+    if (level >= 1 && _u_pro[level - 1]) materialize(level - 1);
+  }
   virtual void settle();
   int _bottom_level;  // first level of the LDS bottom, -1 if none fits
   virtual int bottom_start() const;
@@ -154,6 +164,9 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   bool store_residuals;
   // run the small coarse levels of each V-cycle as one LDS-resident launch
   bool use_bottom;
+  // with use_fused_relax: a prolongation is added by the fused relaxation
+  // that follows it instead of a launch of its own
+  bool use_fused_prolong;
 
   Sol_MultigridPressure3DDevice();
   ~Sol_MultigridPressure3DDevice();

@@ -305,7 +305,8 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   store_residuals = false;
   use_bottom = true;
   _bottom_level = -1;
-  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = false;
+  use_fused_prolong = true;
+  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = _u_pro[l] = false;
 }
 
 template <typename T>
@@ -373,6 +374,10 @@ bool Sol_MultigridPressure3DDevice<T>::defer_ok(int level) const {
 
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::materialize(int level) {
+  if (_u_pro[level]) {
+    _u_pro[level] = false;
+    run_prolong(level + 1, level);
+  }
   if (!_u_zero[level]) return;
   _u_zero[level] = false;
   check_ok(get_u(level).clear_zero());
@@ -408,6 +413,13 @@ void Sol_MultigridPressure3DDevice<T>::bottom_vcycle(int top) {
     h[3 * l + 1] = hy(top + l);
     h[3 * l + 2] = hz(top + l);
   }
+  for (int l = top; l <= C; l++) {
+    if (_u_pro[l]) {  // (a deferred zero stays deferred: the bottom takes it as a flag)
+      _u_pro[l] = false;
+      run_prolong(l + 1, l);
+    }
+    flush_reader(l);
+  }
   ocu_bc_set cbc = to_c_bc(get_bc_at_level(top));
   int coarse_iters = max3(nx(C) * ny(C), ny(C) * nz(C), nx(C) * nz(C)) / 2;
   ok_status(ocu_mg_bottom_vcycle(u, b, p, h, nlev, &cbc, nu1, nu2, coarse_iters, _coarse_mode, _u_zero[top] ? 1 : 0,
@@ -418,6 +430,8 @@ void Sol_MultigridPressure3DDevice<T>::bottom_vcycle(int top) {
 
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::clear_zero(int level) {
+  flush_reader(level);
+  _u_pro[level] = false;  // the grid becomes zero: the pending correction is moot
   if (defer_ok(level)) {
     _u_zero[level] = true;
     return;
@@ -429,6 +443,8 @@ void Sol_MultigridPressure3DDevice<T>::clear_zero(int level) {
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::apply_boundary_conditions(int level) {
   if (_u_zero[level]) return;  // the deferred zero grid carries its ghosts
+  if (_u_pro[level]) return;   // the pending prolongation writes them
+  flush_reader(level);
   if (!apply_3d_boundary_conditions_level1(get_u(level), get_bc_at_level(level), hx(level), hy(level), hz(level))) {
     printf("[ERROR] Sol_MultigridPressure3DDevice::apply_boundary_conditions - failed at level %d \n", level);
     add_error();
@@ -438,6 +454,7 @@ void Sol_MultigridPressure3DDevice<T>::apply_boundary_conditions(int level) {
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::relax_colour(int level, int color) {
   materialize(level);
+  flush_reader(level);
   ocu_mg_level_params p;
   level_params(level, &p);
   ocu_grid3d u = get_u(level).desc(), b = get_b(level).desc();
@@ -448,6 +465,7 @@ void Sol_MultigridPressure3DDevice<T>::relax_colour(int level, int color) {
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::relax_colour_bc(int level, int color) {
   materialize(level);
+  flush_reader(level);
   ocu_mg_level_params p;
   level_params(level, &p);
   ocu_bc_set cbc = to_c_bc(get_bc_at_level(level));
@@ -484,6 +502,7 @@ bool Sol_MultigridPressure3DDevice<T>::relax_bc_ok(int level) const {
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::relax_coarsest(int level, int iterations, RelaxOrder order) {
   materialize(level);
+  flush_reader(level);
   ocu_mg_level_params p;
   level_params(level, &p);
   ocu_grid3d u = get_u(level).desc(), b = get_b(level).desc();
@@ -509,12 +528,18 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
     return (order == RO_BLACK_RED || (order == RO_SYMMETRIC && it >= iterations / 2)) ? 1 : 0;
   };
 
+  flush_reader(level);
+  if (iterations <= 0) {
+    materialize(level);
+    return;
+  }
   if (use_fused_relax && _fused_ok) {
     // pairs of out-of-place sweeps u -> alt -> u, so the result always ends
     // in u; each sweep also writes the level-1 ghosts of its output (the
     // fill the reference runs after every iteration), and the first one
-    // starts from the zero grid when u is a deferred clear.  An odd last
-    // iteration runs in place, one launch per colour.
+    // starts from the zero grid when u is a deferred clear, or adds a
+    // pending prolongation while it reads u.  An odd last iteration runs in
+    // place, one launch per colour.
     ocu_mg_level_params p;
     level_params(level, &p);
     ocu_bc_set cbc = to_c_bc(get_bc_at_level(level));
@@ -525,11 +550,19 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
     const double lhx = hx(level), lhy = hy(level), lhz = hz(level);
     int it = 0;
     for (; it + 1 < iterations; it += 2) {
-      const int zero = _u_zero[level] ? OCU_FUSED_ZERO_INPUT : 0;
-      _u_zero[level] = false;
-      ok_status(ocu_mg_relax_fused_bc(&du, &da, &db, &p, &cbc, lhx, lhy, lhz, first_colour(it),
-                                      OCU_FUSED_WRITE_GHOSTS | zero, s),
-                "Sol_MultigridPressure3DDevice::relax");
+      if (_u_pro[level]) {
+        _u_pro[level] = false;
+        ocu_grid3d dc = get_u(level + 1).desc();
+        ok_status(ocu_mg_relax_fused_prolong_bc(&du, &da, &db, &dc, 1, &p, &cbc, lhx, lhy, lhz, first_colour(it),
+                                                OCU_FUSED_WRITE_GHOSTS, s),
+                  "Sol_MultigridPressure3DDevice::relax");
+      } else {
+        const int zero = _u_zero[level] ? OCU_FUSED_ZERO_INPUT : 0;
+        _u_zero[level] = false;
+        ok_status(ocu_mg_relax_fused_bc(&du, &da, &db, &p, &cbc, lhx, lhy, lhz, first_colour(it),
+                                        OCU_FUSED_WRITE_GHOSTS | zero, s),
+                  "Sol_MultigridPressure3DDevice::relax");
+      }
       ok_status(ocu_mg_relax_fused_bc(&da, &du, &db, &p, &cbc, lhx, lhy, lhz, first_colour(it + 1),
                                       OCU_FUSED_WRITE_GHOSTS, s),
                 "Sol_MultigridPressure3DDevice::relax");
@@ -596,6 +629,20 @@ template <typename T>
 void Sol_MultigridPressure3DDevice<T>::prolong(int coarse_level, int fine_level) {
   materialize(coarse_level);
   materialize(fine_level);
+  flush_reader(fine_level);
+  ocu_grid3d uc = get_u(coarse_level).desc(), uf = get_u(fine_level).desc();
+  if (use_fused_prolong && use_fused_relax && _fused_ok && coarse_level == fine_level + 1) {
+    ocu_bc_set cbc = to_c_bc(get_bc_at_level(fine_level));
+    if (ocu_mg_relax_fused_prolong_ok(&uf, &uc, &cbc)) {
+      _u_pro[fine_level] = true;  // the next fused relaxation of the fine level adds it
+      return;
+    }
+  }
+  run_prolong(coarse_level, fine_level);
+}
+
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::run_prolong(int coarse_level, int fine_level) {
   ocu_grid3d uc = get_u(coarse_level).desc(), uf = get_u(fine_level).desc();
   void *s = ThreadManager::get_compute_stream();
   if (_fused_ok && uf.nx == 2 * uc.nx && uf.ny == 2 * uc.ny && uf.nz == 2 * uc.nz) {
@@ -625,7 +672,7 @@ bool Sol_MultigridPressure3DDevice<T>::initialize_storage(int nx_val, int ny_val
            nx_val, ny_val, nz_val);
     return false;
   }
-  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = false;
+  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = _u_pro[l] = false;
   _update_bc_between_colors = bc.xpos.check_type(BC_PERIODIC) || bc.xneg.check_type(BC_PERIODIC) ||
                               bc.ypos.check_type(BC_PERIODIC) || bc.yneg.check_type(BC_PERIODIC) ||
                               bc.zpos.check_type(BC_PERIODIC) || bc.zneg.check_type(BC_PERIODIC);

@@ -140,6 +140,17 @@ def mg_relax_fused_bc(u_in, u_out, b, params, bc, hx, hy, hz, first_color=0, fla
                       "mg_relax_fused_bc")
 
 
+def mg_relax_fused_prolong_bc(u_in, u_out, b, uc, params, bc, hx, hy, hz, first_color=0, flags=0, quirk=1):
+    return _lib.check(_lib.hip().ocu_mg_relax_fused_prolong_bc(u_in.ptr(), u_out.ptr(), b.ptr(), uc.ptr(), quirk,
+                                                               C.byref(params), C.byref(bc), hx, hy, hz,
+                                                               first_color, flags, None),
+                      "mg_relax_fused_prolong_bc")
+
+
+def mg_relax_fused_prolong_ok(u, uc, bc):
+    return bool(_lib.hip().ocu_mg_relax_fused_prolong_ok(u.ptr(), uc.ptr(), C.byref(bc)))
+
+
 def mg_residual(u, b, r, h, fx, fy, fz):
     _lib.check(_lib.hip().ocu_mg_residual(u.ptr(), b.ptr(), r.ptr(), h, fx, fy, fz, None), "mg_residual")
 

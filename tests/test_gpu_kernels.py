@@ -351,6 +351,51 @@ def test_prolong(gpu, dtype, quirk, cn):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("cn", [(4, 3, 128), (5, 4, 100), (3, 5, 70), (2, 3, 96)])
+@pytest.mark.parametrize("case", ["neumann0", "neumann_v", "dirichlet"])
+@pytest.mark.parametrize("first,flags", [(0, T.FUSED_WRITE_GHOSTS), (1, T.FUSED_WRITE_GHOSTS), (0, 0)])
+def test_relax_fused_prolong_equals_prolong_then_sweep(gpu, dtype, cn, case, first, flags):
+    """the prolongation-fused sweep against ocu_mg_prolong_bc followed by the
+    fused sweep: u_out bit for bit over the whole buffer (random values in
+    every ghost and padding cell of both inputs), u_in untouched"""
+    bc = BC_CASES[case]
+    hf = (.1, .2, .05)
+    fine = tuple(2 * s for s in cn)
+    uc = random_grid(*cn, dtype, 181)
+    uf = random_grid(*fine, dtype, 182)
+    b = random_grid(*fine, dtype, 183)
+    out0 = random_grid(*fine, dtype, 184)
+    p = O.level_params(bc, 0, *hf)
+    g = {k: D().DeviceGrid.from_host(v) for k, v in (("uc", uc), ("uf", uf), ("b", b), ("o1", out0), ("o2", out0))}
+    assert D().mg_relax_fused_prolong_ok(g["uf"], g["uc"], bc)
+    D().mg_relax_fused_prolong_bc(g["uf"], g["o1"], g["b"], g["uc"], p, bc, *hf, first, flags)
+    assert_bitexact(g["uf"].to_host(), uf, "u_in untouched")
+    D().mg_prolong_bc(g["uc"], g["uf"], bc, *hf)
+    D().mg_relax_fused_bc(g["uf"], g["o2"], g["b"], p, bc, *hf, first, flags)
+    assert_bitexact(g["o1"].to_host(), g["o2"].to_host(), f"fused prolong {case}")
+
+
+def test_relax_fused_prolong_rejects(gpu):
+    from opencurrent_amd._lib import OcuError
+    bc = BC_CASES["neumann0"]
+    p = O.level_params(bc, 0, .1, .1, .1)
+    for fz in (32, 128):  # one or two cells per lane: the separate prolongation runs instead
+        uc, uf, b, o = (D().DeviceGrid(*s) for s in ((4, 4, fz // 2), (8, 8, fz), (8, 8, fz), (8, 8, fz)))
+        assert not D().mg_relax_fused_prolong_ok(uf, uc, bc)
+        with pytest.raises(OcuError):
+            D().mg_relax_fused_prolong_bc(uf, o, b, uc, p, bc, .1, .1, .1)
+    uc, uf, b, o = (D().DeviceGrid(*s) for s in ((4, 4, 128), (8, 8, 256), (8, 8, 256), (8, 8, 256)))
+    assert D().mg_relax_fused_prolong_ok(uf, uc, bc)
+    assert not D().mg_relax_fused_prolong_ok(uf, uc, BC_CASES["mixed"])  # periodic faces
+    uc2 = D().DeviceGrid(4, 5, 128)
+    assert not D().mg_relax_fused_prolong_ok(uf, uc2, bc)
+    with pytest.raises(OcuError):
+        D().mg_relax_fused_prolong_bc(uf, o, b, uc, p, bc, .1, .1, .1, 0, T.FUSED_ZERO_INPUT)
+    with pytest.raises(OcuError):
+        D().mg_relax_fused_prolong_bc(uf, uf, b, uc, p, bc, .1, .1, .1)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("case", ["neumann0", "dirichlet", "mixed", "neumann_v"])
 @pytest.mark.parametrize("order", [0, 1, 2])
 def test_coarse_relax_reference_semantics(gpu, dtype, case, order):
