@@ -32,11 +32,20 @@ void *ocu_get_compute_stream(void);
  *     the sweep, deferred zero clears (Neumann/Dirichlet faces);
  *   OCU_MG_LDS_BOTTOM: the coarse levels of each V-cycle as one launch of a
  *     single LDS-resident workgroup;
- *   (these two give results bit-identical to the plain per-level path)
+ *   OCU_MG_FUSED_PROLONG (with OCU_MG_FUSED_RELAX): a prolongation into a
+ *     level with rows of 129..256 cells is added by the fused sweep that
+ *     follows it instead of a launch of its own;
+ *   (these three give results bit-identical to the plain per-level path)
  *   OCU_MG_ADAPTIVE_IR (mixed solver): after the first refinement iteration,
  *     as many V-cycles (at most `cycles`) as the observed contraction per
  *     cycle needs to reach half the tolerance. */
-enum ocu_mg_options { OCU_MG_FUSED_RELAX = 1, OCU_MG_LDS_BOTTOM = 2, OCU_MG_ADAPTIVE_IR = 4, OCU_MG_DEFAULT = 7 };
+enum ocu_mg_options {
+  OCU_MG_FUSED_RELAX = 1,
+  OCU_MG_LDS_BOTTOM = 2,
+  OCU_MG_ADAPTIVE_IR = 4,
+  OCU_MG_FUSED_PROLONG = 8,
+  OCU_MG_DEFAULT = 15
+};
 
 /* ---- Sol_MultigridPressure3DDevice<T> (ocuequation/sol_mgpressure3d.h:36-181) ----
  * T follows rhs->dtype.  initialize_storage(nx,ny,nz,hx,hy,hz,&rhs) with the

@@ -902,8 +902,16 @@ __global__ __launch_bounds__(64 * NW) void k_prolong_rows(DGrid<T> uc, DGrid<T> 
     load(r, ic, c[r]);
     load(r, ic + 1, p[r]);
   }
+  // the fine row of the next plane is read one iteration ahead (other
+  // planes than the one being written), so its latency overlaps the work
+  auto load_f = [&](int I) {
+    return *reinterpret_cast<const VT *>(uf.p + (int64_t)I * uf.xs + (int64_t)J * uf.ys + (active ? k0 : 0));
+  };
+  VT fnext = load_f(I0);
   for (int I = I0; I < I1; ++I) {
     const int dx = I & 1;
+    VT f = fnext;
+    if (I + 1 < I1) fnext = load_f(I + 1);
     if (I > I0 && !dx) {  // entering coarse plane ic+1
       ++ic;
 #pragma unroll
@@ -928,7 +936,6 @@ __global__ __launch_bounds__(64 * NW) void k_prolong_rows(DGrid<T> uc, DGrid<T> 
     T qzh = 0;
     if (qrow) qzh = from_upper(p[1].v.v[0], p[1].g);
     T *frow = uf.p + (int64_t)I * uf.xs + (int64_t)J * uf.ys;
-    VT f = *reinterpret_cast<const VT *>(frow + (active ? k0 : 0));
 #pragma unroll
     for (int e = 0; e < V; e++) {
       const int mm = e >> 1, dz = e & 1;

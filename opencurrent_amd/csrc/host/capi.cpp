@@ -136,6 +136,7 @@ void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, cons
     h->d->disable_relax_on_host = cfg->disable_relax_on_host != 0;
     h->d->use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
     h->d->use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
+    h->d->use_fused_prolong = (options & OCU_MG_FUSED_PROLONG) != 0;
     ok = h->rhs_d.init_external(*rhs) &&
          h->d->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_d);
   } else {
@@ -144,6 +145,7 @@ void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, cons
     h->f->disable_relax_on_host = cfg->disable_relax_on_host != 0;
     h->f->use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
     h->f->use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
+    h->f->use_fused_prolong = (options & OCU_MG_FUSED_PROLONG) != 0;
     ok = h->rhs_f.init_external(*rhs) &&
          h->f->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_f);
   }
@@ -207,6 +209,7 @@ void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz,
   h->s.ir_fmg_first = fmg_first != 0;
   h->s.use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
   h->s.use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
+  h->s.use_fused_prolong = (options & OCU_MG_FUSED_PROLONG) != 0;
   h->s.ir_adaptive = (options & OCU_MG_ADAPTIVE_IR) != 0;
   if (!h->rhs.init_external(*rhs) || !h->s.initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs)) {
     delete h;

@@ -52,6 +52,7 @@ class Sol_MultigridPressureMixed3DDeviceD : public Sol_MultigridPressure3DBase {
   // options of the FP32 hierarchy (Sol_MultigridPressure3DDevice fields)
   bool use_fused_relax;
   bool use_bottom;
+  bool use_fused_prolong;  // passed to the FP32 solver (see Sol_MultigridPressure3DDevice)
   // statistics of the last solve
   int ir_iterations;
   int total_vcycles;

@@ -25,6 +25,7 @@ Sol_MultigridPressureMixed3DDeviceD::Sol_MultigridPressureMixed3DDeviceD() {
   ir_adaptive = true;
   use_fused_relax = true;
   use_bottom = true;
+  use_fused_prolong = true;
   ir_iterations = 0;
   total_vcycles = 0;
   rhs_l2 = 0;
@@ -65,6 +66,7 @@ bool Sol_MultigridPressureMixed3DDeviceD::initialize_storage(int nx_val, int ny_
   _mg_f._coarse_mode = 1;
   _mg_f.use_fused_relax = use_fused_relax;
   _mg_f.use_bottom = use_bottom;
+  _mg_f.use_fused_prolong = use_fused_prolong;
   if (!_mg_f.initialize_storage(nx_val, ny_val, nz_val, hx_val, hy_val, hz_val, &_r32)) {
     printf("[ERROR] Sol_MultigridPressureMixed3DDeviceD::initialize_storage - failed to initialize float solver\n");
     return false;

@@ -17,11 +17,12 @@ from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, OCU_F32, OCU_F
 from .device import desc_to_host, host_to_desc
 
 
-MG_FUSED_RELAX, MG_LDS_BOTTOM, MG_ADAPTIVE_IR = 1, 2, 4
+MG_FUSED_RELAX, MG_LDS_BOTTOM, MG_ADAPTIVE_IR, MG_FUSED_PROLONG = 1, 2, 4, 8
 
 
-def _options(use_fused_relax, use_bottom):
-    return (MG_FUSED_RELAX if use_fused_relax else 0) | (MG_LDS_BOTTOM if use_bottom else 0)
+def _options(use_fused_relax, use_bottom, use_fused_prolong=True):
+    return ((MG_FUSED_RELAX if use_fused_relax else 0) | (MG_LDS_BOTTOM if use_bottom else 0) |
+            (MG_FUSED_PROLONG if use_fused_prolong else 0))
 
 
 def _history(fn, h, extra=()):
@@ -35,11 +36,11 @@ class MultigridPressure3D:
     """Sol_MultigridPressure3DDevice<T> (ocuequation/sol_mgpressure3d.h:130-181)."""
 
     def __init__(self, rhs, hx, hy, hz, bc, nu1=2, nu2=2, convergence=CONVERGENCE_L2, sym=False,
-                 disable_relax_on_host=False, use_fused_relax=True, use_bottom=True):
+                 disable_relax_on_host=False, use_fused_relax=True, use_bottom=True, use_fused_prolong=True):
         self._rhs = rhs  # keep alive
         self.cfg = MGConfig.make(bc, nu1, nu2, convergence, sym, disable_relax_on_host)
         self.h = _lib.host().ocu_mg_create(rhs.ptr(), hx, hy, hz, C.byref(self.cfg),
-                                           _options(use_fused_relax, use_bottom))
+                                           _options(use_fused_relax, use_bottom, use_fused_prolong))
         if not self.h:
             raise _lib.OcuError(f"Sol_MultigridPressure3DDevice::initialize_storage failed: {_lib.last_error()}")
 
@@ -81,11 +82,13 @@ class MultigridPressureMixed3D:
     """Sol_MultigridPressureMixed3DDeviceD: FP32 V-cycles inside FP64 iterative refinement."""
 
     def __init__(self, rhs, hx, hy, hz, bc, nu1=2, nu2=2, convergence=CONVERGENCE_L2, cycles=4,
-                 cycles_first=4, fmg_first=False, use_fused_relax=True, use_bottom=True, adaptive=True):
+                 cycles_first=4, fmg_first=False, use_fused_relax=True, use_bottom=True, adaptive=True,
+                 use_fused_prolong=True):
         self._rhs = rhs
         self.cfg = MGConfig.make(bc, nu1, nu2, convergence)
         self.h = _lib.host().ocu_mgmixed_create(rhs.ptr(), hx, hy, hz, C.byref(self.cfg), cycles, cycles_first,
-                                                int(fmg_first), _options(use_fused_relax, use_bottom) |
+                                                int(fmg_first),
+                                                _options(use_fused_relax, use_bottom, use_fused_prolong) |
                                                 (MG_ADAPTIVE_IR if adaptive else 0))
         if not self.h:
             raise _lib.OcuError(f"Sol_MultigridPressureMixed3DDeviceD::initialize_storage failed: {_lib.last_error()}")
