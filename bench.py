@@ -35,6 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # same counters read for a plain copy of known size; recorded with the
 # profile summary profiles/level0_kernels_256_hbm_traffic.md
 SWEEP_TRAFFIC_256 = 241_400_000
+SWEEP_PRO_TRAFFIC_256 = 254_400_000
 
 
 def parse():
@@ -111,7 +112,22 @@ def time_fused_sweep(n, reps, dtype=np.float32):
     ms = ev.stop_ms() / reps
     w = np.dtype(dtype).itemsize
     alg_bytes = 3 * w * n ** 3  # read u and b, write u, once per cell
-    return ms, alg_bytes
+    # the same sweep adding the prolongation of the (n/2)^3 level, as the
+    # solver runs it after each coarse correction
+    pro = None
+    uc = DeviceGrid(n // 2, n // 2, n // 2, dtype=dtype)
+    uc.tensor.uniform_(-1, 1)
+    if hip.ocu_mg_relax_fused_prolong_ok(u.ptr(), uc.ptr(), C.byref(bc)):
+        pargs = (u.ptr(), o.ptr(), b.ptr(), uc.ptr(), 1, C.byref(p), C.byref(bc), 1.0 / n, 1.0 / n, 1.0 / n, 0,
+                 T.FUSED_WRITE_GHOSTS, stream)
+        for _ in range(5):
+            _lib.check(hip.ocu_mg_relax_fused_prolong_bc(*pargs), "fused prolongation sweep")
+        ev.start()
+        for _ in range(reps):
+            hip.ocu_mg_relax_fused_prolong_bc(*pargs)
+        pms = ev.stop_ms() / reps
+        pro = (pms, alg_bytes + w * (n // 2) ** 3)  # + the coarse grid, read once
+    return ms, alg_bytes, pro
 
 
 def time_fp64_mg_64(hip, stream):
@@ -478,7 +494,7 @@ def main():
         del mg
 
     # ---- roofline of the dominant kernel
-    kms, kbytes = time_fused_sweep(n, args.kernel_reps)
+    kms, kbytes, pro = time_fused_sweep(n, args.kernel_reps)
     achieved = kbytes / (kms / 1e3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -488,6 +504,15 @@ def main():
                        if n == 256 else None,
                        "kernel": "k_relax_fused_rows<float,4,1,12> (level-0 red+black sweep with ghost writes)",
                        "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes}
+    if pro:
+        # the level-0 sweeps that follow a coarse correction add its
+        # prolongation while they read u (7 of the 18 level-0 sweeps per solve)
+        pa = pro[1] / (pro[0] / 1e3) / 1e9
+        out["roofline"]["prolongation_sweep"] = {
+            "kernel": "k_relax_fused_rows<float,4,1,12,PRO> (level-0 sweep adding the 128^3 prolongation)",
+            "avg_launch_us": round(pro[0] * 1e3, 2), "alg_bytes_per_launch": pro[1], "achieved": round(pa, 1),
+            "frac": round(pa / HBM_PEAK_GBS, 4),
+            "traffic": SWEEP_PRO_TRAFFIC_256 if n == 256 else None}
 
     if not args.no_extra:
         for key, fn in (("c1_fp64_mg_64", lambda: time_fp64_mg_64(hip, stream)), ("c3_ns_step", time_ns_step),

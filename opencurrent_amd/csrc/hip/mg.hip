@@ -6,6 +6,7 @@
 // keeps the reference's operation order; the library is built with
 // -ffp-contract=off so the results match the CPU restatement bit for bit.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <mutex>
 
@@ -269,60 +270,71 @@ struct CRow {
   T g;
 };
 
-// The prolongation of one fine row (I, J) added to f (V fine cells per lane,
-// V >= 2): the four coarse rows it reads (planes ic and ic +- 1 toward the
-// fine plane, rows jc and jc +- 1 toward the fine row), and for the corner
-// quirk the row of plane ic+1, are loaded here.  The operations and their
-// order are those of k_prolong_rows, so the sum is the same bit for bit.
-template <typename T, int V>
-__device__ __forceinline__ void prolong_row_add(const DGrid<T> &uc, int I, int J, int quirk, int lane, bool active,
-                                                int hi_lane, int nzf, Vec<T, V> &f) {
+// A wave's window of the coarse grid for the prolongation of its fine row J:
+// coarse rows jc = J/2 (row 0) and jc +- 1 toward J (row 1) of the coarse
+// planes ic-1, ic, ic+1 (slots 0, 1, 2): VC cells per lane in v, the row's
+// z ghost in g.  Plain arrays indexed by constants only, so the window stays
+// in registers.
+template <typename T, int VC>
+struct CWin {
+  T v[3][2][VC];
+  T g[3][2];
+};
+
+// The prolongation of fine row (I, J) added to f (V fine cells per lane,
+// V >= 2) from a window centred on ic = I/2: the coarse plane toward the
+// fine plane is slot 2 (odd I) or slot 0 (even I); the corner quirk takes
+// the row-1 corner of plane ic+1.  The operations and their order are those
+// of k_prolong_rows, so the sum is the same bit for bit.
+template <typename T, int V, int DX>
+__device__ __forceinline__ void prolong_row_add_dx(int J, int quirk, const CWin<T, V / 2> &w, Vec<T, V> &f) {
   constexpr int VC = V / 2;
-  typedef CRow<T, VC> CR;
-  const int kc = active ? lane * VC : 0;
-  const int gk = lane == 0 ? -1 : uc.nz;
-  const bool g_in_data = lane == hi_lane && nzf < 64 * V;
-  const int jc = J >> 1, dy = J & 1, ic = I >> 1, dx = I & 1;
-  const int jc2 = jc + (dy ? 1 : -1), icx = ic + (dx ? 1 : -1);
-  auto ld = [&](int i, int j, CR &d) {
-    const T *p = uc.p + (int64_t)i * uc.xs + (int64_t)j * uc.ys;
-    d.v = *reinterpret_cast<const Vec<T, VC> *>(p + kc);
-    d.g = p[gk];
-    if (g_in_data) d.v.v[0] = d.g;
-  };
-  CR c[2], X[2], q1;
-  ld(ic, jc, c[0]);
-  ld(ic, jc2, c[1]);
-  ld(icx, jc, X[0]);
-  ld(icx, jc2, X[1]);
-  const bool qrow = quirk && dx == 0 && dy == 1;  // its dz = 1 children take the corner from plane ic+1
-  T qzh = 0;
-  if (qrow) {
-    ld(ic + 1, jc2, q1);
-    qzh = from_upper(q1.v.v[0], q1.g);
-  }
-  T zl[2][2], zh[2][2];  // [plane: 0 = c, 1 = X][row]
+  constexpr int dx = DX, xs = DX ? 2 : 0;  // the slot toward the fine plane, a constant
+  const int dy = J & 1;
+  T c[2][VC], x[2][VC], cg[2], xg[2];
 #pragma unroll
   for (int r = 0; r < 2; r++) {
-    zl[0][r] = from_lower(c[r].v.v[VC - 1], c[r].g);
-    zh[0][r] = from_upper(c[r].v.v[0], c[r].g);
-    zl[1][r] = from_lower(X[r].v.v[VC - 1], X[r].g);
-    zh[1][r] = from_upper(X[r].v.v[0], X[r].g);
+#pragma unroll
+    for (int k = 0; k < VC; k++) {
+      c[r][k] = w.v[1][r][k];
+      x[r][k] = w.v[xs][r][k];
+    }
+    cg[r] = w.g[1][r];
+    xg[r] = w.g[xs][r];
+  }
+  const bool qrow = quirk && dx == 0 && dy == 1;  // its dz = 1 children take the corner from plane ic+1
+  const T qzh = from_upper(w.v[2][1][0], w.g[2][1]);
+  T zl[2][2], zh[2][2];  // [plane: 0 = c, 1 = x][row]
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    zl[0][r] = from_lower(c[r][VC - 1], cg[r]);
+    zh[0][r] = from_upper(c[r][0], cg[r]);
+    zl[1][r] = from_lower(x[r][VC - 1], xg[r]);
+    zh[1][r] = from_upper(x[r][0], xg[r]);
   }
 #pragma unroll
   for (int e = 0; e < V; e++) {
     const int mm = e >> 1, dz = e & 1;
-    auto zs = [&](const CR &row, T lo, T hi) {
-      return dz ? (mm + 1 < VC ? row.v.v[mm + 1] : hi) : (mm > 0 ? row.v.v[mm - 1] : lo);
+    // coarse value toward the child in z of a row (its ghost-side values lo, hi)
+    auto zs = [&](const T (&a)[VC], T lo, T hi) {
+      return dz ? (mm + 1 < VC ? a[mm + 1] : hi) : (mm > 0 ? a[mm - 1] : lo);
     };
-    const T C = c[0].v.v[mm], Xv = X[0].v.v[mm], Y = c[1].v.v[mm];
+    const T C = c[0][mm], Xv = x[0][mm], Y = c[1][mm];
     const T Z = zs(c[0], zl[0][0], zh[0][0]);
-    const T XY = X[1].v.v[mm];
-    const T XZ = zs(X[0], zl[1][0], zh[1][0]);
+    const T XY = x[1][mm];
+    const T XZ = zs(x[0], zl[1][0], zh[1][0]);
     const T YZ = zs(c[1], zl[0][1], zh[0][1]);
-    const T XYZ = (qrow && dz) ? (mm + 1 < VC ? q1.v.v[mm + 1] : qzh) : zs(X[1], zl[1][1], zh[1][1]);
+    const T XYZ = (qrow && dz) ? (mm + 1 < VC ? w.v[2][1][mm + 1] : qzh) : zs(x[1], zl[1][1], zh[1][1]);
     f.v[e] = f.v[e] + PW<T>::c0 * C + PW<T>::c1 * (Xv + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
   }
+}
+
+// (a branch on the wave-uniform plane parity keeps every window index a
+// constant, so the window is never addressed and stays in registers)
+template <typename T, int V>
+__device__ __forceinline__ void prolong_row_add(int I, int J, int quirk, const CWin<T, V / 2> &w, Vec<T, V> &f) {
+  if (I & 1) prolong_row_add_dx<T, V, 1>(J, quirk, w, f);
+  else prolong_row_add_dx<T, V, 0>(J, quirk, w, f);
 }
 
 // ------------------------------------------------------------------------
@@ -406,8 +418,14 @@ __device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, i
 // the face transforms of the prolonged boundary cells; u_in is not written.
 // wghost: also write the level-1 ghost layer of u_out from each boundary
 // cell.  ZERO, PRO and wghost need Neumann/Dirichlet faces.
+// FP32 sweeps keep to 80 VGPRs (six waves per SIMD: two 12-wave blocks per
+// CU, each hiding the other's barriers); the prolongation variant would
+// otherwise take 81
+template <typename T>
+constexpr int fused_rows_min_waves() { return sizeof(T) == 4 ? 6 : 1; }
+
 template <typename T, int V, int R, int NW, bool ZERO, bool PRO>
-__global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
+__global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
                                                                typename DevCFor<T>::type c, int first_color,
                                                                int tiles_j, int ichunk, int ntiles,
                                                                GhostL1<T> gw, int wghost, DGrid<T> uc, int quirk) {
@@ -448,26 +466,103 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
   auto ring1 = [&](int r) { return t0 + r >= 1 && t0 + r <= NR - 2; };
   auto tile = [&](int r) { return t0 + r >= 2 && t0 + r <= NR - 3; };
 
+  // PRO: the row as the plain load would clamp it (jj, pp), its interior
+  // source row (jr, pq) and the ghost offset between them.  The coarse rows
+  // come from a per-row window of three coarse planes that slides with the
+  // march (one new plane every two fine planes), loaded before the barrier
+  // and summed after it in pro_finish.
+  constexpr int VC = PRO ? V / 2 : 1;
+  [[maybe_unused]] CWin<T, VC> win[R];
+  [[maybe_unused]] int wic = INT_MIN;  // coarse plane at the window's centre
+  auto pro_rows = [&](int r, int p, int &jj, int &pp, int &jr, int &pq) {
+    jj = min(max(jb + r, -1), ny);
+    pp = min(max(p, -1), nx);
+    jr = min(max(jj, 0), ny - 1);
+    pq = min(max(pp, 0), nx - 1);
+  };
+  // row q (0: jc, 1: jc +- 1) of coarse plane ic into slot `slot` of row r's window
+  auto cload1 = [&](int r, int ic, int q, int slot) {
+    if constexpr (PRO) {
+      int jj, pp, jr, pq;
+      pro_rows(r, 0, jj, pp, jr, pq);
+      const int jc = jr >> 1, jc2 = jc + ((jr & 1) ? 1 : -1);
+      const int kcc = active ? lane * VC : 0, gkc = lane == 0 ? -1 : uc.nz;
+      const bool gin = lane == hi_lane && nz < 64 * V;
+      const T *row = uc.p + (int64_t)ic * uc.xs + (int64_t)(q ? jc2 : jc) * uc.ys;
+      const Vec<T, VC> d = *reinterpret_cast<const Vec<T, VC> *>(row + kcc);
+      const T g = row[gkc];
+#pragma unroll
+      for (int k = 0; k < VC; k++) win[r].v[slot][q][k] = (k == 0 && gin) ? g : d.v[k];
+      win[r].g[slot][q] = g;
+    }
+  };
+  // centre the window on coarse plane ic (ic never decreases along the march)
+  auto wcentre = [&](int ic) {
+    if constexpr (PRO) {
+      if (ic == wic) return;
+      if (ic == wic + 1) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+#pragma unroll
+            for (int k = 0; k < VC; k++) {
+              win[r].v[0][q][k] = win[r].v[1][q][k];
+              win[r].v[1][q][k] = win[r].v[2][q][k];
+            }
+            win[r].g[0][q] = win[r].g[1][q];
+            win[r].g[1][q] = win[r].g[2][q];
+            cload1(r, ic + 1, q, 2);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+            cload1(r, ic - 1, q, 0);
+            cload1(r, ic, q, 1);
+            cload1(r, ic + 1, q, 2);
+          }
+        }
+      }
+      wic = ic;
+    }
+  };
+  auto pro_load = [&](int r, int p, VT &f) {
+    if constexpr (PRO) {
+      int jj, pp, jr, pq;
+      pro_rows(r, p, jj, pp, jr, pq);
+      f = *reinterpret_cast<const VT *>(uin.p + (int64_t)pq * uin.xs + (int64_t)jr * uin.ys + kc);
+    }
+  };
+  auto pro_finish = [&](int r, int p, VT f, VT &d, T &g) {
+    if constexpr (PRO) {
+      int jj, pp, jr, pq;
+      pro_rows(r, p, jj, pp, jr, pq);
+      prolong_row_add<T, V>(pq, jr, quirk, win[r], f);
+      if (pp != pq || jj != jr) {
+#pragma unroll
+        for (int e = 0; e < V; e++) f.v[e] = gw.chain(pp - pq, jj - jr, 0, f.v[e]);
+      }
+      const T top = (nz == 64 * V) ? f.v[V - 1] : from_lower(f.v[V - 1], (T)0);
+      g = lane == 0 ? gw.xf(5, f.v[0]) : gw.xf(4, top);
+      d = f;
+      if (g_in_data) d.v[0] = g;
+    }
+  };
+  // the coarse plane whose window a fine plane p needs
+  auto pro_ic = [&](int p) { return min(max(min(max(p, -1), nx), 0), nx - 1) >> 1; };
   auto load = [&](int r, int p, VT &d, T &g) {
     if constexpr (ZERO) {  // homogeneous faces: every ghost of the zero grid is +0.0
 #pragma unroll
       for (int e = 0; e < V; e++) d.v[e] = (T)0;
       g = (T)0;
     } else if constexpr (PRO) {
-      // the row as the plain load would clamp it, its interior source row
-      // and the ghost offset between them
-      const int jj = min(max(jb + r, -1), ny), pp = min(max(p, -1), nx);
-      const int jr = min(max(jj, 0), ny - 1), pr = min(max(pp, 0), nx - 1);
-      VT f = *reinterpret_cast<const VT *>(uin.p + (int64_t)pr * uin.xs + (int64_t)jr * uin.ys + kc);
-      prolong_row_add<T, V>(uc, pr, jr, quirk, lane, active, hi_lane, nz, f);
-      if (pp != pr || jj != jr) {
-#pragma unroll
-        for (int e = 0; e < V; e++) f.v[e] = gw.chain(pp - pr, jj - jr, 0, f.v[e]);
-      }
-      const T top = (nz == 64 * V) ? f.v[V - 1] : from_lower(f.v[V - 1], (T)0);
-      g = lane == 0 ? gw.xf(5, f.v[0]) : gw.xf(4, top);
-      d = f;
-      if (g_in_data) d.v[0] = g;
+      VT f;
+      wcentre(pro_ic(p));
+      pro_load(r, p, f);
+      pro_finish(r, p, f, d, g);
     } else {
       const T *row = urow[r] + (int64_t)min(max(p, -1), nx) * uin.xs;
       d = *reinterpret_cast<const VT *>(row + kc);
@@ -497,13 +592,23 @@ __global__ __launch_bounds__(64 * NW) void k_relax_fused_rows(DGrid<T> uin, DGri
 
   for (int p = i0 - 1; p <= i1; ++p) {
     const bool p_in = p >= 0 && p < nx;
+    // PRO: the rows of plane p+1 (and a new coarse plane every other
+    // iteration) are loaded before the barrier and summed after it, so the
+    // barrier hides their latency as it hides the plain load's
+    [[maybe_unused]] VT nraw[R];
+    if constexpr (PRO) wcentre(pro_ic(p + 1));
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      load(r, p + 1, n[r], gn[r]);
+      if constexpr (PRO) pro_load(r, p + 1, nraw[r]);
+      else load(r, p + 1, n[r], gn[r]);
       bcur[r] = loadb(r, p);
       sU0[t0 + r][lane] = cc[r];
     }
     __syncthreads();
+    if constexpr (PRO) {
+#pragma unroll
+      for (int r = 0; r < R; r++) pro_finish(r, p + 1, nraw[r], n[r], gn[r]);
+    }
 
     // first colour on plane p (rows of ring 1); j-neighbours inside the
     // wave come from registers, across waves from LDS
