@@ -1,6 +1,8 @@
-"""The reference's C++ API as a drop-in: a caller written against the ocu
-headers (tests/cpp/mg_dropin.cpp) compiles and links against libocu.so +
-libocu_hip.so unchanged, and on the GPU solves its Poisson problem."""
+"""The reference's C++ API as a drop-in: callers written against the ocu
+headers (tests/cpp/mg_dropin.cpp, tests/cpp/advection_dropin.cpp) compile
+and link against libocu.so + libocu_hip.so unchanged, and on the GPU solve
+their Poisson problem and pass the reference advection tests' symmetry,
+conservation and band checks."""
 import subprocess
 from pathlib import Path
 
@@ -20,8 +22,9 @@ def build(tmp_path, name):
     return exe
 
 
-def test_dropin_caller_compiles_and_links(tmp_path):
-    assert build(tmp_path, "mg_dropin").exists()
+@pytest.mark.parametrize("name", ["mg_dropin", "advection_dropin"])
+def test_dropin_caller_compiles_and_links(tmp_path, name):
+    assert build(tmp_path, name).exists()
 
 
 @pytest.mark.gpu
@@ -31,3 +34,16 @@ def test_dropin_caller_solves_on_the_gpu(gpu, tmp_path, n):
     p = subprocess.run([str(exe), str(n)], capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "ok=1" in p.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", [1, 2, 4])
+def test_dropin_scalar_advection_matches_reference_checks(gpu, tmp_path, scheme):
+    """advection3ddoubletest.cpp's symmetry test (six flow directions agree
+    to 1e-12 and conserve the sum) and band test (exact zeros outside the
+    band), through Eqn_ScalarAdvection3DD, for the reference's two schemes
+    and second-order upwind"""
+    exe = build(tmp_path, "advection_dropin")
+    p = subprocess.run([str(exe), str(scheme)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "ok=0" not in p.stdout
