@@ -1277,9 +1277,16 @@ struct BotLevel {
   GhostL1<T> gw;
 };
 
+// periodic axes of the bottom (x, y, z): a ghost there is a copy of the
+// interior cell at the other end instead of a face transform
+struct BotPer {
+  int a[3];
+};
+
 template <typename T, typename TC>
 struct BotArgs {
   BotLevel<T> L[BOT_MAX];
+  BotPer per;
   int nlev, nu1, nu2, zero_top;
   HostC<TC> hc;     // coarsest level: relax_on_host arithmetic in TC
   GhostL1<TC> gwc;  // and its ghost transforms in TC
@@ -1291,23 +1298,44 @@ static_assert(sizeof(BotArgs<double, double>) <= 4096 && sizeof(BotArgs<float, d
 
 __device__ __forceinline__ int box_cells(const BotGeom &g) { return (g.nx + 2) * (g.ny + 2) * (g.nz + 2); }
 
-// face ghosts of the cell at LDS index id = (i,j,k) with value v
+// face ghosts of the cell at LDS index id = (i,j,k) with value v: beside it
+// across a Neumann/Dirichlet face, or at the other end of a periodic axis
+// (with an even number of cells there, the two ends are of different
+// colours, so no thread of the colour being relaxed reads that ghost)
 template <typename T>
-__device__ __forceinline__ void face_ghosts(T *u, const BotGeom &g, const GhostL1<T> &gw, int id, int i, int j,
-                                            int k, T v) {
-  if (i == 0) u[id - g.lxs] = gw.xf(1, v);
-  if (i == g.nx - 1) u[id + g.lxs] = gw.xf(0, v);
-  if (j == 0) u[id - g.lys] = gw.xf(3, v);
-  if (j == g.ny - 1) u[id + g.lys] = gw.xf(2, v);
-  if (k == 0) u[id - 1] = gw.xf(5, v);
-  if (k == g.nz - 1) u[id + 1] = gw.xf(4, v);
+__device__ __forceinline__ void face_ghosts(T *u, const BotGeom &g, const GhostL1<T> &gw, const BotPer &per, int id,
+                                            int i, int j, int k, T v) {
+  if (i == 0) {
+    if (per.a[0]) u[id + g.nx * g.lxs] = v;
+    else u[id - g.lxs] = gw.xf(1, v);
+  }
+  if (i == g.nx - 1) {
+    if (per.a[0]) u[id - g.nx * g.lxs] = v;
+    else u[id + g.lxs] = gw.xf(0, v);
+  }
+  if (j == 0) {
+    if (per.a[1]) u[id + g.ny * g.lys] = v;
+    else u[id - g.lys] = gw.xf(3, v);
+  }
+  if (j == g.ny - 1) {
+    if (per.a[1]) u[id - g.ny * g.lys] = v;
+    else u[id + g.lys] = gw.xf(2, v);
+  }
+  if (k == 0) {
+    if (per.a[2]) u[id + g.nz] = v;
+    else u[id - 1] = gw.xf(5, v);
+  }
+  if (k == g.nz - 1) {
+    if (per.a[2]) u[id - g.nz] = v;
+    else u[id + 1] = gw.xf(4, v);
+  }
 }
 
 // one colour with the device arithmetic (k < 2*(nz/2)), threads over the
 // colour's cells only
 template <typename T>
 __device__ void bot_relax_dev(T *u, const T *b, const BotGeom &g, const typename DevCFor<T>::type &c,
-                              const GhostL1<T> &gw, int color) {
+                              const GhostL1<T> &gw, const BotPer &per, int color) {
   const int h = (int)g.mhd.n, n = g.nx * g.ny * h;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const int row = mdiv(t, g.mhd), i = mdiv(row, g.my), j = row - i * g.ny;
@@ -1317,14 +1345,14 @@ __device__ void bot_relax_dev(T *u, const T *b, const BotGeom &g, const typename
     const T v = relax_pt(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs], u[id + g.lxs],
                          b[id], d);
     u[id] = v;
-    face_ghosts(u, g, gw, id, i, j, k, v);
+    face_ghosts(u, g, gw, per, id, i, j, k, v);
   }
 }
 
 // one colour with the relax_on_host arithmetic (every k)
 template <typename T>
 __device__ void bot_relax_host(T *u, const T *b, const BotGeom &g, const HostC<T> &c, const GhostL1<T> &gw,
-                               int color) {
+                               const BotPer &per, int color) {
   const int h = (int)g.mhh.n, n = g.nx * g.ny * h;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const int row = mdiv(t, g.mhh), i = mdiv(row, g.my), j = row - i * g.ny;
@@ -1337,13 +1365,16 @@ __device__ void bot_relax_host(T *u, const T *b, const BotGeom &g, const HostC<T
     const T v = relax_pt_host(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs],
                               u[id + g.lxs], b[id], xm_, ym_, zm_);
     u[id] = v;
-    face_ghosts(u, g, gw, id, i, j, k, v);
+    face_ghosts(u, g, gw, per, id, i, j, k, v);
   }
 }
 
-// the whole level-1 ghost layer (faces, edges, corners) from the interior
+// the whole level-1 ghost layer (faces, edges, corners) from the interior:
+// out-of-range indices clamp to the adjacent cell and take that face's
+// transform, x then y then z, or on a periodic axis wrap to the other end
+// untransformed (the reference fill's x, y, z passes)
 template <typename T>
-__device__ void bot_fill(T *u, const BotGeom &g, const GhostL1<T> &gw) {
+__device__ void bot_fill(T *u, const BotGeom &g, const GhostL1<T> &gw, const BotPer &per) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int nZ = (nx + 2) * (ny + 2), nY = (nx + 2) * nz, nX = ny * nz;
   for (int t = threadIdx.x; t < 2 * (nZ + nY + nX); t += blockDim.x) {
@@ -1371,20 +1402,23 @@ __device__ void bot_fill(T *u, const BotGeom &g, const GhostL1<T> &gw) {
       i = side ? nx : -1;
     }
     const int sx = i < 0 ? -1 : i >= nx ? 1 : 0, sy = j < 0 ? -1 : j >= ny ? 1 : 0, sz = k < 0 ? -1 : k >= nz ? 1 : 0;
-    const int src = g.lshift + (i - sx) * g.lxs + (j - sy) * g.lys + (k - sz);
-    u[g.lshift + i * g.lxs + j * g.lys + k] = gw.chain(sx, sy, sz, u[src]);
+    const int si = per.a[0] ? i - sx * nx : i - sx, sj = per.a[1] ? j - sy * ny : j - sy,
+              sk = per.a[2] ? k - sz * nz : k - sz;
+    const int src = g.lshift + si * g.lxs + sj * g.lys + sk;
+    u[g.lshift + i * g.lxs + j * g.lys + k] =
+        gw.chain(per.a[0] ? 0 : sx, per.a[1] ? 0 : sy, per.a[2] ? 0 : sz, u[src]);
   }
 }
 
 template <typename T>
-__device__ void bot_relax(T *u, const T *b, const BotLevel<T> &L, int iters) {
+__device__ void bot_relax(T *u, const T *b, const BotLevel<T> &L, const BotPer &per, int iters) {
   for (int it = 0; it < iters; it++) {
-    bot_relax_dev(u, b, L.g, L.dc, L.gw, 0);
+    bot_relax_dev(u, b, L.g, L.dc, L.gw, per, 0);
     __syncthreads();
-    bot_relax_dev(u, b, L.g, L.dc, L.gw, 1);
+    bot_relax_dev(u, b, L.g, L.dc, L.gw, per, 1);
     __syncthreads();
   }
-  bot_fill(u, L.g, L.gw);
+  bot_fill(u, L.g, L.gw, per);
   __syncthreads();
 }
 
@@ -1465,7 +1499,7 @@ __global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> A) {
 
   // down: relax, zero start below, residual -> restriction
   for (int l = 0; l < C; l++) {
-    bot_relax(U(l), B(l), A.L[l], A.nu1);
+    bot_relax(U(l), B(l), A.L[l], A.per, A.nu1);
     bot_zero(U(l + 1), 2 * box_cells(A.L[l + 1].g));  // u and b boxes (b ghosts stay zero)
     __syncthreads();
     bot_res_restrict(U(l), B(l), A.L[l].g, A.L[l].rc, B(l + 1), A.L[l + 1].g);
@@ -1489,10 +1523,10 @@ __global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> A) {
     }
     for (int it = 0; it < A.coarse_iters; it++)
       for (int rb = 0; rb < 2; rb++) {
-        bot_relax_host(uc, bc, g, A.hc, A.gwc, rb);
+        bot_relax_host(uc, bc, g, A.hc, A.gwc, A.per, rb);
         __syncthreads();
       }
-    bot_fill(uc, g, A.gwc);
+    bot_fill(uc, g, A.gwc, A.per);
     __syncthreads();
     if (MODE == 1) {
       for (int t = threadIdx.x; t < box; t += blockDim.x) U(C)[t] = (T)uc[t];
@@ -1504,9 +1538,9 @@ __global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> A) {
   for (int l = C - 1; l >= 0; l--) {
     bot_prolong(U(l + 1), A.L[l + 1].g, U(l), A.L[l].g);
     __syncthreads();
-    bot_fill(U(l), A.L[l].g, A.L[l].gw);
+    bot_fill(U(l), A.L[l].g, A.L[l].gw, A.per);
     __syncthreads();
-    bot_relax(U(l), B(l), A.L[l], A.nu2);
+    bot_relax(U(l), B(l), A.L[l], A.per, A.nu2);
   }
 
   // write back every level (u boxes, b interiors below the top)
@@ -1845,6 +1879,9 @@ int launch_bottom(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_p
   A.zero_top = zero_top;
   A.hc = host_c<TC>(&p[nlev - 1]);
   A.gwc = ghost_l1<TC>(bc, h[3 * (nlev - 1)], h[3 * (nlev - 1) + 1], h[3 * (nlev - 1) + 2]);
+  A.per.a[0] = bc->xpos.type == OCU_BC_PERIODIC;
+  A.per.a[1] = bc->ypos.type == OCU_BC_PERIODIC;
+  A.per.a[2] = bc->zpos.type == OCU_BC_PERIODIC;
   A.coarse_iters = coarse_iters;
   k_mg_bottom<T, TC, MODE><<<1, BOT_THREADS, bytes, stream>>>(A);
   OCU_HIP_LAUNCHED("ocu_mg_bottom_vcycle");
@@ -1879,9 +1916,21 @@ int ocu_mg_bottom_vcycle(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_
                       u[l].nx >= 1 && u[l].ny >= 1 && u[l].nz >= 1,
                   "%s: level %d is not a coarsening of level %d", what, l, l - 1);
   }
-  if (!gfaces_local(bc)) {
-    set_error("%s: needs Neumann or Dirichlet faces", what);
+  if (!gfaces_paired(bc)) {
+    set_error("%s: needs Neumann or Dirichlet faces or periodic face pairs", what);
     return OCU_ERR_UNSUPPORTED;
+  }
+  {
+    const int per[3] = {bc->xpos.type == OCU_BC_PERIODIC, bc->ypos.type == OCU_BC_PERIODIC,
+                        bc->zpos.type == OCU_BC_PERIODIC};
+    for (int l = 0; l < nlev; l++) {
+      const int n[3] = {u[l].nx, u[l].ny, u[l].nz};
+      for (int a = 0; a < 3; a++)
+        if (per[a] && n[a] % 2) {
+          set_error("%s: periodic axis %d of level %d has an odd number of cells (%d)", what, a, l, n[a]);
+          return OCU_ERR_UNSUPPORTED;
+        }
+    }
   }
   OCU_REQUIRE(!zero_top || gfaces_homogeneous(bc), "%s: zero start needs homogeneous faces", what);
   const size_t bytes = ocu_mg_bottom_lds_bytes(u, nlev, coarse_mode);

@@ -107,6 +107,9 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   Grid3DDevice<T> _pressure;
   bool _update_bc_between_colors;
   bool _fused_ok;  // every face Neumann or Dirichlet
+  // the LDS bottom can run: faces Neumann, Dirichlet or periodic in pairs,
+  // with an even number of cells along each periodic axis on its levels
+  bool _bottom_ok;
   // 0: the coarsest level is relaxed with the reference's host arithmetic in T;
   // 1: in double on float grids (the coarsest solve of the mixed solver)
   int _coarse_mode;

@@ -296,6 +296,7 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   _r_grid = _u_grid = _b_grid = _u_alt = 0;
   _update_bc_between_colors = false;
   _fused_ok = false;
+  _bottom_ok = false;
   _coarse_mode = 0;
   _norm_ws = 0;
   _norm_host = 0;
@@ -391,7 +392,7 @@ void Sol_MultigridPressure3DDevice<T>::settle() {
 
 template <typename T>
 int Sol_MultigridPressure3DDevice<T>::bottom_start() const {
-  if (!use_bottom || _bottom_level < 1 || !use_fused_relax || !_fused_ok || store_residuals ||
+  if (!use_bottom || _bottom_level < 1 || !use_fused_relax || !_bottom_ok || store_residuals ||
       disable_relax_on_host || make_symmetric_operator)
     return -1;
   return _bottom_level;
@@ -766,6 +767,17 @@ bool Sol_MultigridPressure3DDevice<T>::initialize_storage(int nx_val, int ny_val
     size_t bytes = ocu_mg_bottom_lds_bytes(d, C - top + 1, _coarse_mode);
     if (bytes == 0 || bytes > 65536) break;
     _bottom_level = top;
+  }
+  _bottom_ok = _fused_ok;
+  if (!_fused_ok && _bottom_level >= 1 && relax_bc_ok(0)) {
+    // periodic pairs: every bottom level needs an even count along them
+    _bottom_ok = true;
+    const BoundaryCondition *hi[3] = {&bc.xpos, &bc.ypos, &bc.zpos};
+    for (int l = _bottom_level; l <= C; l++) {
+      const int n[3] = {nx(l), ny(l), nz(l)};
+      for (int a = 0; a < 3; a++)
+        if (hi[a]->type == BC_PERIODIC && n[a] % 2) _bottom_ok = false;
+    }
   }
   return !any_error();
 }
