@@ -132,6 +132,12 @@ int ocu_mg_relax_bc(const ocu_grid3d *u, const ocu_grid3d *b, int color, const o
 int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                        const ocu_mg_level_params *p, const ocu_bc_set *bc, int first_color,
                        void *stream);
+/* 1 when ocu_mg_relax_fused_bc with ghost writes runs for these grids and
+ * faces: every face Neumann or Dirichlet, or periodic x and/or y pairs (even
+ * counts along them, z not periodic, rows of at most 256 cells split evenly
+ * over the lanes, aligned); else 0 */
+int ocu_mg_relax_fused_ok(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                          const ocu_bc_set *bc);
 /* ocu_mg_relax_fused with two options (flags, OR-ed):
  *   OCU_FUSED_WRITE_GHOSTS: also write the first ghost layer of u_out exactly
  *     as apply_3d_boundary_conditions_level1(u_out, bc, hx, hy, hz) would
@@ -139,8 +145,10 @@ int ocu_mg_relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const oc
  *   OCU_FUSED_ZERO_INPUT: u_in is taken to be the zero grid with its ghosts
  *     filled by that same fill (the state after clear_zero + level-1 fill,
  *     sol_mgpressure3d.cpp:168-170) and is not read; u_in may equal u_out.
- * Either flag needs every face Neumann or Dirichlet (else
- * OCU_ERR_UNSUPPORTED). */
+ * Either flag needs every face Neumann or Dirichlet, except that ghost
+ * writes also take periodic x/y pairs on whole rows (ocu_mg_relax_fused_ok;
+ * else OCU_ERR_UNSUPPORTED).  With periodic faces the sweep equals the
+ * reference's two colours with a fill between and after them. */
 int ocu_mg_relax_fused_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                           const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx,
                           double hy, double hz, int first_color, int flags, void *stream);
@@ -190,6 +198,11 @@ int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int qui
  * and every face Neumann or Dirichlet. */
 int ocu_mg_prolong_bc(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk,
                       const ocu_bc_set *bc, double hx, double hy, double hz, void *stream);
+/* 1 when ocu_mg_prolong_bc runs for this pair and these faces: fine exactly
+ * twice the coarse grid and either every face Neumann or Dirichlet, or
+ * periodic x/y pairs with a non-periodic z on whole rows; else 0 */
+int ocu_mg_prolong_bc_ok(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine,
+                         const ocu_bc_set *bc);
 /* The bottom of a V-cycle in one workgroup: levels top..top+nlev-1 (the
  * last the coarsest) of Sol_MultigridPressure3DDevice, held in LDS.  Does
  * what the per-level calls do on them (sol_mgpressure3d.cpp:184-239):

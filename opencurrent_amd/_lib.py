@@ -76,6 +76,8 @@ HIP_SIGS = {
     "ocu_mg_relax_fused_bc": (I, [GP, GP, GP, MP, BP, D, D, D, I, I, P]),
     "ocu_mg_relax_fused_prolong_bc": (I, [GP, GP, GP, GP, I, MP, BP, D, D, D, I, I, P]),
     "ocu_mg_relax_fused_prolong_ok": (I, [GP, GP, BP]),
+    "ocu_mg_relax_fused_ok": (I, [GP, GP, GP, BP]),
+    "ocu_mg_prolong_bc_ok": (I, [GP, GP, BP]),
     "ocu_mg_residual": (I, [GP, GP, GP, D, D, D, D, P]),
     "ocu_mg_restrict": (I, [GP, GP, P]),
     "ocu_mg_residual_restrict": (I, [GP, GP, GP, GP, D, D, D, D, DP, P]),

@@ -356,13 +356,20 @@ __device__ __forceinline__ void prolong_row_add(int I, int J, int quirk, const C
 // (k0 .. k0+V-1, values v): the row at offset (sx,sy) gets the x/y face
 // transforms (skipped for sx = sy = 0, the row itself) and its two z-end
 // ghosts the z transforms on top.
+//
+// With a periodic x (y) axis the ghost that derives from the low boundary
+// plane (row) is the high ghost, plane nx (row ny), a copy, and the other
+// way round: (sx, sy) name the boundary side of the cell, and the target
+// and the transforms follow from the periodic flags.
 template <typename T, int V>
 __device__ __forceinline__ void write_ghost_row(const DGrid<T> &o, const GhostL1<T> &gw, int q, int j, int k0,
-                                                const Vec<T, V> &v, int sx, int sy) {
+                                                const Vec<T, V> &v, int sx, int sy, int perx = 0, int pery = 0) {
   Vec<T, V> w;
+  const int tx = perx ? 0 : sx, ty = pery ? 0 : sy;  // transformed axes
 #pragma unroll
-  for (int e = 0; e < V; e++) w.v[e] = gw.chain(sx, sy, 0, v.v[e]);
-  T *row = o.p + (int64_t)(q + sx) * o.xs + (int64_t)(j + sy) * o.ys;
+  for (int e = 0; e < V; e++) w.v[e] = gw.chain(tx, ty, 0, v.v[e]);
+  const int dq = sx ? (perx ? -sx * o.nx : sx) : 0, dj = sy ? (pery ? -sy * o.ny : sy) : 0;
+  T *row = o.p + (int64_t)(q + dq) * o.xs + (int64_t)(j + dj) * o.ys;
   if (sx || sy) *reinterpret_cast<Vec<T, V> *>(row + k0) = w;
   if (k0 == 0) row[-1] = gw.xf(5, w.v[0]);
   if (k0 + V == o.nz) row[o.nz] = gw.xf(4, w.v[V - 1]);
@@ -428,7 +435,8 @@ template <typename T, int V, int R, int NW, bool ZERO, bool PRO>
 __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
                                                                typename DevCFor<T>::type c, int first_color,
                                                                int tiles_j, int ichunk, int ntiles,
-                                                               GhostL1<T> gw, int wghost, DGrid<T> uc, int quirk) {
+                                                               GhostL1<T> gw, int wghost, DGrid<T> uc, int quirk,
+                                                               int perx, int pery) {
   typedef Vec<T, V> VT;
   constexpr int NR = R * NW;
   __shared__ VT sU0[NR][64];
@@ -454,15 +462,26 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
   // iterations: addresses are clamped into the allocated ghost box (rows
   // -1..ny, planes -1..nx, cells 0..nz-1) and the values read for rows,
   // planes or lanes outside it only feed updates that are dropped.
+  //
+  // Periodic x or y (even counts, without ZERO or PRO): planes and rows
+  // beyond the box are read from the other end instead of the ghost layer,
+  // and the first colour is computed on them too, so the second colour at
+  // the boundary sees the first colour's new values across the periodic
+  // face, as the reference's fill between colours gives it.
   const int kc = active ? k0 : 0;
+  auto wrapn = [](int a, int n) { return ((a % n) + n) % n; };
   const T *urow[R];
   const T *brow[R];
+  int jw[R];  // the row's j, wrapped on a periodic y axis
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    urow[r] = uin.p + (int64_t)min(max(jb + r, -1), ny) * uin.ys;
-    brow[r] = b.p + (int64_t)min(max(jb + r, 0), ny - 1) * b.ys + kc;
+    jw[r] = pery ? wrapn(jb + r, ny) : jb + r;
+    urow[r] = uin.p + (int64_t)(pery ? jw[r] : min(max(jb + r, -1), ny)) * uin.ys;
+    brow[r] = b.p + (int64_t)(pery ? jw[r] : min(max(jb + r, 0), ny - 1)) * b.ys + kc;
   }
   auto inner_j = [&](int r) { return jb + r >= 0 && jb + r < ny; };
+  auto plane_u = [&](int p) { return perx ? wrapn(p, nx) : min(max(p, -1), nx); };
+  auto plane_b = [&](int p) { return perx ? wrapn(p, nx) : min(max(p, 0), nx - 1); };
   auto ring1 = [&](int r) { return t0 + r >= 1 && t0 + r <= NR - 2; };
   auto tile = [&](int r) { return t0 + r >= 2 && t0 + r <= NR - 3; };
 
@@ -564,14 +583,14 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       pro_load(r, p, f);
       pro_finish(r, p, f, d, g);
     } else {
-      const T *row = urow[r] + (int64_t)min(max(p, -1), nx) * uin.xs;
+      const T *row = urow[r] + (int64_t)plane_u(p) * uin.xs;
       d = *reinterpret_cast<const VT *>(row + kc);
       g = row[gk];
       if (g_in_data) d.v[0] = g;
     }
   };
   auto loadb = [&](int r, int p) -> VT {
-    return *reinterpret_cast<const VT *>(brow[r] + (int64_t)min(max(p, 0), nx - 1) * b.xs);
+    return *reinterpret_cast<const VT *>(brow[r] + (int64_t)plane_b(p) * b.xs);
   };
 
   // per row: U0 planes p (cc) and p+1 (n, loaded during iteration p); U1
@@ -616,12 +635,12 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
     for (int r = 0; r < R; r++) {
       const T lo = from_lower(cc[r].v[V - 1], gc[r]), hi = from_upper(cc[r].v[0], gc[r]);
       u1[r] = cc[r];
-      if (ring1(r) && inner_j(r) && p_in) {
-        const int j = jb + r;
+      if (ring1(r) && (inner_j(r) || pery) && (p_in || perx)) {
+        const int j = jw[r], pw = perx ? wrapn(p, nx) : p;
         const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[t0 - 1][lane];
         const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[t0 + R][lane];
-        relax_row<T, V>(c, diag_xy<T>(c, p, j, nx, ny), (first_color ^ p ^ j) & 1, k0, kupd, nz, cc[r], lo, hi, ym,
-                        yp, u1m1[r], n[r], bcur[r], u1[r]);
+        relax_row<T, V>(c, diag_xy<T>(c, pw, j, nx, ny), (first_color ^ pw ^ j) & 1, k0, kupd, nz, cc[r], lo, hi,
+                        ym, yp, u1m1[r], n[r], bcur[r], u1[r]);
       }
       sU1[p & 1][t0 + r][lane] = u1[r];
     }
@@ -644,10 +663,10 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
           *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
           if (wghost) {  // ghosts of this row: z ends, x and y face rows
             write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 0);
-            if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, -1);
-            if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 1);
-            if (q == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, -1, 0);
-            if (q == nx - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 1, 0);
+            if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, -1, perx, pery);
+            if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 1, perx, pery);
+            if (q == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, -1, 0, perx, pery);
+            if (q == nx - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 1, 0, perx, pery);
           }
         }
       }
@@ -672,8 +691,8 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       const int q = sx < 0 ? 0 : nx - 1;
       if (q < i0 || q >= i1) continue;
       const VT v = *reinterpret_cast<const VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0);
-      if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, -1);
-      if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, 1);
+      if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, -1, perx, pery);
+      if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, 1, perx, pery);
     }
   }
 }
@@ -970,7 +989,8 @@ __global__ __launch_bounds__(256) void k_prolong(DGrid<T> uc, DGrid<T> uf, int q
 
 template <typename T, int V, int NW>
 __global__ __launch_bounds__(64 * NW) void k_prolong_rows(DGrid<T> uc, DGrid<T> uf, int quirk, GhostL1<T> gw,
-                                                          int wghost, int tiles_j, int ichunk, int ntiles) {
+                                                          int wghost, int tiles_j, int ichunk, int ntiles, int perx,
+                                                          int pery) {
   constexpr int VC = V / 2;
   typedef Vec<T, V> VT;
   typedef CRow<T, VC> CR;
@@ -1060,14 +1080,14 @@ __global__ __launch_bounds__(64 * NW) void k_prolong_rows(DGrid<T> uc, DGrid<T> 
       if (wghost) {
         const bool xl = I == 0, xh = I == uf.nx - 1, yl = J == 0, yh = J == uf.ny - 1;
         write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, 0);
-        if (yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, -1);
-        if (yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, 1);
-        if (xl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, 0);
-        if (xh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, 0);
-        if (xl && yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, -1);
-        if (xl && yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, 1);
-        if (xh && yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, -1);
-        if (xh && yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, 1);
+        if (yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, -1, perx, pery);
+        if (yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 0, 1, perx, pery);
+        if (xl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, 0, perx, pery);
+        if (xh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, 0, perx, pery);
+        if (xl && yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, -1, perx, pery);
+        if (xl && yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, -1, 1, perx, pery);
+        if (xh && yl) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, -1, perx, pery);
+        if (xh && yh) write_ghost_row<T, V>(uf, gw, I, J, k0, f, 1, 1, perx, pery);
       }
     }
   }
@@ -1630,7 +1650,8 @@ int fused_ichunk(int nx, int tiles_j, int slots) {
 template <typename T, int V, bool ZERO, bool PRO = false>
 int launch_fused_rows(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                       const typename DevCFor<T>::type &c, int first_color, const GhostL1<T> &gw, int wghost,
-                      hipStream_t stream, const ocu_grid3d *uc = nullptr, int quirk = 0) {
+                      hipStream_t stream, const ocu_grid3d *uc = nullptr, int quirk = 0, int perx = 0,
+                      int pery = 0) {
   typedef RowsShape<T, V> RS;
   constexpr int NR = RS::R * RS::NW;
   auto kern = k_relax_fused_rows<T, V, RS::R, RS::NW, ZERO, PRO>;
@@ -1641,7 +1662,7 @@ int launch_fused_rows(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu
   const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
   kern<<<nblocks, dim3(64, RS::NW), 0, stream>>>(dgrid<T>(u_in), dgrid<T>(u_out), dgrid<T>(b), c, first_color,
                                                  tiles_j, ichunk, ntiles, gw, wghost,
-                                                 PRO ? dgrid<T>(uc) : dgrid<T>(u_in), quirk);
+                                                 PRO ? dgrid<T>(uc) : dgrid<T>(u_in), quirk, perx, pery);
   OCU_HIP_LAUNCHED("ocu_mg_relax_fused");
 }
 
@@ -1664,13 +1685,27 @@ int launch_fused_tiled(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const oc
 template <typename T, bool ZERO>
 int launch_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
                  const typename DevCFor<T>::type &c, int first_color, const GhostL1<T> &gw, int wghost,
-                 hipStream_t stream) {
+                 hipStream_t stream, int perx = 0, int pery = 0) {
   switch (row_vec_width(u_in, u_out, b)) {
-    case 1: return launch_fused_rows<T, 1, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
-    case 2: return launch_fused_rows<T, 2, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
-    case 4: return launch_fused_rows<T, 4, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
+    case 1:
+      return launch_fused_rows<T, 1, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream, nullptr, 0, perx, pery);
+    case 2:
+      return launch_fused_rows<T, 2, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream, nullptr, 0, perx, pery);
+    case 4:
+      return launch_fused_rows<T, 4, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream, nullptr, 0, perx, pery);
     default: return launch_fused_tiled<T, ZERO>(u_in, u_out, b, c, first_color, gw, wghost, stream);
   }
+}
+
+// The whole-row fused sweep also takes periodic x and/or y face pairs (even
+// counts along them) with Neumann/Dirichlet z faces, without the zero-input
+// mode: 1 when that applies to this grid and these faces.
+int fused_periodic_ok(const ocu_grid3d *u, const ocu_grid3d *o, const ocu_grid3d *b, const ocu_bc_set *bc) {
+  if (!gfaces_paired(bc) || bc->zpos.type == OCU_BC_PERIODIC) return 0;
+  const int px = bc->xpos.type == OCU_BC_PERIODIC, py = bc->ypos.type == OCU_BC_PERIODIC;
+  if (!px && !py) return 0;
+  if ((px && u->nx % 2) || (py && u->ny % 2)) return 0;
+  return row_vec_width(u, o, b) ? 1 : 0;
 }
 
 int relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b, const ocu_mg_level_params *p,
@@ -1688,11 +1723,12 @@ int relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3
   OCU_REQUIRE(zero || u_in->base != u_out->base, "%s: u_out must not alias u_in", what);
   OCU_REQUIRE(p && bc, "%s: null params", what);
   OCU_REQUIRE(first_color == 0 || first_color == 1, "%s: bad first_color %d", what, first_color);
-  if (any_periodic(bc)) {
-    set_error("%s: periodic faces need a ghost refresh between colours", what);
+  const bool periodic = any_periodic(bc);
+  if (periodic && !fused_periodic_ok(u_in, u_out, b, bc)) {
+    set_error("%s: periodic faces need x/y pairs with even counts, non-periodic z and whole rows", what);
     return OCU_ERR_UNSUPPORTED;
   }
-  if ((wghost || zero) && !gfaces_local(bc)) {
+  if (!periodic && (wghost || zero) && !gfaces_local(bc)) {
     set_error("%s: ghost writes and zero input need Neumann or Dirichlet faces", what);
     return OCU_ERR_UNSUPPORTED;
   }
@@ -1701,16 +1737,17 @@ int relax_fused(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3
     return OCU_ERR_UNSUPPORTED;
   }
   if ((int64_t)u_in->nx * u_in->ny * u_in->nz == 0) return OCU_OK;
+  const int px = bc->xpos.type == OCU_BC_PERIODIC, py = bc->ypos.type == OCU_BC_PERIODIC;
   if (u_in->dtype == OCU_F64) {
     const DevC64 c = dev_c64(p);
     const GhostL1<double> gw = ghost_l1<double>(bc, hx, hy, hz);
-    return zero ? launch_fused<double, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream))
-                : launch_fused<double, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream));
+    return zero ? launch_fused<double, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream), px, py)
+                : launch_fused<double, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream), px, py);
   }
   const DevC32 c = dev_c32(p);
   const GhostL1<float> gw = ghost_l1<float>(bc, hx, hy, hz);
-  return zero ? launch_fused<float, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream))
-              : launch_fused<float, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream));
+  return zero ? launch_fused<float, true>(u_in, u_out, b, c, first_color, gw, wghost, S(stream), px, py)
+              : launch_fused<float, false>(u_in, u_out, b, c, first_color, gw, wghost, S(stream), px, py);
 }
 
 // The whole-row width of the prolongation-fused sweep (4), or 0 when it
@@ -1766,7 +1803,7 @@ int launch_res_restrict_rows(const ocu_grid3d *u, const ocu_grid3d *b, const ocu
 
 template <typename T, int V>
 int launch_prolong_rows(const ocu_grid3d *uc, const ocu_grid3d *uf, int quirk, const GhostL1<T> &gw, int wghost,
-                        hipStream_t stream) {
+                        hipStream_t stream, int perx = 0, int pery = 0) {
   constexpr int NW = 8;
   const int tiles_j = (uf->ny + NW - 1) / NW;
   // coarse planes per wave: about 4096 waves in all
@@ -1774,8 +1811,18 @@ int launch_prolong_rows(const ocu_grid3d *uc, const ocu_grid3d *uf, int quirk, c
   const int ntiles = tiles_j * ((uc->nx + ichunk - 1) / ichunk);
   const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
   k_prolong_rows<T, V, NW><<<nblocks, dim3(64, NW), 0, stream>>>(dgrid<T>(uc), dgrid<T>(uf), quirk, gw, wghost,
-                                                                 tiles_j, ichunk, ntiles);
+                                                                 tiles_j, ichunk, ntiles, perx, pery);
   OCU_HIP_LAUNCHED("ocu_mg_prolong");
+}
+
+// ghost writes of the prolongation: Neumann/Dirichlet faces (either
+// kernel), or periodic x/y pairs with non-periodic z on the whole-row kernel
+// (fine exactly twice the coarse grid, so the periodic counts are even)
+bool prolong_bc_ok(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, const ocu_bc_set *bc) {
+  if (!any_periodic(bc)) return gfaces_local(bc);
+  if (!gfaces_paired(bc) || bc->zpos.type == OCU_BC_PERIODIC) return false;
+  if (u_fine->nx != 2 * u_coarse->nx || u_fine->ny != 2 * u_coarse->ny || u_fine->nz != 2 * u_coarse->nz) return false;
+  return res_row_width(u_fine, u_fine, u_coarse) != 0;
 }
 
 int prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, const ocu_bc_set *bc, double hx,
@@ -1787,15 +1834,19 @@ int prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, con
               "ocu_mg_prolong: fine grid must be twice the coarse grid");
   OCU_REQUIRE(u_coarse->gx >= 1 && u_coarse->gy >= 1 && u_coarse->gz >= 1, "ocu_mg_prolong: coarse needs ghosts");
   const int wghost = bc ? 1 : 0;
+  int perx = 0, pery = 0;
   if (wghost) {
     // every fine boundary cell must be produced by this launch
     OCU_REQUIRE(u_fine->nx == 2 * u_coarse->nx && u_fine->ny == 2 * u_coarse->ny && u_fine->nz == 2 * u_coarse->nz,
                 "ocu_mg_prolong_bc: the fine grid must be exactly twice the coarse grid");
     OCU_REQUIRE(u_fine->gx >= 1 && u_fine->gy >= 1 && u_fine->gz >= 1, "ocu_mg_prolong_bc: fine needs ghosts");
-    if (!gfaces_local(bc)) {
-      set_error("ocu_mg_prolong_bc: ghost writes need Neumann or Dirichlet faces");
+    if (!prolong_bc_ok(u_coarse, u_fine, bc)) {
+      set_error("ocu_mg_prolong_bc: ghost writes need Neumann or Dirichlet faces, or periodic x/y pairs on "
+                "whole rows");
       return OCU_ERR_UNSUPPORTED;
     }
+    perx = bc->xpos.type == OCU_BC_PERIODIC;
+    pery = bc->ypos.type == OCU_BC_PERIODIC;
   }
   if ((int64_t)u_coarse->nx * u_coarse->ny * u_coarse->nz == 0) return OCU_OK;
   static const ocu_bc_set none = {};
@@ -1805,14 +1856,14 @@ int prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, con
     switch (res_row_width(u_fine, u_fine, u_coarse)) {
       case 2:
         return f64 ? launch_prolong_rows<double, 2>(u_coarse, u_fine, quirk, ghost_l1<double>(fb, hx, hy, hz), wghost,
-                                                     S(stream))
+                                                     S(stream), perx, pery)
                    : launch_prolong_rows<float, 2>(u_coarse, u_fine, quirk, ghost_l1<float>(fb, hx, hy, hz), wghost,
-                                                    S(stream));
+                                                    S(stream), perx, pery);
       case 4:
         return f64 ? launch_prolong_rows<double, 4>(u_coarse, u_fine, quirk, ghost_l1<double>(fb, hx, hy, hz), wghost,
-                                                     S(stream))
+                                                     S(stream), perx, pery)
                    : launch_prolong_rows<float, 4>(u_coarse, u_fine, quirk, ghost_l1<float>(fb, hx, hy, hz), wghost,
-                                                    S(stream));
+                                                    S(stream), perx, pery);
       default: break;
     }
   }
@@ -2009,6 +2060,15 @@ int ocu_mg_relax_fused_bc(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const
   return relax_fused(u_in, u_out, b, p, bc, hx, hy, hz, first_color, flags, stream);
 }
 
+int ocu_mg_relax_fused_ok(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu_grid3d *b,
+                          const ocu_bc_set *bc) {
+  if (!u_in || !u_out || !b || !bc || check_grid(u_in, "ocu_mg_relax_fused_ok") ||
+      check_grid(u_out, "ocu_mg_relax_fused_ok") || check_grid(b, "ocu_mg_relax_fused_ok"))
+    return 0;
+  if (!any_periodic(bc)) return gfaces_local(bc) ? 1 : 0;
+  return fused_periodic_ok(u_in, u_out, b, bc);
+}
+
 int ocu_mg_relax_fused_prolong_ok(const ocu_grid3d *u, const ocu_grid3d *u_coarse, const ocu_bc_set *bc) {
   if (!u || !u_coarse || check_grid(u, "ocu_mg_relax_fused_prolong_ok") ||
       check_grid(u_coarse, "ocu_mg_relax_fused_prolong_ok"))
@@ -2154,6 +2214,17 @@ int ocu_norm_finalize(const double *norm_ws, double *out, void *stream) {
 
 int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, void *stream) {
   return prolong(u_coarse, u_fine, quirk, nullptr, 1.0, 1.0, 1.0, stream);
+}
+
+int ocu_mg_prolong_bc_ok(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, const ocu_bc_set *bc) {
+  if (!u_coarse || !u_fine || !bc || check_grid(u_coarse, "ocu_mg_prolong_bc_ok") ||
+      check_grid(u_fine, "ocu_mg_prolong_bc_ok"))
+    return 0;
+  if (u_fine->nx != 2 * u_coarse->nx || u_fine->ny != 2 * u_coarse->ny || u_fine->nz != 2 * u_coarse->nz) return 0;
+  if (u_fine->gx < 1 || u_fine->gy < 1 || u_fine->gz < 1 || u_coarse->gx < 1 || u_coarse->gy < 1 ||
+      u_coarse->gz < 1)
+    return 0;
+  return prolong_bc_ok(u_coarse, u_fine, bc) ? 1 : 0;
 }
 
 int ocu_mg_prolong_bc(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, const ocu_bc_set *bc,

@@ -126,6 +126,9 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   // reads u, and anything else that reads level l, or writes level l+1,
   // runs it first
   bool _u_pro[kMaxLevels];
+  // the fused sweep runs at level l (every face Neumann/Dirichlet, or the
+  // periodic pairs it takes)
+  bool _fused_lvl[kMaxLevels];
 
   Grid3DDevice<T> &get_u(int level) { return *_u_grid[level]; }
   Grid3DDevice<T> &get_r(int level) { return *_r_grid[level]; }

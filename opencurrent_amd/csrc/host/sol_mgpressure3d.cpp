@@ -307,7 +307,7 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   use_bottom = true;
   _bottom_level = -1;
   use_fused_prolong = true;
-  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = _u_pro[l] = false;
+  for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = _u_pro[l] = _fused_lvl[l] = false;
 }
 
 template <typename T>
@@ -366,7 +366,7 @@ void Sol_MultigridPressure3DDevice<T>::level_params(int level, void *out) const 
 // relaxation, which can start from the zero grid without reading it.
 template <typename T>
 bool Sol_MultigridPressure3DDevice<T>::defer_ok(int level) const {
-  if (!(use_fused_relax && _fused_ok && level < _num_levels - 1 && nu1 >= 2)) return false;
+  if (!(use_fused_relax && _fused_lvl[level] && level < _num_levels - 1 && nu1 >= 2)) return false;
   // the zero grid's ghosts are all zero only for homogeneous faces
   BoundaryConditionSet lbc = get_bc_at_level(level);
   return lbc.xpos.value == 0 && lbc.xneg.value == 0 && lbc.ypos.value == 0 && lbc.yneg.value == 0 &&
@@ -534,7 +534,7 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
     materialize(level);
     return;
   }
-  if (use_fused_relax && _fused_ok) {
+  if (use_fused_relax && _fused_lvl[level]) {
     // pairs of out-of-place sweeps u -> alt -> u, so the result always ends
     // in u; each sweep also writes the level-1 ghosts of its output (the
     // fill the reference runs after every iteration), and the first one
@@ -646,10 +646,10 @@ template <typename T>
 void Sol_MultigridPressure3DDevice<T>::run_prolong(int coarse_level, int fine_level) {
   ocu_grid3d uc = get_u(coarse_level).desc(), uf = get_u(fine_level).desc();
   void *s = ThreadManager::get_compute_stream();
-  if (_fused_ok && uf.nx == 2 * uc.nx && uf.ny == 2 * uc.ny && uf.nz == 2 * uc.nz) {
+  ocu_bc_set fbc = to_c_bc(get_bc_at_level(fine_level));
+  if (use_fused_relax && ocu_mg_prolong_bc_ok(&uc, &uf, &fbc)) {
     // prolongation and the level-1 fill of the fine grid in one launch
-    ocu_bc_set cbc = to_c_bc(get_bc_at_level(fine_level));
-    ok_status(ocu_mg_prolong_bc(&uc, &uf, 1, &cbc, hx(fine_level), hy(fine_level), hz(fine_level), s),
+    ok_status(ocu_mg_prolong_bc(&uc, &uf, 1, &fbc, hx(fine_level), hy(fine_level), hz(fine_level), s),
               "Sol_MultigridPressure3DDevice::prolong");
   } else {
     ok_status(ocu_mg_prolong(&uc, &uf, 1, s), "Sol_MultigridPressure3DDevice::prolong");
@@ -756,6 +756,13 @@ bool Sol_MultigridPressure3DDevice<T>::initialize_storage(int nx_val, int ny_val
     return false;
   }
   _norm_dev = (double *)hd;
+
+  // levels whose relaxation runs as fused sweeps (periodic pairs included)
+  for (int l = 0; l < _num_levels; l++) {
+    ocu_bc_set cbc = to_c_bc(get_bc_at_level(l));
+    ocu_grid3d du = get_u(l).desc(), da = _u_alt[l]->desc(), db = get_b(l).desc();
+    _fused_lvl[l] = _fused_ok || ocu_mg_relax_fused_ok(&du, &da, &db, &cbc) == 1;
+  }
 
   // the smallest level (>= 1) from which every coarser level fits one
   // workgroup's LDS together

@@ -147,6 +147,14 @@ def mg_relax_fused_prolong_bc(u_in, u_out, b, uc, params, bc, hx, hy, hz, first_
                       "mg_relax_fused_prolong_bc")
 
 
+def mg_prolong_bc_ok(uc, uf, bc):
+    return bool(_lib.hip().ocu_mg_prolong_bc_ok(uc.ptr(), uf.ptr(), C.byref(bc)))
+
+
+def mg_relax_fused_ok(u_in, u_out, b, bc):
+    return bool(_lib.hip().ocu_mg_relax_fused_ok(u_in.ptr(), u_out.ptr(), b.ptr(), C.byref(bc)))
+
+
 def mg_relax_fused_prolong_ok(u, uc, bc):
     return bool(_lib.hip().ocu_mg_relax_fused_prolong_ok(u.ptr(), uc.ptr(), C.byref(bc)))
 

@@ -227,6 +227,92 @@ def test_relax_fused_bc(gpu, dtype, size, case, flags):
             assert np.array_equal(a.buf.view(np.uint8), got.buf.view(np.uint8)), "aliased zero-input call"
 
 
+PERIODIC_FUSED_CASES = {
+    "per_xy": bcset(T.BC_PERIODIC, T.BC_PERIODIC, T.BC_PERIODIC, T.BC_PERIODIC, (T.BC_NEUMANN, .7),
+                    (T.BC_NEUMANN, 0.0)),
+    "per_x": bcset(T.BC_PERIODIC, T.BC_PERIODIC, (T.BC_DIRICHELET, .321), (T.BC_DIRICHELET, -1.0),
+                   (T.BC_NEUMANN, .25), (T.BC_DIRICHELET, .5)),
+    "per_y": bcset((T.BC_NEUMANN, .1), (T.BC_DIRICHELET, .3), T.BC_PERIODIC, T.BC_PERIODIC, (T.BC_DIRICHELET, 1.0),
+                   (T.BC_NEUMANN, 0.0)),
+}
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", [(16, 12, 20), (40, 30, 70), (8, 26, 256), (12, 16, 200), (2, 4, 33), (6, 2, 64),
+                                  (4, 34, 128)])
+@pytest.mark.parametrize("case", list(PERIODIC_FUSED_CASES))
+@pytest.mark.parametrize("flags", [T.FUSED_WRITE_GHOSTS, 0, T.FUSED_ZERO_INPUT | T.FUSED_WRITE_GHOSTS])
+def test_relax_fused_periodic_pairs(gpu, dtype, size, case, flags):
+    """periodic x and/or y pairs: one fused iteration against the reference's
+    two colours with the level-1 fill between them (and after them, with
+    ghost writes), over the interior and the whole ghost layer"""
+    bc = PERIODIC_FUSED_CASES[case]
+    zero = bool(flags & T.FUSED_ZERO_INPUT)
+    if zero:
+        bc = homogeneous(bc)
+    hx, hy, hz = 1 / 32, 1 / 20, 1 / 16
+    u = HostGrid(*size, dtype=dtype)
+    if not zero:
+        u.interior()[:] = (2 * uniform(47, u.interior().shape) - 1).astype(dtype)
+    O.lib().orc_bc_fill(0, u.ptr(), C.byref(bc), hx, hy, hz)
+    b = random_grid(*size, dtype, 48)
+    p = O.level_params(bc, 0, hx, hy, hz)
+    # zero input is not read: hand the kernel garbage there
+    uin = random_grid(*size, dtype, 50) if zero else u
+    gin, gb = D().DeviceGrid.from_host(uin), D().DeviceGrid.from_host(b)
+    assert D().mg_relax_fused_ok(gin, gin, gb, bc)
+    for first in (0, 1):
+        want = u.copy()
+        O.lib().orc_mg_relax(want.ptr(), b.ptr(), first, C.byref(p), T.RELAX_DEVICE)
+        O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), hx, hy, hz)
+        O.lib().orc_mg_relax(want.ptr(), b.ptr(), 1 - first, C.byref(p), T.RELAX_DEVICE)
+        if flags & T.FUSED_WRITE_GHOSTS:
+            O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), hx, hy, hz)
+        start = u.copy()
+        start.buf[:] = (2 * uniform(49, start.buf.shape) - 1).astype(dtype)  # ghosts must all be written
+        start.interior()[:] = u.interior()
+        gout = D().DeviceGrid.from_host(start)
+        D().mg_relax_fused_bc(gin, gout, gb, p, bc, hx, hy, hz, first, flags)
+        got = gout.to_host()
+        if flags & T.FUSED_WRITE_GHOSTS:
+            assert_ghost_layer_bitexact(got, want, f"fused periodic {case} {size} first={first}")
+        else:
+            assert np.array_equal(got.interior(), want.interior()), f"fused periodic {case} {size} first={first}"
+    assert_bitexact(gin.to_host(), uin, "input untouched")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("cn", [(3, 5, 64), (4, 3, 128), (2, 2, 50), (5, 1, 33)])
+@pytest.mark.parametrize("case", list(PERIODIC_FUSED_CASES))
+def test_prolong_bc_periodic_pairs(gpu, dtype, cn, case):
+    """prolongation with ghost writes on periodic x/y pairs against the
+    prolongation followed by the level-1 fill, over the whole ghost layer"""
+    bc = PERIODIC_FUSED_CASES[case]
+    hf = (.1, .2, .05)
+    uc = filled(*cn, dtype, 191, bc, .2)
+    uf = random_grid(*(2 * s for s in cn), dtype, 192)
+    want = uf.copy()
+    O.lib().orc_mg_prolong(uc.ptr(), want.ptr(), 1)
+    O.lib().orc_bc_fill(0, want.ptr(), C.byref(bc), *hf)
+    guc, guf = D().DeviceGrid.from_host(uc), D().DeviceGrid.from_host(uf)
+    assert D().mg_prolong_bc_ok(guc, guf, bc)
+    D().mg_prolong_bc(guc, guf, bc, *hf)
+    assert_ghost_layer_bitexact(guf.to_host(), want, f"prolong periodic {case} {cn}")
+
+
+def test_relax_fused_periodic_rejects(gpu):
+    """odd counts along a periodic axis, periodic z and rows the whole-row
+    kernel does not take stay on the colour path"""
+    from opencurrent_amd._lib import OcuError
+    bc = PERIODIC_FUSED_CASES["per_xy"]
+    p = O.level_params(bc, 0, .1, .1, .1)
+    for size, b_ in (((5, 4, 16), bc), ((4, 6, 300), bc), ((4, 4, 16), BC_CASES["periodic"])):
+        u, o, b = (D().DeviceGrid(*size) for _ in range(3))
+        assert not D().mg_relax_fused_ok(u, o, b, b_)
+        with pytest.raises(OcuError):
+            D().mg_relax_fused_bc(u, o, b, p, b_, .1, .1, .1, 0, T.FUSED_WRITE_GHOSTS)
+
+
 def homogeneous(bc):
     h = BCSet()
     for f in BCSet.FACES:
@@ -289,9 +375,9 @@ def test_fused_bc_rejects_nonlocal_faces(gpu):
         D().mg_prolong_bc(D().DeviceGrid(4, 4, 4, dtype=np.float32), u, bc, .1, .1, .1)
 
 
-def test_relax_fused_rejects_periodic(gpu):
+def test_relax_fused_rejects_periodic_z(gpu):
     from opencurrent_amd._lib import OcuError
-    bc = BC_CASES["mixed"]
+    bc = BC_CASES["periodic"]
     u = D().DeviceGrid(8, 8, 8)
     o = D().DeviceGrid(8, 8, 8)
     b = D().DeviceGrid(8, 8, 8)
