@@ -214,12 +214,14 @@ int ocu_mg_prolong_bc_ok(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine,
  * back.  Arrays u[l], b[l], p[l] and h[3l..3l+2] (hx,hy,hz) describe level
  * top+l; zero_top: the top level's u is the zero grid (not read; needs
  * homogeneous faces).  Faces must be Neumann or Dirichlet; the boxes must
- * fit ocu_mg_bottom_lds_bytes() <= 65536. */
+ * fit ocu_mg_bottom_lds_bytes() <= ocu_mg_bottom_lds_max(). */
 int ocu_mg_bottom_vcycle(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_params *p,
                          const double *h, int nlev, const ocu_bc_set *bc, int nu1, int nu2,
                          int coarse_iters, int coarse_mode, int zero_top, void *stream);
 /* LDS bytes a bottom launch over these levels needs (0: invalid request) */
 size_t ocu_mg_bottom_lds_bytes(const ocu_grid3d *u, int nlev, int coarse_mode);
+/* LDS bytes one bottom (or coarse-relax) workgroup may use: 160 KiB */
+size_t ocu_mg_bottom_lds_max(void);
 /* Coarsest-level relaxation in one workgroup (replaces relax_on_host,
  * sol_mgpressure3ddev.cu:134-188, and its two PCIe round trips).
  * mode 0: reference semantics - `iterations` RB-GS iterations with the

@@ -431,7 +431,7 @@ __device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, i
 template <typename T>
 constexpr int fused_rows_min_waves() { return sizeof(T) == 4 ? 6 : 1; }
 
-template <typename T, int V, int R, int NW, bool ZERO, bool PRO>
+template <typename T, int V, int R, int NW, bool ZERO, bool PRO, bool PER>
 __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
                                                                typename DevCFor<T>::type c, int first_color,
                                                                int tiles_j, int ichunk, int ntiles,
@@ -463,25 +463,30 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
   // -1..ny, planes -1..nx, cells 0..nz-1) and the values read for rows,
   // planes or lanes outside it only feed updates that are dropped.
   //
-  // Periodic x or y (even counts, without ZERO or PRO): planes and rows
-  // beyond the box are read from the other end instead of the ghost layer,
-  // and the first colour is computed on them too, so the second colour at
-  // the boundary sees the first colour's new values across the periodic
-  // face, as the reference's fill between colours gives it.
+  // PER, periodic x (perx) and/or y (pery) with even counts, not with PRO:
+  // planes and rows beyond the box are read from the other end instead of
+  // the ghost layer, and the first colour is computed on them too, so the
+  // second colour at the boundary sees the first colour's new values across
+  // the periodic face, as the reference's fill between colours gives it.
+  // Planes stay within -2..nx+1 and the rows that feed kept updates within
+  // -2..ny+1, so one step of the wrap is enough (n >= 2); rows further out
+  // (a last tile past 2*ny on tiny grids) are clamped into the box.
+  static_assert(!(PER && PRO), "periodic faces are not taken with the prolongation");
+  const int px = PER ? perx : 0, py = PER ? pery : 0;
+  auto wrap1 = [](int a, int n) { return a < 0 ? a + n : a >= n ? a - n : a; };
   const int kc = active ? k0 : 0;
-  auto wrapn = [](int a, int n) { return ((a % n) + n) % n; };
   const T *urow[R];
   const T *brow[R];
   int jw[R];  // the row's j, wrapped on a periodic y axis
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    jw[r] = pery ? wrapn(jb + r, ny) : jb + r;
-    urow[r] = uin.p + (int64_t)(pery ? jw[r] : min(max(jb + r, -1), ny)) * uin.ys;
-    brow[r] = b.p + (int64_t)(pery ? jw[r] : min(max(jb + r, 0), ny - 1)) * b.ys + kc;
+    jw[r] = py ? min(max(wrap1(jb + r, ny), 0), ny - 1) : jb + r;
+    urow[r] = uin.p + (int64_t)(py ? jw[r] : min(max(jb + r, -1), ny)) * uin.ys;
+    brow[r] = b.p + (int64_t)(py ? jw[r] : min(max(jb + r, 0), ny - 1)) * b.ys + kc;
   }
   auto inner_j = [&](int r) { return jb + r >= 0 && jb + r < ny; };
-  auto plane_u = [&](int p) { return perx ? wrapn(p, nx) : min(max(p, -1), nx); };
-  auto plane_b = [&](int p) { return perx ? wrapn(p, nx) : min(max(p, 0), nx - 1); };
+  auto plane_u = [&](int p) { return px ? wrap1(p, nx) : min(max(p, -1), nx); };
+  auto plane_b = [&](int p) { return px ? wrap1(p, nx) : min(max(p, 0), nx - 1); };
   auto ring1 = [&](int r) { return t0 + r >= 1 && t0 + r <= NR - 2; };
   auto tile = [&](int r) { return t0 + r >= 2 && t0 + r <= NR - 3; };
 
@@ -635,8 +640,8 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
     for (int r = 0; r < R; r++) {
       const T lo = from_lower(cc[r].v[V - 1], gc[r]), hi = from_upper(cc[r].v[0], gc[r]);
       u1[r] = cc[r];
-      if (ring1(r) && (inner_j(r) || pery) && (p_in || perx)) {
-        const int j = jw[r], pw = perx ? wrapn(p, nx) : p;
+      if (ring1(r) && (inner_j(r) || py) && (p_in || px)) {
+        const int j = jw[r], pw = px ? wrap1(p, nx) : p;
         const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[t0 - 1][lane];
         const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[t0 + R][lane];
         relax_row<T, V>(c, diag_xy<T>(c, pw, j, nx, ny), (first_color ^ pw ^ j) & 1, k0, kupd, nz, cc[r], lo, hi,
@@ -663,10 +668,10 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
           *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
           if (wghost) {  // ghosts of this row: z ends, x and y face rows
             write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 0);
-            if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, -1, perx, pery);
-            if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 1, perx, pery);
-            if (q == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, -1, 0, perx, pery);
-            if (q == nx - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 1, 0, perx, pery);
+            if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, -1, px, py);
+            if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 0, 1, px, py);
+            if (q == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, -1, 0, px, py);
+            if (q == nx - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, 1, 0, px, py);
           }
         }
       }
@@ -691,8 +696,8 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       const int q = sx < 0 ? 0 : nx - 1;
       if (q < i0 || q >= i1) continue;
       const VT v = *reinterpret_cast<const VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0);
-      if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, -1, perx, pery);
-      if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, 1, perx, pery);
+      if (j == 0) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, -1, px, py);
+      if (j == ny - 1) write_ghost_row<T, V>(uout, gw, q, j, k0, v, sx, 1, px, py);
     }
   }
 }
@@ -1654,7 +1659,10 @@ int launch_fused_rows(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu
                       int pery = 0) {
   typedef RowsShape<T, V> RS;
   constexpr int NR = RS::R * RS::NW;
-  auto kern = k_relax_fused_rows<T, V, RS::R, RS::NW, ZERO, PRO>;
+  // the periodic variant only where a face pair is periodic: the plain
+  // sweep carries none of its wrap logic
+  auto kern = (!PRO && (perx || pery)) ? k_relax_fused_rows<T, V, RS::R, RS::NW, ZERO, PRO, !PRO>
+                                       : k_relax_fused_rows<T, V, RS::R, RS::NW, ZERO, PRO, false>;
   const int nx = u_in->nx, ny = u_in->ny;
   const int tiles_j = (ny + NR - 5) / (NR - 4);
   const int ichunk = fused_ichunk(nx, tiles_j, device_slots(kern, 64 * RS::NW));
@@ -1908,7 +1916,9 @@ size_t bottom_layout(const ocu_grid3d *u, int nlev, int mode, BotArgs<T, TC> *A)
   return off;
 }
 
-constexpr size_t BOT_LDS_MAX = 65536;
+// LDS of one workgroup on gfx950 (160 KiB per CU, all of it available to a
+// single workgroup)
+constexpr size_t BOT_LDS_MAX = 163840;
 
 template <typename T, typename TC, int MODE>
 int launch_bottom(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_params *p, const double *h, int nlev,
@@ -1934,6 +1944,9 @@ int launch_bottom(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_p
   A.per.a[1] = bc->ypos.type == OCU_BC_PERIODIC;
   A.per.a[2] = bc->zpos.type == OCU_BC_PERIODIC;
   A.coarse_iters = coarse_iters;
+  // dynamic LDS beyond 64 KiB is requested explicitly
+  OCU_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mg_bottom<T, TC, MODE>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   k_mg_bottom<T, TC, MODE><<<1, BOT_THREADS, bytes, stream>>>(A);
   OCU_HIP_LAUNCHED("ocu_mg_bottom_vcycle");
 }
@@ -1941,6 +1954,8 @@ int launch_bottom(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_mg_level_p
 }  // namespace
 
 extern "C" {
+
+size_t ocu_mg_bottom_lds_max(void) { return BOT_LDS_MAX; }
 
 size_t ocu_mg_bottom_lds_bytes(const ocu_grid3d *u, int nlev, int coarse_mode) {
   if (!u || nlev < 2 || nlev > BOT_MAX) return 0;
@@ -2252,8 +2267,14 @@ int ocu_mg_coarse_relax(const ocu_grid3d *u, const ocu_grid3d *b, int iterations
   size_t box = (size_t)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
   size_t tsize = (mode == 1 || u->dtype == OCU_F64) ? 8 : 4;
   size_t shmem = 2 * box * tsize;
-  OCU_REQUIRE(shmem <= 65536, "ocu_mg_coarse_relax: grid %dx%dx%d too large for one workgroup", g.nx, g.ny, g.nz);
+  OCU_REQUIRE(shmem <= BOT_LDS_MAX, "ocu_mg_coarse_relax: grid %dx%dx%d too large for one workgroup", g.nx, g.ny,
+              g.nz);
   FacesS F = faces_s(bc);
+  // dynamic LDS beyond 64 KiB is requested explicitly
+  const void *kc = mode == 1 ? reinterpret_cast<const void *>(&k_coarse<float, double, 1>)
+                   : u->dtype == OCU_F64 ? reinterpret_cast<const void *>(&k_coarse<double, double, 0>)
+                                         : reinterpret_cast<const void *>(&k_coarse<float, float, 0>);
+  OCU_HIP_TRY(hipFuncSetAttribute(kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   if (mode == 1) {
     k_coarse<float, double, 1><<<1, 256, shmem, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), g, host_c<double>(p), F,
                                                              hx, hy, hz, iterations, order);

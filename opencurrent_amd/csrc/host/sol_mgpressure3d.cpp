@@ -772,7 +772,7 @@ bool Sol_MultigridPressure3DDevice<T>::initialize_storage(int nx_val, int ny_val
     ocu_grid3d d[kMaxLevels];
     for (int l = top; l <= C; l++) d[l - top] = get_u(l).desc();
     size_t bytes = ocu_mg_bottom_lds_bytes(d, C - top + 1, _coarse_mode);
-    if (bytes == 0 || bytes > 65536) break;
+    if (bytes == 0 || bytes > ocu_mg_bottom_lds_max()) break;
     _bottom_level = top;
   }
   _bottom_ok = _fused_ok;
