@@ -94,6 +94,7 @@ HIP_SIGS = {
     "ocu_gradient_adjoint": (I, [GP, GP, GP, GP, D, D, D, D, P]),
     "ocu_mg_bottom_vcycle": (I, [GP, GP, MP, DP, I, BP, I, I, I, I, I, P]),
     "ocu_mg_bottom_lds_bytes": (C.c_size_t, [GP, I, I]),
+    "ocu_mg_bottom_lds_max": (C.c_size_t, []),
     "ocu_mg_coarse_relax": (I, [GP, GP, I, I, MP, BP, D, D, D, I, P]),
     "ocu_ir_residual": (I, [GP, GP, D, GP, D, D, D, D, D, DP, P]),
     "ocu_norm3_finalize": (I, [DP, DP, P]),
