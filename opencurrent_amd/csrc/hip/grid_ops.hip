@@ -130,6 +130,32 @@ __global__ void k_reduce_stage1(DGrid<T> g, int op, double *parts) {
   if (threadIdx.x == 0) parts[blockIdx.x] = r;
 }
 
+// maxabs, max and min (OP 0..2) give the same value in any order, so they
+// walk the grid for bandwidth: each wave takes whole rows (rows dealt
+// round-robin over the waves), every lane keeps four loads of a row in
+// flight; one partial per block as in k_reduce_stage1
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_reduce_max_rows(DGrid<T> g, double *parts) {
+  __shared__ double sh[8];
+  double acc = red_identity(OP);
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * 4, w0 = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int rows = g.nx * g.ny, nz = g.nz;
+  for (int r = w0; r < rows; r += nw) {
+    const int i = r / g.ny, j = r - i * g.ny;
+    const T *p = &g.at(i, j, 0);
+    int k = lane;
+    for (; k + 192 < nz; k += 256) {
+      const double a = (double)p[k], b = (double)p[k + 64], c = (double)p[k + 128], d = (double)p[k + 192];
+      acc = red_combine(OP, red_combine(OP, acc, red_process(OP, a)), red_process(OP, b));
+      acc = red_combine(OP, red_combine(OP, acc, red_process(OP, c)), red_process(OP, d));
+    }
+    for (; k < nz; k += 64) acc = red_combine(OP, acc, red_process(OP, (double)p[k]));
+  }
+  double r = block_reduce(OP, acc, sh);
+  if (threadIdx.x == 0) parts[blockIdx.x] = r;
+}
+
 __global__ void k_reduce_stage2(int op, const double *parts, int n, double *out) {
   __shared__ double sh[8];
   double acc = red_identity(op);
@@ -238,8 +264,22 @@ int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, v
   OCU_REQUIRE(workspace && out, "ocu_grid_reduce: null workspace/out");
   OCU_REQUIRE(g->dtype != OCU_I32, "ocu_grid_reduce: int grids are not supported");
   double *parts = (double *)workspace;
-  if (g->dtype == OCU_F64) k_reduce_stage1<double><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<double>(g), op, parts);
-  else k_reduce_stage1<float><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<float>(g), op, parts);
+  OCU_REQUIRE((int64_t)g->nx * g->ny < INT_MAX, "ocu_grid_reduce: too many rows");
+#define RMAX(TT, OP) k_reduce_max_rows<TT, OP><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<TT>(g), parts)
+  if (op <= 2 && g->dtype == OCU_F64) {
+    if (op == 0) RMAX(double, 0);
+    else if (op == 1) RMAX(double, 1);
+    else RMAX(double, 2);
+  } else if (op <= 2) {
+    if (op == 0) RMAX(float, 0);
+    else if (op == 1) RMAX(float, 1);
+    else RMAX(float, 2);
+  } else if (g->dtype == OCU_F64) {
+    k_reduce_stage1<double><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<double>(g), op, parts);
+  } else {
+    k_reduce_stage1<float><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<float>(g), op, parts);
+  }
+#undef RMAX
   k_reduce_stage2<<<1, 512, 0, S(stream)>>>(op, parts, NORM_PARTS, out);
   OCU_HIP_LAUNCHED("ocu_grid_reduce");
 }

@@ -770,6 +770,30 @@ def test_ir_residual(gpu, size):
     assert abs(sx - xc.sum()) <= 1e-12 * np.abs(xc).sum()
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", [(1, 1, 1), (3, 5, 7), (17, 9, 64), (4, 3, 255), (2, 6, 300), (2, 2, 513)])
+def test_grid_reduce_max_ops(gpu, dtype, size):
+    """maxabs, max and min (rows per wave, four loads per lane in flight)
+    over row lengths below, at and past one pass of 256 cells, with the
+    extremes planted in the ghost layer (never read) and, in a second grid,
+    in the last cell of the last row"""
+    g = random_grid(*size, dtype, 901)
+    full = g.full()
+    full[0], full[-1] = 50.0, -50.0
+    full[:, 0], full[:, -1] = -60.0, 60.0
+    full[:, :, 0], full[:, :, size[2] + 1:] = 70.0, -70.0  # k = -1, and k = nz with the padding
+    ai = g.interior().astype(np.float64)
+    gd = D().DeviceGrid.from_host(g)
+    assert D().grid_reduce(gd, 0) == np.abs(ai).max()
+    assert D().grid_reduce(gd, 1) == ai.max()
+    assert D().grid_reduce(gd, 2) == ai.min()
+    for v in (3.0, -3.0):
+        g.interior()[-1, -1, -1] = v
+        gd = D().DeviceGrid.from_host(g)
+        assert D().grid_reduce(gd, 0) == 3.0
+        assert D().grid_reduce(gd, 1 if v > 0 else 2) == v
+
+
 @pytest.mark.parametrize("size", [(16, 12, 20), (8, 6, 256), (5, 7, 200), (3, 4, 130), (4, 9, 64), (2, 3, 1),
                                   (6, 5, 300)])
 def test_grid_stats3(gpu, size):
