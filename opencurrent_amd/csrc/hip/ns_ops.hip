@@ -4,6 +4,7 @@
 // requires congruent grids, sol_selfadvection3ddev.cu:622-626), so one
 // index serves every array.
 #include "ocu_hip_common.h"
+#include "rows.h"
 
 using namespace ocu_hip;
 
@@ -398,6 +399,86 @@ __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ phi, T 
   d[id] = dp + coef * lap;
 }
 
+// The same d += coef * lap(phi) by rows: each wave owns a row j (V cells
+// per lane) and marches along the planes of its i-chunk with planes i-1, i,
+// i+1 of its row in registers; rows j-1 and j+1 of plane i are read beside
+// them (other waves' rows: L2 hits) and k-neighbours come by DPP with the
+// row's z ghosts.  Same operations in the same order as k_laplacian.  Tiles
+// are dealt to the XCDs in contiguous runs (gridDim.x a multiple of 8).
+constexpr int LAP_NW = 4;
+template <typename T, int V>
+__global__ __launch_bounds__(64 * LAP_NW) void k_laplacian_rows(DGrid<T> phi, DGrid<T> d, Inv<T> ih2, T coef, int nx,
+                                                                int ny, int nz, int tiles_j, int ichunk,
+                                                                int ntiles) {
+  typedef Vec<T, V> VT;
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int per = gridDim.x / 8;
+  const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  const int tj = t % tiles_j, ti = t / tiles_j;
+  const int j = tj * LAP_NW + w;
+  if (j >= ny) return;
+  const int k0 = lane * V;
+  const bool active = k0 < nz;
+  const int kc = active ? k0 : 0;
+  const int hi_lane = (nz == 64 * V) ? 63 : nz / V;
+  const int gk = lane == 0 ? -1 : nz;
+  const bool g_in_data = lane == hi_lane && nz < 64 * V;
+  const int i0 = ti * ichunk, i1 = min(nx, i0 + ichunk);
+  const T *pr = phi.p + (int64_t)j * phi.ys;
+  T *dr = d.p + (int64_t)j * d.ys + kc;
+  auto row = [&](int i, int dj) {
+    return *reinterpret_cast<const VT *>(pr + (int64_t)i * phi.xs + (int64_t)dj * phi.ys + kc);
+  };
+  auto rowg = [&](int i, VT &v, T &g) {
+    const T *p = pr + (int64_t)i * phi.xs;
+    v = *reinterpret_cast<const VT *>(p + kc);
+    g = p[gk];
+    if (g_in_data) v.v[0] = g;
+  };
+  VT prev = row(i0 - 1, 0), cur, nxt;
+  T gc, gn;
+  rowg(i0, cur, gc);
+  for (int i = i0; i < i1; ++i) {
+    rowg(i + 1, nxt, gn);
+    const VT ym = row(i, -1), yp = row(i, 1);
+    VT dv = *reinterpret_cast<const VT *>(dr + (int64_t)i * d.xs);
+    const T lo = from_lower(cur.v[V - 1], gc), hi = from_upper(cur.v[0], gc);
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+      const T p = cur.v[e], zm = e > 0 ? cur.v[e - 1] : lo, zp = e < V - 1 ? cur.v[e + 1] : hi;
+      const T lap = ih2.z * (zp + zm - ((T)2) * p) + ih2.y * (yp.v[e] + ym.v[e] - ((T)2) * p) +
+                    ih2.x * (nxt.v[e] + prev.v[e] - ((T)2) * p);
+      dv.v[e] = dv.v[e] + coef * lap;
+    }
+    if (active) *reinterpret_cast<VT *>(dr + (int64_t)i * d.xs) = dv;
+    prev = cur;
+    cur = nxt;
+    gc = gn;
+  }
+}
+
+template <typename T>
+int launch_laplacian_rows(const ocu_grid3d *phi, const ocu_grid3d *dphi, Inv<T> ih2, T coef, int nx, int ny, int nz,
+                          hipStream_t stream) {
+  const int V = nz <= 64 ? 1 : nz <= 128 ? 2 : nz <= 256 ? 4 : 0;
+  if (!V || nz % V || !row_aligned(phi, V) || !row_aligned(dphi, V)) return 0;
+  const int tiles_j = (ny + LAP_NW - 1) / LAP_NW;
+  // about 2048 blocks: short chunks re-read the x-neighbour planes
+  const int ichunk = std::max(1, (int)(((int64_t)nx * tiles_j + 2047) / 2048));
+  const int ntiles = tiles_j * ((nx + ichunk - 1) / ichunk);
+  const int nblocks = (ntiles + 7) / 8 * 8;
+  const dim3 blk(64, LAP_NW);
+#define LAPR(VV)                                                                                                   \
+  k_laplacian_rows<T, VV><<<nblocks, blk, 0, stream>>>(dgrid<T>(phi), dgrid<T>(dphi), ih2, coef, nx, ny, nz, tiles_j, \
+                                                       ichunk, ntiles)
+  if (V == 1) LAPR(1);
+  else if (V == 2) LAPR(2);
+  else LAPR(4);
+#undef LAPR
+  return 1;
+}
+
 // Eqn_IncompressibleNS3D_add_thermal_force (eqn_incompressns3d.cu:25-44)
 template <typename T>
 __global__ __launch_bounds__(256) void k_thermal(T *__restrict__ d, const T *__restrict__ t, T coef, int64_t xs,
@@ -713,6 +794,12 @@ int ocu_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int
   if ((int64_t)nx * ny * nz == 0) return OCU_OK;
   int64_t xs = (int64_t)phi->pny * phi->pnz, ys = phi->pnz;
   dim3 g = stencil_grid(nx, ny, nz);
+  const Inv<double> ih2d{(double)(1 / (hx * hx)), (double)(1 / (hy * hy)), (double)(1 / (hz * hz))};
+  const Inv<float> ih2f{(float)(1 / (hx * hx)), (float)(1 / (hy * hy)), (float)(1 / (hz * hz))};
+  if (phi->dtype == OCU_F64 ? launch_laplacian_rows<double>(phi, dphi, ih2d, coefficient, nx, ny, nz, S(stream))
+                            : launch_laplacian_rows<float>(phi, dphi, ih2f, (float)coefficient, nx, ny, nz,
+                                                           S(stream)))
+    OCU_HIP_LAUNCHED("ocu_laplacian_acc");
   if (phi->dtype == OCU_F64)
     k_laplacian<double><<<g, stencil_block(), 0, S(stream)>>>(
         at0<double>(phi), at0<double>(dphi),

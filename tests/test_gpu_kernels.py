@@ -512,6 +512,23 @@ def test_coarse_relax_reference_semantics(gpu, dtype, case, order):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("size", [(5, 7, 3), (9, 4, 64), (3, 5, 100), (4, 3, 256), (3, 2, 260)])
+def test_lincomb_interior(gpu, dtype, size):
+    """the interior-only combination (padding over 3%) on rows of several
+    lengths, bit for bit, also in place"""
+    a, b, r = (random_grid(*size, dtype, 870 + q) for q in range(3))
+    want = r.copy()
+    O.lib().orc_lincomb2(want.ptr(), .75, a.ptr(), -1.25, b.ptr())
+    gr, ga, gb = (D().DeviceGrid.from_host(x) for x in (r, a, b))
+    D().grid_lincomb(gr, .75, ga, -1.25, gb)
+    assert_bitexact(gr.to_host(), want, "lincomb interior")
+    want2 = a.copy()
+    O.lib().orc_lincomb2(want2.ptr(), 1.0, a.ptr(), .5, b.ptr())
+    D().grid_lincomb(ga, 1.0, ga, .5, gb)
+    assert_bitexact(ga.to_host(), want2, "lincomb interior in place")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_grid_ops(gpu, dtype):
     # 64^3 with padding (66,66,80): > 3% waste -> interior-only update
     a = random_grid(64, 64, 64, dtype, 81)
@@ -711,8 +728,12 @@ def test_divergence_gradient_adjoints(gpu, dtype, n):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-def test_laplacian_thermal(gpu, dtype):
-    nx, ny, nz = 16, 12, 20
+@pytest.mark.parametrize("size", [(16, 12, 20), (5, 7, 200), (3, 4, 130), (8, 6, 256), (4, 9, 64), (2, 3, 1),
+                                  (6, 5, 300)])
+def test_laplacian_thermal(gpu, dtype, size):
+    """the Laplacian (whole rows of 1, 2 or 4 cells per lane, or tiled past
+    256) and the thermal force against the oracle, bit for bit"""
+    nx, ny, nz = size
     u, v, w, t = mac_grids(nx, ny, nz, dtype)
     for q, g in enumerate((u, t)):
         g.buf[:] = (2 * uniform(110 + q, g.buf.shape) - 1).astype(dtype)
