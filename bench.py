@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-extra", action="store_true", help="skip the C1/C3/C4/C5 side measurements")
+    ap.add_argument("--only", choices=["c1", "c3", "c4", "c5"],
+                    help="run only this side measurement and print its JSON (for profiling it alone)")
     return ap.parse_args()
 
 
@@ -401,6 +403,11 @@ def main():
     device.require_gpu()
     hip = _lib.hip()
     stream = _lib.host().ocu_get_compute_stream()
+    if args.only:
+        side = {"c1": lambda: time_fp64_mg_64(hip, stream), "c3": time_ns_step, "c4": time_adjoint,
+                "c5": time_lockex}
+        print(json.dumps({args.only: side[args.only]()}))
+        return
 
     n = args.n
     cells = n ** 3
