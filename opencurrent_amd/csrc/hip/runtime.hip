@@ -188,7 +188,7 @@ int ocu_stream_end_capture(void *stream, void **graph_exec) {
   OCU_HIP_TRY(hipStreamEndCapture(S(stream), &g));
   hipGraphExec_t ge;
   hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);  // the executable graph no longer needs it; instantiation's status is what counts
   if (e != hipSuccess) {
     set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
     return OCU_ERR_RUNTIME;

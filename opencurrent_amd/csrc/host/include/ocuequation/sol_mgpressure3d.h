@@ -67,7 +67,7 @@ class Sol_MultigridPressure3DBase : public Solver {
   // first level of a one-launch bottom of the V-cycle (levels bottom..coarse
   // in one workgroup), or -1; bottom_vcycle(top) runs that part from `top`
   virtual int bottom_start() const { return -1; }
-  virtual void bottom_vcycle(int top) {}
+  virtual void bottom_vcycle(int /*top*/) {}
 
  public:
   BoundaryConditionSet bc;
