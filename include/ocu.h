@@ -102,10 +102,12 @@ typedef struct ocu_ns_params {
   int32_t mixed_projection;
 } ocu_ns_params;
 
-/* init_* are host buffers laid out by Eqn_IncompressibleNS3DParams::init_grids
- * (eqn_incompressns3d.h:78-86), element type by dtype. */
-void *ocu_ns_create(int dtype, const ocu_ns_params *p, const void *init_u, const void *init_v,
-                    const void *init_w, const void *init_temp);
+/* init_* describe host buffers (base = host pointer) laid out by
+ * Eqn_IncompressibleNS3DParams::init_grids (eqn_incompressns3d.h:78-86) for
+ * p->nx, ny, nz, element type dtype; a descriptor that differs from that
+ * layout or type is rejected (NULL, ocu_last_error says which). */
+void *ocu_ns_create(int dtype, const ocu_ns_params *p, const ocu_grid3d *init_u, const ocu_grid3d *init_v,
+                    const ocu_grid3d *init_w, const ocu_grid3d *init_temp);
 int ocu_ns_advance_one_step(void *h, double dt);
 int ocu_ns_advance(void *h, double dt);
 int ocu_ns_max_stable_timestep(void *h, double *dt);
@@ -113,6 +115,28 @@ int ocu_ns_max_stable_timestep(void *h, double *dt);
 int ocu_ns_field(void *h, int which, ocu_grid3d *out);
 int ocu_ns_num_steps(void *h);
 void ocu_ns_destroy(void *h);
+
+/* ---- Eqn_ScalarAdvection3D<T> (ocuequation/eqn_scalaradvection3d.h,
+ * eqn_scalaradvection3d.cpp): forward-Euler passive advection of a scalar
+ * by a fixed MAC velocity, with the no-corner ghost fill after each step. */
+typedef struct ocu_scalar_params {
+  int32_t nx, ny, nz;
+  double hx, hy, hz;
+  ocu_bc_set bc;
+  int32_t advection_scheme;
+} ocu_scalar_params;
+
+/* init_* describe host buffers laid out by
+ * Eqn_ScalarAdvection3DParams::init_grids (u, v, w: MAC paddings (0,1,1),
+ * (1,0,1), (1,1,0); initial values (1,1,1)), checked like ocu_ns_create's. */
+void *ocu_scalar_create(int dtype, const ocu_scalar_params *p, const ocu_grid3d *init_u, const ocu_grid3d *init_v,
+                        const ocu_grid3d *init_w, const ocu_grid3d *init_phi);
+int ocu_scalar_advance_one_step(void *h, double dt);
+int ocu_scalar_advance(void *h, double dt);
+int ocu_scalar_max_stable_timestep(void *h, double *dt);
+/* which: 0 u, 1 v, 2 w, 3 phi, 4 deriv_phidt (device descriptors) */
+int ocu_scalar_field(void *h, int which, ocu_grid3d *out);
+void ocu_scalar_destroy(void *h);
 
 #ifdef __cplusplus
 }

@@ -131,6 +131,16 @@ class NSParams(C.Structure):
     ]
 
 
+class ScalarParams(C.Structure):
+    """ocu_scalar_params (include/ocu.h)"""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+        ("bc", BCSet),
+        ("advection_scheme", C.c_int32),
+    ]
+
+
 HOST_SIGS = {
     "ocu_set_compute_stream": (I, [P]),
     "ocu_get_compute_stream": (P, []),
@@ -152,13 +162,19 @@ HOST_SIGS = {
     "ocu_project_divergence": (I, [P, GP]),
     "ocu_project_pressure": (I, [P, GP]),
     "ocu_project_destroy": (None, [P]),
-    "ocu_ns_create": (P, [I, C.POINTER(NSParams), P, P, P, P]),
+    "ocu_ns_create": (P, [I, C.POINTER(NSParams), GP, GP, GP, GP]),
     "ocu_ns_advance_one_step": (I, [P, D]),
     "ocu_ns_advance": (I, [P, D]),
     "ocu_ns_max_stable_timestep": (I, [P, DP]),
     "ocu_ns_field": (I, [P, I, GP]),
     "ocu_ns_num_steps": (I, [P]),
     "ocu_ns_destroy": (None, [P]),
+    "ocu_scalar_create": (P, [I, C.POINTER(ScalarParams), GP, GP, GP, GP]),
+    "ocu_scalar_advance_one_step": (I, [P, D]),
+    "ocu_scalar_advance": (I, [P, D]),
+    "ocu_scalar_max_stable_timestep": (I, [P, DP]),
+    "ocu_scalar_field": (I, [P, I, GP]),
+    "ocu_scalar_destroy": (None, [P]),
 }
 
 

@@ -105,12 +105,18 @@ class AdvectProjectProblem:
         self.mg_adj = MultigridPressure3D(self.lam_p, hx, hy, hz, poisson_bc(self.flow_bc),
                                           convergence=CONVERGENCE_LINF)
         self.phi0, self.target, self.dphi = (D.DeviceGrid.congruent(like) for _ in range(3))
+        # seed = phi_steps - target of the last forward; backward works on
+        # copies in lam / lam2, so it can be repeated
+        self.seed = D.DeviceGrid.congruent(like)
         self.lam, self.lam2 = D.DeviceGrid.congruent(like), D.DeviceGrid.congruent(like)
         nseg = (self.steps + self.ce - 1) // self.ce
         self.ck = [D.DeviceGrid.congruent(like) for _ in range(nseg)]
         self.seg = [D.DeviceGrid.congruent(like) for _ in range(min(self.ce, self.steps))]
         self.masks = [parameter_mask(g, q) for q, g in enumerate((self.u, self.v, self.w))]
         self.loss = None
+        # number of forward passes run; a gradient belongs to the forward
+        # whose checkpoints it replays (see AdvectProjectLoss)
+        self.generation = 0
 
     # -- pieces ----------------------------------------------------------
     def _project(self, proj):
@@ -166,20 +172,24 @@ class AdvectProjectProblem:
             if n % self.ce == 0:
                 D.grid_copy(self.ck[n // self.ce], self.phi)
             self._step(self.phi, self.phi)
-        self.lam.tensor.zero_()
-        D.grid_lincomb(self.lam, 1.0, self.phi, -1.0, self.target)
-        D.grid_clear_ghosts(self.lam)
-        self.loss = .5 * D.grid_reduce(self.lam, 4)
+        self.seed.tensor.zero_()
+        D.grid_lincomb(self.seed, 1.0, self.phi, -1.0, self.target)
+        D.grid_clear_ghosts(self.seed)
+        self.loss = .5 * D.grid_reduce(self.seed, 4)
+        self.generation += 1
         return self.loss
 
     def backward(self):
         """gradients of the last forward's loss w.r.t. u, v, w (flat tensors)
-        and w.r.t. the initial scalar's interior"""
-        assert self.loss is not None, "forward first"
+        and w.r.t. the initial scalar's interior; repeatable until the next
+        forward"""
+        if self.loss is None:
+            raise RuntimeError("AdvectProjectProblem.backward: run forward first")
         nx, ny, nz = self.n
         for g in (self.au, self.av, self.aw):
             g.tensor.zero_()
         lam, lam2 = self.lam, self.lam2
+        D.grid_copy(lam, self.seed)
         for s in reversed(range(len(self.ck))):
             n0, n1 = s * self.ce, min(s * self.ce + self.ce, self.steps)
             D.grid_copy(self.seg[0], self.ck[s])
@@ -198,7 +208,6 @@ class AdvectProjectProblem:
             x = g.tensor.clone()
             x[~m] = 0
             grads.append(x)
-        self.lam, self.lam2 = lam, lam2
         return grads[0], grads[1], grads[2], grad_phi0
 
 
@@ -206,15 +215,24 @@ def _function():
     torch = D.torch()
 
     class AdvectProjectLoss(torch.autograd.Function):
-        """loss = problem.forward(u, v, w); gradients from problem.backward()"""
+        """loss = problem.forward(u, v, w); gradients from problem.backward().
+        The problem keeps one forward's checkpoints: a loss whose problem has
+        run forward again since cannot be differentiated (RuntimeError);
+        backward of the same loss may be repeated (retain_graph)."""
 
         @staticmethod
         def forward(ctx, u, v, w, problem):
             ctx.problem = problem
-            return torch.tensor(problem.forward(u, v, w), dtype=torch.float64, device=u.device)
+            loss = problem.forward(u, v, w)
+            ctx.generation = problem.generation
+            return torch.tensor(loss, dtype=torch.float64, device=u.device)
 
         @staticmethod
         def backward(ctx, g):
+            if ctx.problem.generation != ctx.generation:
+                raise RuntimeError("advect_project_loss: the problem ran forward again after this loss was "
+                                   "computed, so its checkpoints belong to another input; use one problem per "
+                                   "loss in a graph")
             gu, gv, gw, _ = ctx.problem.backward()
             s = g.to(gu.dtype)
             return (gu * s).to(gu.dtype), (gv * s).to(gv.dtype), (gw * s).to(gw.dtype), None

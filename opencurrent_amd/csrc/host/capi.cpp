@@ -7,6 +7,7 @@
 
 #include "ocu_hip.h"
 #include "ocuequation/eqn_incompressns3d.h"
+#include "ocuequation/eqn_scalaradvection3d.h"
 #include "ocuequation/sol_mgpressure3d.h"
 #include "ocuequation/sol_mgpressuremixed3d.h"
 #include "ocuequation/sol_project3d.h"
@@ -85,9 +86,28 @@ struct NSHandle {
   }
 };
 
+// A host descriptor handed in for one of the init grids must be exactly the
+// layout and type the params' init_grids gave that grid: its buffer is
+// copied over the whole allocation.
 template <typename T>
-bool fill_params(Eqn_IncompressibleNS3DParams<T> &P, const ocu_ns_params *p, const void *iu, const void *iv,
-                 const void *iw, const void *it) {
+bool host_grid_from(Grid3DHost<T> &dst, const ocu_grid3d *src, const char *what) {
+  ocu_grid3d want = dst.desc();
+  if (!src || !src->base || src->dtype != want.dtype || src->nx != want.nx || src->ny != want.ny ||
+      src->nz != want.nz || src->gx != want.gx || src->gy != want.gy || src->gz != want.gz ||
+      src->pnx != want.pnx || src->pny != want.pny || src->pnz != want.pnz || src->shift != want.shift ||
+      src->allocated != want.allocated) {
+    printf("[ERROR] %s - descriptor does not match init_grids' layout (%d,%d,%d) padded (%d,%d,%d) dtype %d\n",
+           what, want.nx, want.ny, want.nz, want.pnx, want.pny, want.pnz, want.dtype);
+    return false;
+  }
+  memcpy(dst.buffer(), src->base, (size_t)want.allocated * sizeof(T));
+  return true;
+}
+
+template <typename T>
+bool fill_params(Eqn_IncompressibleNS3DParams<T> &P, const ocu_ns_params *p, const ocu_grid3d *iu,
+                 const ocu_grid3d *iv, const ocu_grid3d *iw, const ocu_grid3d *it) {
+  if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0) return false;
   P.init_grids(p->nx, p->ny, p->nz, true);
   P.hx = p->hx; P.hy = p->hy; P.hz = p->hz;
   P.max_divergence = p->max_divergence;
@@ -102,13 +122,32 @@ bool fill_params(Eqn_IncompressibleNS3DParams<T> &P, const ocu_ns_params *p, con
   P.time_step = (TimeStepType)p->time_step;
   P.cfl_factor = p->cfl_factor;
   P.mixed_precision_projection = p->mixed_projection != 0;
-  size_t n = (size_t)P.init_u.num_allocated_elements() * sizeof(T);
-  memcpy(P.init_u.buffer(), iu, n);
-  memcpy(P.init_v.buffer(), iv, n);
-  memcpy(P.init_w.buffer(), iw, n);
-  memcpy(P.init_temp.buffer(), it, n);
-  return true;
+  return host_grid_from(P.init_u, iu, "ocu_ns_create init_u") && host_grid_from(P.init_v, iv, "ocu_ns_create init_v") &&
+         host_grid_from(P.init_w, iw, "ocu_ns_create init_w") &&
+         host_grid_from(P.init_temp, it, "ocu_ns_create init_temp");
 }
+
+template <typename T>
+bool fill_scalar_params(Eqn_ScalarAdvection3DParams<T> &P, const ocu_scalar_params *p, const ocu_grid3d *iu,
+                        const ocu_grid3d *iv, const ocu_grid3d *iw, const ocu_grid3d *iphi) {
+  if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0) return false;
+  P.init_grids(p->nx, p->ny, p->nz, true);
+  P.hx = p->hx; P.hy = p->hy; P.hz = p->hz;
+  P.bc = from_c(p->bc);
+  P.advection_scheme = (InterpolationType)p->advection_scheme;
+  return host_grid_from(P.u, iu, "ocu_scalar_create init_u") && host_grid_from(P.v, iv, "ocu_scalar_create init_v") &&
+         host_grid_from(P.w, iw, "ocu_scalar_create init_w") &&
+         host_grid_from(P.initial_values, iphi, "ocu_scalar_create init_phi");
+}
+
+struct ScalarHandle {
+  Eqn_ScalarAdvection3D<float> *f = nullptr;
+  Eqn_ScalarAdvection3D<double> *d = nullptr;
+  ~ScalarHandle() {
+    delete f;
+    delete d;
+  }
+};
 
 }  // namespace
 
@@ -323,8 +362,8 @@ void ocu_project_destroy(void *hh) { delete (ProjHandle *)hh; }
 
 // ------------------------------------------------------------------- NS eqn
 
-void *ocu_ns_create(int dtype, const ocu_ns_params *p, const void *iu, const void *iv, const void *iw,
-                    const void *it) {
+void *ocu_ns_create(int dtype, const ocu_ns_params *p, const ocu_grid3d *iu, const ocu_grid3d *iv,
+                    const ocu_grid3d *iw, const ocu_grid3d *it) {
   if (!p || !iu || !iv || !iw || !it) return nullptr;
   NSHandle *h = new (std::nothrow) NSHandle();
   if (!h) return nullptr;
@@ -332,14 +371,16 @@ void *ocu_ns_create(int dtype, const ocu_ns_params *p, const void *iu, const voi
   bool ok = false;
   if (dtype == OCU_F64) {
     Eqn_IncompressibleNS3DParams<double> P;
-    fill_params(P, p, iu, iv, iw, it);
-    h->d = new Eqn_IncompressibleNS3D<double>();
-    ok = h->d->set_parameters(P);
+    if (fill_params(P, p, iu, iv, iw, it)) {
+      h->d = new Eqn_IncompressibleNS3D<double>();
+      ok = h->d->set_parameters(P);
+    }
   } else if (dtype == OCU_F32) {
     Eqn_IncompressibleNS3DParams<float> P;
-    fill_params(P, p, iu, iv, iw, it);
-    h->f = new Eqn_IncompressibleNS3D<float>();
-    ok = h->f->set_parameters(P);
+    if (fill_params(P, p, iu, iv, iw, it)) {
+      h->f = new Eqn_IncompressibleNS3D<float>();
+      ok = h->f->set_parameters(P);
+    }
   }
   if (!ok) {
     delete h;
@@ -387,5 +428,67 @@ int ocu_ns_num_steps(void *hh) {
 }
 
 void ocu_ns_destroy(void *hh) { delete (NSHandle *)hh; }
+
+// ------------------------------------------------------- scalar advection
+
+void *ocu_scalar_create(int dtype, const ocu_scalar_params *p, const ocu_grid3d *iu, const ocu_grid3d *iv,
+                        const ocu_grid3d *iw, const ocu_grid3d *iphi) {
+  if (!p) return nullptr;
+  ScalarHandle *h = new (std::nothrow) ScalarHandle();
+  if (!h) return nullptr;
+  bool ok = false;
+  if (dtype == OCU_F64) {
+    Eqn_ScalarAdvection3DParams<double> P;
+    if (fill_scalar_params(P, p, iu, iv, iw, iphi)) {
+      h->d = new Eqn_ScalarAdvection3D<double>();
+      ok = h->d->set_parameters(P);
+    }
+  } else if (dtype == OCU_F32) {
+    Eqn_ScalarAdvection3DParams<float> P;
+    if (fill_scalar_params(P, p, iu, iv, iw, iphi)) {
+      h->f = new Eqn_ScalarAdvection3D<float>();
+      ok = h->f->set_parameters(P);
+    }
+  }
+  if (!ok) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int ocu_scalar_advance_one_step(void *hh, double dt) {
+  ScalarHandle *h = (ScalarHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return (h->d ? h->d->advance_one_step(dt) : h->f->advance_one_step(dt)) ? 1 : 0;
+}
+
+int ocu_scalar_advance(void *hh, double dt) {
+  ScalarHandle *h = (ScalarHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return (h->d ? h->d->advance(dt) : h->f->advance(dt)) ? 1 : 0;
+}
+
+int ocu_scalar_max_stable_timestep(void *hh, double *dt) {
+  ScalarHandle *h = (ScalarHandle *)hh;
+  if (!h || !dt) return OCU_ERR_INVALID;
+  *dt = h->d ? h->d->get_max_stable_timestep() : h->f->get_max_stable_timestep();
+  return OCU_OK;
+}
+
+int ocu_scalar_field(void *hh, int which, ocu_grid3d *out) {
+  ScalarHandle *h = (ScalarHandle *)hh;
+  if (!h || !out || which < 0 || which > 4) return OCU_ERR_INVALID;
+  if (h->d) {
+    *out = which == 0 ? h->d->u.desc() : which == 1 ? h->d->v.desc() : which == 2 ? h->d->w.desc()
+           : which == 3 ? h->d->phi.desc() : h->d->deriv_phidt.desc();
+  } else {
+    *out = which == 0 ? h->f->u.desc() : which == 1 ? h->f->v.desc() : which == 2 ? h->f->w.desc()
+           : which == 3 ? h->f->phi.desc() : h->f->deriv_phidt.desc();
+  }
+  return OCU_OK;
+}
+
+void ocu_scalar_destroy(void *hh) { delete (ScalarHandle *)hh; }
 
 }  // extern "C"

@@ -213,6 +213,10 @@ bool Grid3DDevice<T>::init_external(const ocu_grid3d &d) {
     return false;
   }
   if (this->_owns && this->_buffer) ocu_free(this->_buffer);
+  // nothing is held from here on, so a failed check below leaves an empty grid
+  this->_buffer = nullptr;
+  this->_shifted_buffer = nullptr;
+  this->_owns = false;
   this->compute_layout(d.nx, d.ny, d.nz, d.gx, d.gy, d.gz, d.pnx - (d.nx + 2 * d.gx), d.pny - (d.ny + 2 * d.gy),
                        d.pnz - (d.nz + 2 * d.gz));
   if (this->_shift_amount != d.shift || this->_allocated_elements > d.allocated) {

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, OCU_F32, OCU_F64, dtype_code)
+from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, OCU_F32, OCU_F64, dtype_code, layout)
 from .device import desc_to_host, host_to_desc
 
 
@@ -159,6 +159,25 @@ class ProjectDivergence3D:
             self.h = None
 
 
+_INIT_PADS = ((0, 1, 1), (1, 0, 1), (1, 1, 0), (1, 1, 1))
+
+
+def _check_init_grids(what, params, dt, grids):
+    """The init grids must have the init_grids layouts of params' size
+    (eqn_incompressns3d.h:78-86, eqn_scalaradvection3d.h init_grids) and
+    one element type; the library checks the same and copies whole
+    allocations."""
+    nx, ny, nz = params.nx, params.ny, params.nz
+    dims = ((nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1), (nx, ny, nz))
+    for name, g, d, pad in zip(("u", "v", "w", "scalar"), grids, dims, _INIT_PADS):
+        want = layout(*d, 1, 1, 1, *pad, dtype=dt)
+        got = g.desc
+        fields = ("nx", "ny", "nz", "gx", "gy", "gz", "pnx", "pny", "pnz", "dtype", "shift", "allocated")
+        if any(getattr(got, f) != getattr(want, f) for f in fields):
+            raise ValueError(f"{what}: init grid {name} does not have the init_grids layout for "
+                             f"{(nx, ny, nz)} / dtype {dt}")
+
+
 class IncompressibleNS3D:
     """Eqn_IncompressibleNS3D<T> (ocuequation/eqn_incompressns3d.h:132-175).
 
@@ -168,10 +187,12 @@ class IncompressibleNS3D:
 
     def __init__(self, params, init_u, init_v, init_w, init_temp):
         dt = dtype_code(init_u.buf.dtype)
+        _check_init_grids("IncompressibleNS3D", params, dt, (init_u, init_v, init_w, init_temp))
         self.dtype = dt
         self.params = params
-        self.h = _lib.host().ocu_ns_create(dt, C.byref(params), init_u.buf.ctypes.data, init_v.buf.ctypes.data,
-                                           init_w.buf.ctypes.data, init_temp.buf.ctypes.data)
+        self._init = (init_u, init_v, init_w, init_temp)
+        self.h = _lib.host().ocu_ns_create(dt, C.byref(params), init_u.ptr(), init_v.ptr(), init_w.ptr(),
+                                           init_temp.ptr())
         if not self.h:
             raise _lib.OcuError(f"Eqn_IncompressibleNS3D::set_parameters failed: {_lib.last_error()}")
 
@@ -202,4 +223,43 @@ class IncompressibleNS3D:
     def __del__(self):
         if getattr(self, "h", None):
             _lib.host().ocu_ns_destroy(self.h)
+            self.h = None
+
+
+class ScalarAdvection3D:
+    """Eqn_ScalarAdvection3D<T> (ocuequation/eqn_scalaradvection3d.h,
+    eqn_scalaradvection3d.cpp): forward-Euler passive advection of a scalar
+    by a fixed MAC velocity.  init_u/v/w/phi are HostGrids laid out like
+    Eqn_ScalarAdvection3DParams::init_grids."""
+
+    def __init__(self, params, init_u, init_v, init_w, init_phi):
+        dt = dtype_code(init_u.buf.dtype)
+        _check_init_grids("ScalarAdvection3D", params, dt, (init_u, init_v, init_w, init_phi))
+        self.dtype = dt
+        self.params = params
+        self.h = _lib.host().ocu_scalar_create(dt, C.byref(params), init_u.ptr(), init_v.ptr(), init_w.ptr(),
+                                               init_phi.ptr())
+        if not self.h:
+            raise _lib.OcuError(f"Eqn_ScalarAdvection3D::set_parameters failed: {_lib.last_error()}")
+
+    def advance_one_step(self, dt):
+        return bool(_lib.check(_lib.host().ocu_scalar_advance_one_step(self.h, dt), "advance_one_step"))
+
+    def advance(self, dt):
+        return bool(_lib.check(_lib.host().ocu_scalar_advance(self.h, dt), "advance"))
+
+    def get_max_stable_timestep(self):
+        d = C.c_double()
+        _lib.check(_lib.host().ocu_scalar_max_stable_timestep(self.h, C.byref(d)), "max_stable_timestep")
+        return d.value
+
+    def field(self, which):
+        """0 u, 1 v, 2 w, 3 phi, 4 deriv_phidt, copied to a HostGrid."""
+        d = Grid3DDesc()
+        _lib.check(_lib.host().ocu_scalar_field(self.h, which, C.byref(d)), "field")
+        return desc_to_host(d)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.host().ocu_scalar_destroy(self.h)
             self.h = None

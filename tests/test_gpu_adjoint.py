@@ -188,3 +188,31 @@ def test_torch_autograd_function(gpu):
     assert float(loss.detach()) == l2
     for x, y in zip(v, g2):
         assert torch.equal(x.grad, y)
+
+
+def test_backward_can_be_repeated(gpu):
+    """backward replays the last forward's checkpoints from a kept seed, so
+    a second call (retain_graph) gives the same gradient again"""
+    import torch
+    p = problem(n=(16, 16, 16), steps=6, ce=4)
+    v = [x.clone().requires_grad_(True) for x in velocity(p)]
+    loss = A().advect_project_loss(*v, p)
+    loss.backward(retain_graph=True)
+    first = [x.grad.clone() for x in v]
+    loss.backward()
+    for x, g in zip(v, first):
+        assert torch.equal(x.grad, 2 * g)
+    again = p.backward()[:3]
+    for a, g in zip(again, first):
+        assert torch.equal(a, g)
+
+
+def test_two_losses_on_one_problem_are_refused(gpu):
+    """a loss whose problem ran forward again holds no checkpoints of its own:
+    differentiating it raises instead of returning another input's gradient"""
+    p = problem(n=(16, 16, 16), steps=4, ce=2)
+    v = [x.clone().requires_grad_(True) for x in velocity(p)]
+    l1 = A().advect_project_loss(*v, p)
+    l2 = A().advect_project_loss(*[2 * x for x in v], p)
+    with pytest.raises(RuntimeError, match="ran forward again"):
+        (l1 + l2).backward()

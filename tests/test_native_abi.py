@@ -75,3 +75,25 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_hip", None)
     with pytest.raises(_lib.NativeLibraryMissing):
         _lib.hip()
+
+
+def test_init_grids_of_the_wrong_layout_are_refused_before_the_library():
+    """IncompressibleNS3D / ScalarAdvection3D check their host init grids
+    against the init_grids layouts (eqn_incompressns3d.h:78-86) before the
+    library copies whole allocations from them"""
+    import numpy as np
+    from opencurrent_amd import solvers
+    from opencurrent_amd._lib import NSParams, ScalarParams
+    from opencurrent_amd._types import HostGrid
+    import fields
+    u, v, w, t = fields.mac_grids(8, 8, 8)
+    for P, cls in ((NSParams, solvers.IncompressibleNS3D), (ScalarParams, solvers.ScalarAdvection3D)):
+        p = P()
+        p.nx = p.ny = p.nz = 8
+        with pytest.raises(ValueError):
+            cls(p, HostGrid(8, 8, 8), v, w, t)
+        with pytest.raises(ValueError):
+            cls(p, u, v, w, HostGrid.congruent(t, dtype=np.float32))
+        p.nz = 9
+        with pytest.raises(ValueError):
+            cls(p, u, v, w, t)
