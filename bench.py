@@ -30,12 +30,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "MG time to 1e-10 rel. residual + NS step Mcells/s, FP32-refined vs FP64"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the level-0 fused sweep at 256^3, measured by two
-# rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) and scaled by what the
-# same counters read for a plain copy of known size; recorded with the
-# profile summary profiles/level0_kernels_256_hbm_traffic.md
-SWEEP_TRAFFIC_256 = 241_400_000
-SWEEP_PRO_TRAFFIC_256 = 254_400_000
+# HBM bytes per launch of the level-0 fused sweeps at 256^3 (plain, from the
+# zero grid, adding the prolongation), measured by two rocprofv3 counter
+# passes (FETCH_SIZE, WRITE_SIZE) and scaled by what the same counters read
+# for a plain copy of known size; recorded with the profile summary
+# profiles/level0_kernels_256_hbm_traffic.md
+SWEEP_TRAFFIC_256 = {"plain": 241_400_000, "zero": 152_200_000, "pro": 254_400_000}
 
 
 def parse():
@@ -78,58 +78,64 @@ class Events:
         return ms.value
 
 
-def time_fused_sweep(n, reps, dtype=np.float32):
-    """Average duration of the level-0 fused red+black sweep at n^3 (the
-    dominant kernel of the V-cycle) as the solver launches it - writing the
-    output's ghost layer too - back to back on the solver's stream, timed with
-    HIP events."""
+def time_fused_sweeps(dims, reps, params, dtype=np.float32, variants=("plain", "zero", "pro")):
+    """Average duration of the level-0 fused red+black sweep on a grid of
+    `dims` cells as the solver launches it - writing the output's ghost layer
+    too - back to back on the solver's stream, timed with HIP events, for
+    each kind the solver runs: plain, starting from the deferred zero grid
+    (u not read), and adding the pending prolongation of the coarser level
+    while it reads u.  `params` are the level's relaxation parameters as the
+    solver computes them.  Returns {kind: (ms per launch, algorithmic bytes
+    per launch)}."""
     from opencurrent_amd import _lib
     from opencurrent_amd import _types as T
     from opencurrent_amd.device import DeviceGrid
-    import oracle_lib as O
     hip = _lib.hip()
     stream = _lib.host().ocu_get_compute_stream()
     bc = T.BCSet.uniform(T.BC_NEUMANN)
-    u = DeviceGrid(n, n, n, dtype=dtype)
-    o = DeviceGrid(n, n, n, dtype=dtype)
-    b = DeviceGrid(n, n, n, dtype=dtype)
+    nx, ny, nz = dims
+    u = DeviceGrid(nx, ny, nz, dtype=dtype)
+    o = DeviceGrid(nx, ny, nz, dtype=dtype)
+    b = DeviceGrid(nx, ny, nz, dtype=dtype)
     u.tensor.uniform_(-1, 1)
     b.tensor.uniform_(-1, 1)
-    p = O.level_params(bc, 0, 1.0 / n, 1.0 / n, 1.0 / n)
+    uc = DeviceGrid(nx // 2, ny // 2, nz // 2, dtype=dtype)
+    uc.tensor.uniform_(-1, 1)
     import torch
     torch.cuda.synchronize()
-    args = (u.ptr(), o.ptr(), b.ptr(), C.byref(p), C.byref(bc), 1.0 / n, 1.0 / n, 1.0 / n, 0, T.FUSED_WRITE_GHOSTS,
-            stream)
-    # a plain copy o = 1*u over the whole allocation (k_lincomb_flat<float>:
-    # 4 B read and 4 B written per element, 18 105 423 elements at 256^3):
-    # the reference point that scales the HBM counters
+    h = (params.h, params.h, params.h)
+    cells = nx * ny * nz
+    w = np.dtype(dtype).itemsize
+    launches = {
+        # read u and b, write u, once per cell
+        "plain": (lambda: hip.ocu_mg_relax_fused_bc(u.ptr(), o.ptr(), b.ptr(), C.byref(params), C.byref(bc), *h, 0,
+                                                    T.FUSED_WRITE_GHOSTS, stream), 3 * w * cells),
+        # u is the zero grid and is not read: read b, write u
+        "zero": (lambda: hip.ocu_mg_relax_fused_bc(u.ptr(), o.ptr(), b.ptr(), C.byref(params), C.byref(bc), *h, 0,
+                                                   T.FUSED_WRITE_GHOSTS | T.FUSED_ZERO_INPUT, stream), 2 * w * cells),
+        # + the coarse grid, read once
+        "pro": (lambda: hip.ocu_mg_relax_fused_prolong_bc(u.ptr(), o.ptr(), b.ptr(), uc.ptr(), 1, C.byref(params),
+                                                          C.byref(bc), *h, 0, T.FUSED_WRITE_GHOSTS, stream),
+                3 * w * cells + w * (cells // 8)),
+    }
+    if "pro" in variants and not hip.ocu_mg_relax_fused_prolong_ok(u.ptr(), uc.ptr(), C.byref(bc)):
+        variants = tuple(v for v in variants if v != "pro")
+    # a plain copy o = 1*u over the whole allocation (k_lincomb_flat: 4 B
+    # read and 4 B written per element): the reference point that scales
+    # the HBM counters
     for _ in range(3):
         _lib.check(hip.ocu_grid_lincomb(o.ptr(), 1.0, u.ptr(), 0.0, None, stream), "calibration copy")
-    for _ in range(5):
-        _lib.check(hip.ocu_mg_relax_fused_bc(*args), "fused sweep")
+    out = {}
     ev = Events(hip, stream)
-    ev.start()
-    for _ in range(reps):
-        hip.ocu_mg_relax_fused_bc(*args)
-    ms = ev.stop_ms() / reps
-    w = np.dtype(dtype).itemsize
-    alg_bytes = 3 * w * n ** 3  # read u and b, write u, once per cell
-    # the same sweep adding the prolongation of the (n/2)^3 level, as the
-    # solver runs it after each coarse correction
-    pro = None
-    uc = DeviceGrid(n // 2, n // 2, n // 2, dtype=dtype)
-    uc.tensor.uniform_(-1, 1)
-    if hip.ocu_mg_relax_fused_prolong_ok(u.ptr(), uc.ptr(), C.byref(bc)):
-        pargs = (u.ptr(), o.ptr(), b.ptr(), uc.ptr(), 1, C.byref(p), C.byref(bc), 1.0 / n, 1.0 / n, 1.0 / n, 0,
-                 T.FUSED_WRITE_GHOSTS, stream)
+    for kind in variants:
+        fn, nbytes = launches[kind]
         for _ in range(5):
-            _lib.check(hip.ocu_mg_relax_fused_prolong_bc(*pargs), "fused prolongation sweep")
+            _lib.check(fn(), f"fused sweep ({kind})")
         ev.start()
         for _ in range(reps):
-            hip.ocu_mg_relax_fused_prolong_bc(*pargs)
-        pms = ev.stop_ms() / reps
-        pro = (pms, alg_bytes + w * (n // 2) ** 3)  # + the coarse grid, read once
-    return ms, alg_bytes, pro
+            fn()
+        out[kind] = (ev.stop_ms() / reps, nbytes)
+    return out
 
 
 def time_fp64_mg_64(hip, stream):
@@ -217,6 +223,7 @@ def cpu_ns_step(n, p, init, steps=3):
     import oracle_lib as O
     from opencurrent_amd._types import dtype_code
     u, v, w, t = init
+    cores = O.lib().orc_set_threads(0)
     h = O.lib().orc_ns_create(dtype_code(np.float64), C.byref(p), u.ptr(), v.ptr(), w.ptr(), t.ptr())
     if not h:
         return None
@@ -229,7 +236,7 @@ def cpu_ns_step(n, p, init, steps=3):
     finally:
         O.lib().orc_ns_destroy(h)
     return {"ms_per_step": round(dt * 1e3, 1), "mcells_s": round(n ** 3 / dt / 1e6, 2),
-            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            "cores": cores, "kind": "port",
             "sample": f"{steps} steps after one warm-up"}
 
 
@@ -359,18 +366,38 @@ def time_lockex(n=(512, 512, 256), steps=10):
             "adjoint_device_gb": round(adj_bytes / 1e9, 2), "hbm_gb": 288}
 
 
+def host_cpu():
+    """nproc, the cores this process may run on, and the CPU model (lscpu)"""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
 def cpu_baseline(n, rel_tol, budget_s=25.0):
     """The oracle's FP64 multigrid (reference FMG schedule, OpenMP over
     i-planes) on the host cores, same problem and target, repeated while the
-    budget allows."""
+    budget allows.  Threads: every core this process may use, capped by
+    OMP_NUM_THREADS when the environment sets it (the GPU host allots each
+    GPU's jobs a share of its cores and sets it to that share)."""
     import oracle_lib as O
     from opencurrent_amd import _types as T
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    cpu = host_cpu()
+    cores = cpu["usable"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    cores = O.lib().orc_set_threads(cores)
     b = rhs(n)
     rms = float(np.sqrt((b.interior() ** 2).mean()))
     cfg = T.MGConfig.make(T.BCSet.uniform(T.BC_NEUMANN))
@@ -384,9 +411,9 @@ def cpu_baseline(n, rel_tol, budget_s=25.0):
         if time.perf_counter() - t_all > budget_s * 0.5 or len(times) >= 5:
             break
     t = float(np.median(times))
-    return {"value": round(n ** 3 / t / 1e6, 3), "unit": "Mcells/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": "port",
-            "sample": f"oracle FP64 FMG (reference schedule, OpenMP) on the same {n}^3 problem to "
+    return {"value": round(n ** 3 / t / 1e6, 3), "unit": "Mcells/s", "cores": cores,
+            "kind": "port", "host": cpu,
+            "sample": f"oracle FP64 FMG (reference schedule, OpenMP, {cores} threads) on the same {n}^3 problem to "
                       f"{rel_tol:g} rel. residual, {len(times)} solve(s), median {t:.2f} s, {vc} V-cycles"}
 
 
@@ -436,11 +463,13 @@ def main():
     hip.ocu_stream_synchronize(stream)
     torch.cuda.synchronize()
     ev = Events(hip, stream)
+    counts0 = mx.sweep_counts()
     t0 = time.perf_counter()
     ev.start()
     for _ in range(args.steps):
         res = solve_mixed()
     gpu_ms = ev.stop_ms()
+    counts1 = mx.sweep_counts()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = gpu_ms / args.steps
@@ -500,26 +529,53 @@ def main():
         out["max_rel_err_vs_fp64"] = err
         del mg
 
-    # ---- roofline of the dominant kernel
-    kms, kbytes, pro = time_fused_sweep(n, args.kernel_reps)
+    # ---- roofline: the level-0 FP32 fused sweeps (the smoother).  The
+    # solver runs three kinds; per solve it launches `counts` of each
+    # (measured from its counters over the timed solves), so the dominant
+    # kernel is the kind with the most launch time per solve.
+    counts = [(b - a) / args.steps for a, b in zip(counts0, counts1)]
+    sweeps = time_fused_sweeps((n, n, n), args.kernel_reps, mx.level_params(0))
+    kinds = {"plain": 0, "zero": 1, "pro": 2}
+    names = {"plain": "k_relax_fused_rows<float,4,1,12,false,false,false> (level-0 red+black sweep, ghost writes)",
+             "zero": "k_relax_fused_rows<float,4,1,12,true,false,false> (level-0 sweep from the zero grid)",
+             "pro": "k_relax_fused_rows<float,4,1,12,false,true,false> (level-0 sweep adding the 128^3 "
+                    "prolongation)"}
+    per_solve_us = {k: counts[kinds[k]] * v[0] * 1e3 for k, v in sweeps.items()}
+    dom = max(per_solve_us, key=per_solve_us.get)
+    kms, kbytes = sweeps[dom]
     achieved = kbytes / (kms / 1e3) / 1e9
+    traffic = SWEEP_TRAFFIC_256.get(dom) if n == 256 else None
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4),
-                       "traffic": SWEEP_TRAFFIC_256 if n == 256 else None,
-                       "traffic_source": "rocprofv3 FETCH_SIZE x 1.999 + WRITE_SIZE x 1.000 per launch, scaled "
-                                         "on a plain copy (profiles/level0_kernels_256_hbm_traffic.md)"
-                       if n == 256 else None,
-                       "kernel": "k_relax_fused_rows<float,4,1,12> (level-0 red+black sweep with ghost writes)",
-                       "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes}
-    if pro:
-        # the level-0 sweeps that follow a coarse correction add its
-        # prolongation while they read u (7 of the 18 level-0 sweeps per solve)
-        pa = pro[1] / (pro[0] / 1e3) / 1e9
-        out["roofline"]["prolongation_sweep"] = {
-            "kernel": "k_relax_fused_rows<float,4,1,12,PRO> (level-0 sweep adding the 128^3 prolongation)",
-            "avg_launch_us": round(pro[0] * 1e3, 2), "alg_bytes_per_launch": pro[1], "achieved": round(pa, 1),
-            "frac": round(pa / HBM_PEAK_GBS, 4),
-            "traffic": SWEEP_PRO_TRAFFIC_256 if n == 256 else None}
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                       "traffic_source": "rocprofv3 FETCH_SIZE and WRITE_SIZE per launch (separate passes), each "
+                                         "scaled by what it reads for a plain copy of known size "
+                                         "(profiles/level0_kernels_256_hbm_traffic.md)" if traffic else None,
+                       "kernel": names[dom], "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes,
+                       "launches_per_solve": round(counts[kinds[dom]], 2),
+                       "us_per_solve": round(per_solve_us[dom], 1)}
+    # north_star's smoother figure: all level-0 sweeps of a solve, algorithmic
+    # bytes over time, weighted by how often the solver runs each kind
+    tot_b = sum(counts[kinds[k]] * v[1] for k, v in sweeps.items())
+    tot_s = sum(counts[kinds[k]] * v[0] / 1e3 for k, v in sweeps.items())
+    out["roofline"]["level0_smoother"] = {
+        "frac": round(tot_b / tot_s / 1e9 / HBM_PEAK_GBS, 4) if tot_s else None,
+        "us_per_solve": round(tot_s * 1e6, 1),
+        "kinds": {k: {"launches_per_solve": round(counts[kinds[k]], 2), "avg_launch_us": round(v[0] * 1e3, 2),
+                      "alg_bytes_per_launch": v[1],
+                      "frac": round(v[1] / (v[0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": SWEEP_TRAFFIC_256.get(k) if n == 256 else None}
+                  for k, v in sweeps.items()}}
+    # the same plain sweep on the 512 x 512 x 256 lock-exchange grid, whose
+    # 805 MB working set does not fit the 256 MB last-level cache
+    big = (512, 512, 256)
+    gb_big = device.DeviceGrid(*big, dtype=np.float32)
+    mg_big = solvers.MultigridPressure3D(gb_big, 1.0, 1.0, .5, bc)
+    sb = time_fused_sweeps(big, max(5, args.kernel_reps // 5), mg_big.level_params(0), variants=("plain",))
+    del mg_big, gb_big
+    bms, bbytes = sb["plain"]
+    out["roofline"]["beyond_llc"] = {
+        "kernel": names["plain"] + " on 512x512x256", "avg_launch_us": round(bms * 1e3, 2),
+        "alg_bytes_per_launch": bbytes, "frac": round(bbytes / (bms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     if not args.no_extra:
         for key, fn in (("c1_fp64_mg_64", lambda: time_fp64_mg_64(hip, stream)), ("c3_ns_step", time_ns_step),

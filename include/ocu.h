@@ -60,6 +60,14 @@ int ocu_mg_run_fmg(void *h, int count);
 /* level count, V-cycles of the last call and the level-0 residual history */
 int ocu_mg_stats(void *h, int *num_levels, int *vcycles, double *history, int cap, int *len);
 void ocu_mg_destroy(void *h);
+/* fused red+black sweeps launched at level 0 since the handle was created,
+ * by kind: counts[0] plain, [1] starting from the deferred zero grid, [2]
+ * adding the pending prolongation (profiling: the kinds run at different
+ * speeds); ocu_mgmixed_sweep_counts: those of the FP32 hierarchy */
+int ocu_mg_sweep_counts(void *h, long long *counts);
+/* relaxation parameters of a level as the solver hands them to the kernels
+ * (ocu_mgmixed_get_level_params: the FP32 hierarchy's) */
+int ocu_mg_get_level_params(void *h, int level, ocu_mg_level_params *out);
 
 /* ---- Sol_MultigridPressureMixed3DDeviceD: FP32 V-cycles in FP64 iterative
  * refinement behind the reference's mixed-solver API
@@ -75,6 +83,8 @@ int ocu_mgmixed_solve(void *h, double tol, int max_iter, double *residual);
 int ocu_mgmixed_stats(void *h, int *ir_iterations, int *vcycles, double *rhs_rms, double *history, int cap,
                       int *len);
 void ocu_mgmixed_destroy(void *h);
+int ocu_mgmixed_sweep_counts(void *h, long long *counts);
+int ocu_mgmixed_get_level_params(void *h, int level, ocu_mg_level_params *out);
 
 /* ---- Sol_ProjectDivergence3DDevice<T> (ocuequation/sol_project3d.h:74-93) ----
  * u, v, w: congruent MAC grids (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1).

@@ -151,11 +151,15 @@ HOST_SIGS = {
     "ocu_mg_run_fmg": (I, [P, I]),
     "ocu_mg_stats": (I, [P, IP, IP, DP, I, IP]),
     "ocu_mg_destroy": (None, [P]),
+    "ocu_mg_sweep_counts": (I, [P, C.POINTER(C.c_longlong)]),
+    "ocu_mg_get_level_params": (I, [P, I, C.POINTER(MGLevelParams)]),
     "ocu_mgmixed_create": (P, [GP, D, D, D, CP, I, I, I, I]),
     "ocu_mgmixed_pressure": (I, [P, GP]),
     "ocu_mgmixed_solve": (I, [P, D, I, DP]),
     "ocu_mgmixed_stats": (I, [P, IP, IP, DP, DP, I, IP]),
     "ocu_mgmixed_destroy": (None, [P]),
+    "ocu_mgmixed_sweep_counts": (I, [P, C.POINTER(C.c_longlong)]),
+    "ocu_mgmixed_get_level_params": (I, [P, I, C.POINTER(MGLevelParams)]),
     "ocu_project_create": (P, [GP, GP, GP, BP, I, I, I, D, D, D, I]),
     "ocu_project_solve": (I, [P, D, DP]),
     "ocu_project_solve_divergence_only": (I, [P]),

@@ -181,6 +181,7 @@ __global__ __launch_bounds__(256) void k_relax_host(DGrid<T> u, DGrid<T> b, Host
 // Lateral neighbours go through LDS; the i-neighbours live in registers.
 
 constexpr int FBK = 64, FBJ = 16, FOK = FBK - 4, FOJ = FBJ - 4, FXCD = 8;
+static_assert(NORM_PARTS % FXCD == 0, "norm launches deal NORM_PARTS blocks over the XCDs");
 
 template <typename T, bool ZERO>
 __global__ __launch_bounds__(FBK *FBJ) void k_relax_fused(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
@@ -269,73 +270,6 @@ struct CRow {
   Vec<T, V> v;
   T g;
 };
-
-// A wave's window of the coarse grid for the prolongation of its fine row J:
-// coarse rows jc = J/2 (row 0) and jc +- 1 toward J (row 1) of the coarse
-// planes ic-1, ic, ic+1 (slots 0, 1, 2): VC cells per lane in v, the row's
-// z ghost in g.  Plain arrays indexed by constants only, so the window stays
-// in registers.
-template <typename T, int VC>
-struct CWin {
-  T v[3][2][VC];
-  T g[3][2];
-};
-
-// The prolongation of fine row (I, J) added to f (V fine cells per lane,
-// V >= 2) from a window centred on ic = I/2: the coarse plane toward the
-// fine plane is slot 2 (odd I) or slot 0 (even I); the corner quirk takes
-// the row-1 corner of plane ic+1.  The operations and their order are those
-// of k_prolong_rows, so the sum is the same bit for bit.
-template <typename T, int V, int DX>
-__device__ __forceinline__ void prolong_row_add_dx(int J, int quirk, const CWin<T, V / 2> &w, Vec<T, V> &f) {
-  constexpr int VC = V / 2;
-  constexpr int dx = DX, xs = DX ? 2 : 0;  // the slot toward the fine plane, a constant
-  const int dy = J & 1;
-  T c[2][VC], x[2][VC], cg[2], xg[2];
-#pragma unroll
-  for (int r = 0; r < 2; r++) {
-#pragma unroll
-    for (int k = 0; k < VC; k++) {
-      c[r][k] = w.v[1][r][k];
-      x[r][k] = w.v[xs][r][k];
-    }
-    cg[r] = w.g[1][r];
-    xg[r] = w.g[xs][r];
-  }
-  const bool qrow = quirk && dx == 0 && dy == 1;  // its dz = 1 children take the corner from plane ic+1
-  const T qzh = from_upper(w.v[2][1][0], w.g[2][1]);
-  T zl[2][2], zh[2][2];  // [plane: 0 = c, 1 = x][row]
-#pragma unroll
-  for (int r = 0; r < 2; r++) {
-    zl[0][r] = from_lower(c[r][VC - 1], cg[r]);
-    zh[0][r] = from_upper(c[r][0], cg[r]);
-    zl[1][r] = from_lower(x[r][VC - 1], xg[r]);
-    zh[1][r] = from_upper(x[r][0], xg[r]);
-  }
-#pragma unroll
-  for (int e = 0; e < V; e++) {
-    const int mm = e >> 1, dz = e & 1;
-    // coarse value toward the child in z of a row (its ghost-side values lo, hi)
-    auto zs = [&](const T (&a)[VC], T lo, T hi) {
-      return dz ? (mm + 1 < VC ? a[mm + 1] : hi) : (mm > 0 ? a[mm - 1] : lo);
-    };
-    const T C = c[0][mm], Xv = x[0][mm], Y = c[1][mm];
-    const T Z = zs(c[0], zl[0][0], zh[0][0]);
-    const T XY = x[1][mm];
-    const T XZ = zs(x[0], zl[1][0], zh[1][0]);
-    const T YZ = zs(c[1], zl[0][1], zh[0][1]);
-    const T XYZ = (qrow && dz) ? (mm + 1 < VC ? w.v[2][1][mm + 1] : qzh) : zs(x[1], zl[1][1], zh[1][1]);
-    f.v[e] = f.v[e] + PW<T>::c0 * C + PW<T>::c1 * (Xv + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
-  }
-}
-
-// (a branch on the wave-uniform plane parity keeps every window index a
-// constant, so the window is never addressed and stays in registers)
-template <typename T, int V>
-__device__ __forceinline__ void prolong_row_add(int I, int J, int quirk, const CWin<T, V / 2> &w, Vec<T, V> &f) {
-  if (I & 1) prolong_row_add_dx<T, V, 1>(J, quirk, w, f);
-  else prolong_row_add_dx<T, V, 0>(J, quirk, w, f);
-}
 
 // ------------------------------------------------------------------------
 // fused red+black iteration, whole-row variant
@@ -491,66 +425,50 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
   auto tile = [&](int r) { return t0 + r >= 2 && t0 + r <= NR - 3; };
 
   // PRO: the row as the plain load would clamp it (jj, pp), its interior
-  // source row (jr, pq) and the ghost offset between them.  The coarse rows
-  // come from a per-row window of three coarse planes that slides with the
-  // march (one new plane every two fine planes), loaded before the barrier
-  // and summed after it in pro_finish.
+  // source row (jr, pq) and the ghost offset between them.
+  //
+  // The coarse rows the block's fine rows need (rows cb .. cb+CR-1 of each
+  // coarse plane, with their z ghosts) are staged in LDS, one coarse plane
+  // every two fine planes, in a ring of four planes (slot = plane & 3): wave
+  // w < CR loads coarse row cb+w of the plane the march will need two fine
+  // planes ahead into registers and writes it at the top of the next
+  // iteration, before the first barrier.  A fine row's prolongation then
+  // reads its six coarse row segments (rows jc, jc +- 1 of planes ic, and
+  // ic -+ 1 toward the fine plane) from LDS with their k-neighbours, so no
+  // coarse window sits in registers and no DPP shift is needed.
   constexpr int VC = PRO ? V / 2 : 1;
-  [[maybe_unused]] CWin<T, VC> win[R];
-  [[maybe_unused]] int wic = INT_MIN;  // coarse plane at the window's centre
+  constexpr int CR = PRO ? NR / 2 + 2 : 1;  // coarse rows of a block
+  constexpr int CW = PRO ? 64 * VC + 2 : 1; // a coarse row: k = -1 .. 64*VC
+  static_assert(!PRO || (R == 1 && NW >= CR && (NR - 4) % 2 == 0), "prolongation staging needs a wave per coarse row");
+  __shared__ T sC[PRO ? 4 : 1][CR][CW];
+  const int cb = (jb - t0) / 2 - 1;  // the block's first fine row jb - t0 is even
   auto pro_rows = [&](int r, int p, int &jj, int &pp, int &jr, int &pq) {
     jj = min(max(jb + r, -1), ny);
     pp = min(max(p, -1), nx);
     jr = min(max(jj, 0), ny - 1);
     pq = min(max(pp, 0), nx - 1);
   };
-  // row q (0: jc, 1: jc +- 1) of coarse plane ic into slot `slot` of row r's window
-  auto cload1 = [&](int r, int ic, int q, int slot) {
+  // this wave's coarse row of plane ic into registers (waves w < CR)
+  auto cload = [&](int ic, Vec<T, VC> &d, T &g) {
     if constexpr (PRO) {
-      int jj, pp, jr, pq;
-      pro_rows(r, 0, jj, pp, jr, pq);
-      const int jc = jr >> 1, jc2 = jc + ((jr & 1) ? 1 : -1);
-      const int kcc = active ? lane * VC : 0, gkc = lane == 0 ? -1 : uc.nz;
-      const bool gin = lane == hi_lane && nz < 64 * V;
-      const T *row = uc.p + (int64_t)ic * uc.xs + (int64_t)(q ? jc2 : jc) * uc.ys;
-      const Vec<T, VC> d = *reinterpret_cast<const Vec<T, VC> *>(row + kcc);
-      const T g = row[gkc];
-#pragma unroll
-      for (int k = 0; k < VC; k++) win[r].v[slot][q][k] = (k == 0 && gin) ? g : d.v[k];
-      win[r].g[slot][q] = g;
+      const int row = min(max(cb + w, -1), uc.ny);
+      const T *src = uc.p + (int64_t)min(max(ic, -1), uc.nx) * uc.xs + (int64_t)row * uc.ys;
+      const int kcc = lane * VC < uc.nz ? lane * VC : 0;
+      d = *reinterpret_cast<const Vec<T, VC> *>(src + kcc);
+      g = src[lane == 0 ? -1 : uc.nz];
     }
   };
-  // centre the window on coarse plane ic (ic never decreases along the march)
-  auto wcentre = [&](int ic) {
+  auto cstore = [&](int ic, const Vec<T, VC> &d, T g) {
     if constexpr (PRO) {
-      if (ic == wic) return;
-      if (ic == wic + 1) {
+      if (w < CR) {
+        T *dst = &sC[ic & 3][w][0];
+        if (lane * VC < uc.nz) {
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-#pragma unroll
-          for (int q = 0; q < 2; q++) {
-#pragma unroll
-            for (int k = 0; k < VC; k++) {
-              win[r].v[0][q][k] = win[r].v[1][q][k];
-              win[r].v[1][q][k] = win[r].v[2][q][k];
-            }
-            win[r].g[0][q] = win[r].g[1][q];
-            win[r].g[1][q] = win[r].g[2][q];
-            cload1(r, ic + 1, q, 2);
-          }
+          for (int m = 0; m < VC; m++) dst[1 + lane * VC + m] = d.v[m];
         }
-      } else {
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-#pragma unroll
-          for (int q = 0; q < 2; q++) {
-            cload1(r, ic - 1, q, 0);
-            cload1(r, ic, q, 1);
-            cload1(r, ic + 1, q, 2);
-          }
-        }
+        if (lane == 0) dst[0] = g;
+        if (lane == hi_lane) dst[1 + uc.nz] = g;
       }
-      wic = ic;
     }
   };
   auto pro_load = [&](int r, int p, VT &f) {
@@ -560,11 +478,43 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       f = *reinterpret_cast<const VT *>(uin.p + (int64_t)pq * uin.xs + (int64_t)jr * uin.ys + kc);
     }
   };
+  // f += prolongation of the coarse correction at fine row (pq, jr): the
+  // operations and their order are those of k_prolong_rows (and k_prolong),
+  // so the sum is the same bit for bit
+  auto pro_add = [&](int pq, int jr, VT &f) {
+    if constexpr (PRO) {
+      const int ic = pq >> 1, dx = pq & 1, dy = jr & 1;
+      const int jc = jr >> 1, jc2 = jc + (dy ? 1 : -1);
+      const int xc = dx ? ic + 1 : ic - 1;  // the coarse plane toward the fine plane
+      // coarse cells k = lane*VC-1 .. lane*VC+VC of a row: a[m+1] is cell lane*VC+m
+      auto seg = [&](int plane, int row, T (&a)[VC + 2]) {
+        const T *src = &sC[plane & 3][row - cb][lane * VC];
+#pragma unroll
+        for (int m = 0; m < VC + 2; m++) a[m] = src[m];
+      };
+      T c0[VC + 2], c1[VC + 2], x0[VC + 2], x1[VC + 2], q1[VC + 2];
+      seg(ic, jc, c0);
+      seg(ic, jc2, c1);
+      seg(xc, jc, x0);
+      seg(xc, jc2, x1);
+      const bool qrow = quirk && dx == 0 && dy == 1;  // its dz = 1 children take the corner from plane ic+1
+      if (qrow) seg(ic + 1, jc2, q1);
+#pragma unroll
+      for (int e = 0; e < V; e++) {
+        const int mm = e >> 1, dz = e & 1;
+        const int zt = dz ? mm + 2 : mm;  // the coarse cell toward the child in z
+        const T C = c0[mm + 1], Xv = x0[mm + 1], Y = c1[mm + 1];
+        const T Z = c0[zt], XY = x1[mm + 1], XZ = x0[zt], YZ = c1[zt];
+        const T XYZ = (qrow && dz) ? q1[zt] : x1[zt];
+        f.v[e] = f.v[e] + PW<T>::c0 * C + PW<T>::c1 * (Xv + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+      }
+    }
+  };
   auto pro_finish = [&](int r, int p, VT f, VT &d, T &g) {
     if constexpr (PRO) {
       int jj, pp, jr, pq;
       pro_rows(r, p, jj, pp, jr, pq);
-      prolong_row_add<T, V>(pq, jr, quirk, win[r], f);
+      pro_add(pq, jr, f);
       if (pp != pq || jj != jr) {
 #pragma unroll
         for (int e = 0; e < V; e++) f.v[e] = gw.chain(pp - pq, jj - jr, 0, f.v[e]);
@@ -575,18 +525,13 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       if (g_in_data) d.v[0] = g;
     }
   };
-  // the coarse plane whose window a fine plane p needs
+  // the coarse plane whose neighbourhood a fine plane p needs
   auto pro_ic = [&](int p) { return min(max(min(max(p, -1), nx), 0), nx - 1) >> 1; };
   auto load = [&](int r, int p, VT &d, T &g) {
     if constexpr (ZERO) {  // homogeneous faces: every ghost of the zero grid is +0.0
 #pragma unroll
       for (int e = 0; e < V; e++) d.v[e] = (T)0;
       g = (T)0;
-    } else if constexpr (PRO) {
-      VT f;
-      wcentre(pro_ic(p));
-      pro_load(r, p, f);
-      pro_finish(r, p, f, d, g);
     } else {
       const T *row = urow[r] + (int64_t)plane_u(p) * uin.xs;
       d = *reinterpret_cast<const VT *>(row + kc);
@@ -604,34 +549,93 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
   // which U1 holds unchanged, so U0 of plane p-1 is not kept: u1m1 stands in
   // for it (and starts as U0 of plane i0-2).  Registers stay under 80 so two
   // blocks share a CU and each hides the other's barriers.
-  VT cc[R], n[R], u1m2[R], u1m1[R], u1[R], bprev[R], bcur[R];
+  //
+  // The loads run two planes ahead: iteration p issues u of plane p+2 and b
+  // of plane p+1 and computes with the planes loaded one iteration earlier,
+  // so a wave waits on memory only when an iteration is shorter than the
+  // load latency (with the loads one plane ahead every iteration waited for
+  // a full round trip).  The prefetch past the march's last plane (i1+1
+  // for u, i1 for b) re-reads that plane.  PRO: the raw rows of plane p+2
+  // and its coarse plane are in flight during iteration p; the row is
+  // completed (prolongation added, ghost transforms) after the first
+  // barrier of iteration p+1, from LDS.
+#ifndef OCU_SWEEP_AHEAD
+#define OCU_SWEEP_AHEAD 2
+#endif
+  // loads of the sweeps without the prolongation run this many planes ahead
+  // (1 or 2; the macro exists for tuning builds)
+  constexpr bool AHEAD2 = PRO || OCU_SWEEP_AHEAD == 2;
+  VT cc[R], n[R], nn[R], u1m2[R], u1m1[R], u1[R], bprev[R], bcur[R], bnext[R];
   T ga[R], gc[R], gn[R];
+  [[maybe_unused]] T gnn[R];
+  [[maybe_unused]] Vec<T, VC> cpend;  // PRO: the coarse row loaded last iteration
+  [[maybe_unused]] T cpend_g = 0;
+  [[maybe_unused]] int clast = 0, cpend_ic = INT_MIN;  // last coarse plane loaded / waiting for its store
+  if constexpr (PRO) {
+    // coarse planes for fine planes i0-2 .. i0 (at most four), then the rows
+    // of those three planes
+    const int lo = pro_ic(i0 - 2) - 1;
+    clast = pro_ic(i0) + 1;
+    for (int ic = lo; ic <= clast; ++ic) {
+      Vec<T, VC> d;
+      T g;
+      cload(ic, d, g);
+      cstore(ic, d, g);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      VT f;
+      pro_load(r, i0 - 2, f);
+      pro_finish(r, i0 - 2, f, u1m1[r], ga[r]);
+      pro_load(r, i0 - 1, f);
+      pro_finish(r, i0 - 1, f, cc[r], gc[r]);
+      pro_load(r, i0, nn[r]);  // raw; completed in the first iteration
+    }
+  }
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    load(r, i0 - 2, u1m1[r], ga[r]);
-    load(r, i0 - 1, cc[r], gc[r]);
+    if constexpr (!PRO) {
+      load(r, i0 - 2, u1m1[r], ga[r]);
+      load(r, i0 - 1, cc[r], gc[r]);
+      if constexpr (AHEAD2) load(r, i0, n[r], gn[r]);
+    }
+    if constexpr (AHEAD2) bcur[r] = loadb(r, i0 - 1);
 #pragma unroll
     for (int e = 0; e < V; e++) u1m2[r].v[e] = bprev[r].v[e] = (T)0;
   }
 
   for (int p = i0 - 1; p <= i1; ++p) {
     const bool p_in = p >= 0 && p < nx;
-    // PRO: the rows of plane p+1 (and a new coarse plane every other
-    // iteration) are loaded before the barrier and summed after it, so the
-    // barrier hides their latency as it hides the plain load's
-    [[maybe_unused]] VT nraw[R];
-    if constexpr (PRO) wcentre(pro_ic(p + 1));
+    const int pa = min(p + 2, i1 + 1);  // the plane loaded ahead
+    if constexpr (PRO) {
+      if (cpend_ic != INT_MIN) cstore(cpend_ic, cpend, cpend_g);
+      cpend_ic = INT_MIN;
+      if (pro_ic(pa) + 1 > clast) {  // uniform: a new coarse plane every other fine plane
+        clast = pro_ic(pa) + 1;
+        cload(clast, cpend, cpend_g);
+        cpend_ic = clast;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      if constexpr (PRO) pro_load(r, p + 1, nraw[r]);
-      else load(r, p + 1, n[r], gn[r]);
-      bcur[r] = loadb(r, p);
+      if constexpr (PRO) {
+        VT raw = nn[r];  // plane p+1, completed after the barrier
+        pro_load(r, pa, nn[r]);
+        n[r] = raw;
+      } else if constexpr (AHEAD2) {
+        load(r, pa, nn[r], gnn[r]);
+      } else {
+        load(r, p + 1, n[r], gn[r]);
+      }
+      if constexpr (AHEAD2) bnext[r] = loadb(r, min(p + 1, i1));
+      else bcur[r] = loadb(r, p);
       sU0[t0 + r][lane] = cc[r];
     }
     __syncthreads();
     if constexpr (PRO) {
 #pragma unroll
-      for (int r = 0; r < R; r++) pro_finish(r, p + 1, nraw[r], n[r], gn[r]);
+      for (int r = 0; r < R; r++) pro_finish(r, p + 1, n[r], n[r], gn[r]);
     }
 
     // first colour on plane p (rows of ring 1); j-neighbours inside the
@@ -682,6 +686,10 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       cc[r] = n[r]; gc[r] = gn[r];
       u1m2[r] = u1m1[r]; u1m1[r] = u1[r];
       bprev[r] = bcur[r];
+      if constexpr (AHEAD2) bcur[r] = bnext[r];
+      if constexpr (!PRO && AHEAD2) {
+        n[r] = nn[r]; gn[r] = gnn[r];
+      }
     }
   }
   if (!wghost || !active || (i0 != 0 && i1 != nx)) return;
@@ -827,18 +835,33 @@ __device__ __forceinline__ T res_val(const ResC<T> &c, T b, T zm, T zp, T ym, T 
   return b - (c.fz * (zm + zp) + c.fy * (ym + yp) + c.fx * (xm + xp) - c.diag * uc);
 }
 
-template <typename T, int V, int NW>
+//
+// NORM: the fine residuals' (sum r^2, max|r|) in double, accumulated by each
+// lane in march order, reduced over the wave (fixed butterfly) and over the
+// block's waves in wave order, go to parts[logical tile]; the launch has
+// exactly NORM_PARTS blocks and the blocks past the last tile write zeros, so
+// the partials need no clearing and the result does not depend on timing.
+template <typename T, int V, int NW, bool NORM>
 __global__ __launch_bounds__(64 * NW) void k_res_restrict_rows(DGrid<T> u, DGrid<T> b, DGrid<T> bc, ResC<T> c,
-                                                                int tiles_j, int ichunk, int ntiles) {
+                                                                int tiles_j, int ichunk, int ntiles, double *parts) {
   typedef Vec<T, V> VT;
   typedef Vec<T, V / 2> VC;
   const int per = gridDim.x / FXCD;
   const int logical = (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD);
-  if (logical >= ntiles) return;  // uniform per block
+  if (logical >= ntiles) {  // uniform per block
+    if (NORM && threadIdx.x == 0 && threadIdx.y == 0) {
+      parts[2 * logical] = 0.0;
+      parts[2 * logical + 1] = 0.0;
+    }
+    return;
+  }
   const int tj = logical % tiles_j, ti = logical / tiles_j;
   const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int jc = tj * NW + w;
-  if (jc >= bc.ny) return;  // uniform per wave; no barriers follow
+  double s2 = 0.0, mx = 0.0;
+  // a wave past the last coarse row does no work (uniform per wave); without
+  // NORM it leaves, with NORM it still joins the block's reduction
+  if (jc < bc.ny) {
   const int nz = u.nz;
   const int k0 = lane * V;
   const bool active = k0 < nz;
@@ -884,6 +907,14 @@ __global__ __launch_bounds__(64 * NW) void k_res_restrict_rows(DGrid<T> u, DGrid
       for (int e = 0; e < V; e++)
         rr[q].v[e] = res_val(c, bb[q].v[e], e > 0 ? cur[q].v[e - 1] : lo, e < V - 1 ? cur[q].v[e + 1] : hi, ym.v[e],
                              yp.v[e], prev[q].v[e], nxt[q].v[e], cur[q].v[e]);
+      if (NORM && active) {
+#pragma unroll
+        for (int e = 0; e < V; e++) {
+          const double x = (double)rr[q].v[e];
+          s2 += x * x;
+          mx = fmax(mx, fabs(x));
+        }
+      }
     }
     if (P & 1) {
       VC o;
@@ -901,6 +932,28 @@ __global__ __launch_bounds__(64 * NW) void k_res_restrict_rows(DGrid<T> u, DGrid
       prev[q] = cur[q];
       cur[q] = nxt[q];
       gc[q] = gn[q];
+    }
+  }
+  } else if (!NORM) {
+    return;
+  }
+  if constexpr (NORM) {
+    __shared__ double sh[2][NW];
+    s2 = wave_sum(s2);
+    mx = wave_max(mx);
+    if (lane == 0) {
+      sh[0][w] = s2;
+      sh[1][w] = mx;
+    }
+    __syncthreads();
+    if (w == 0 && lane == 0) {
+      double a = 0.0, m = 0.0;
+      for (int q = 0; q < NW; q++) {
+        a += sh[0][q];
+        m = fmax(m, sh[1][q]);
+      }
+      parts[2 * logical] = a;
+      parts[2 * logical + 1] = m;
     }
   }
 }
@@ -1790,23 +1843,31 @@ int res_row_width(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc
 
 template <int V, typename T>
 int launch_res_restrict_rows_t(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
-                               double fy, double fz, hipStream_t stream) {
+                               double fy, double fz, double *norm_ws, hipStream_t stream) {
   constexpr int NW = 8;
   const int tiles_j = (bc->ny + NW - 1) / NW;
-  // about 4096 waves in all: enough to fill the chip, long marches
-  const int ichunk = std::max(1, std::min(bc->nx, (int)(((int64_t)bc->nx * bc->ny) / 4096)));
+  // about 4096 waves in all: enough to fill the chip, long marches; with
+  // norm partials at most NORM_PARTS tiles (one partial pair per tile)
+  int ichunk = std::max(1, std::min(bc->nx, (int)(((int64_t)bc->nx * bc->ny) / 4096)));
+  if (norm_ws) ichunk = std::max(ichunk, (int)(((int64_t)tiles_j * bc->nx + NORM_PARTS - 1) / NORM_PARTS));
   const int ntiles = tiles_j * ((bc->nx + ichunk - 1) / ichunk);
-  const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
-  k_res_restrict_rows<T, V, NW><<<nblocks, dim3(64, NW), 0, stream>>>(dgrid<T>(u), dgrid<T>(b), dgrid<T>(bc),
-                                                                      res_c<T>(h, fx, fy, fz), tiles_j, ichunk,
-                                                                      ntiles);
+  const ResC<T> c = res_c<T>(h, fx, fy, fz);
+  if (norm_ws) {
+    // NORM_PARTS is a multiple of FXCD and ntiles <= NORM_PARTS
+    k_res_restrict_rows<T, V, NW, true><<<NORM_PARTS, dim3(64, NW), 0, stream>>>(
+        dgrid<T>(u), dgrid<T>(b), dgrid<T>(bc), c, tiles_j, ichunk, ntiles, norm_ws);
+  } else {
+    const int nblocks = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+    k_res_restrict_rows<T, V, NW, false><<<nblocks, dim3(64, NW), 0, stream>>>(
+        dgrid<T>(u), dgrid<T>(b), dgrid<T>(bc), c, tiles_j, ichunk, ntiles, nullptr);
+  }
   OCU_HIP_LAUNCHED("ocu_mg_residual_restrict");
 }
 template <int V>
 int launch_res_restrict_rows(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_grid3d *bc, double h, double fx,
-                             double fy, double fz, hipStream_t stream) {
-  return u->dtype == OCU_F64 ? launch_res_restrict_rows_t<V, double>(u, b, bc, h, fx, fy, fz, stream)
-                             : launch_res_restrict_rows_t<V, float>(u, b, bc, h, fx, fy, fz, stream);
+                             double fy, double fz, double *norm_ws, hipStream_t stream) {
+  return u->dtype == OCU_F64 ? launch_res_restrict_rows_t<V, double>(u, b, bc, h, fx, fy, fz, norm_ws, stream)
+                             : launch_res_restrict_rows_t<V, float>(u, b, bc, h, fx, fy, fz, norm_ws, stream);
 }
 
 template <typename T, int V>
@@ -2174,10 +2235,10 @@ int ocu_mg_residual_restrict(const ocu_grid3d *u, const ocu_grid3d *b, const ocu
   }
   const int cnx = b_coarse->nx, cny = b_coarse->ny, cnz = b_coarse->nz;
   if ((int64_t)cnx * cny * cnz == 0) return OCU_OK;
-  if (!norm_ws && !r) {
+  if (!r) {
     switch (res_row_width(u, b, b_coarse)) {
-      case 2: return launch_res_restrict_rows<2>(u, b, b_coarse, h, fx, fy, fz, S(stream));
-      case 4: return launch_res_restrict_rows<4>(u, b, b_coarse, h, fx, fy, fz, S(stream));
+      case 2: return launch_res_restrict_rows<2>(u, b, b_coarse, h, fx, fy, fz, norm_ws, S(stream));
+      case 4: return launch_res_restrict_rows<4>(u, b, b_coarse, h, fx, fy, fz, norm_ws, S(stream));
       default: break;
     }
   }

@@ -235,6 +235,21 @@ int ocu_mg_stats(void *hh, int *num_levels, int *vcycles, double *history, int c
 
 void ocu_mg_destroy(void *hh) { delete (MGHandle *)hh; }
 
+int ocu_mg_get_level_params(void *hh, int level, ocu_mg_level_params *out) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return (h->d ? h->d->get_level_params(level, out) : h->f->get_level_params(level, out)) ? OCU_OK
+                                                                                          : OCU_ERR_INVALID;
+}
+
+int ocu_mg_sweep_counts(void *hh, long long *counts) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h || !counts) return OCU_ERR_INVALID;
+  const long long *c = h->d ? h->d->level0_sweeps : h->f->level0_sweeps;
+  for (int q = 0; q < 3; q++) counts[q] = c[q];
+  return OCU_OK;
+}
+
 // ------------------------------------------------------------------ mixed IR
 
 void *ocu_mgmixed_create(const ocu_grid3d *rhs, double hx, double hy, double hz, const ocu_mg_config *cfg,
@@ -285,6 +300,20 @@ int ocu_mgmixed_stats(void *hh, int *ir_iterations, int *vcycles, double *rhs_rm
 }
 
 void ocu_mgmixed_destroy(void *hh) { delete (MixedHandle *)hh; }
+
+int ocu_mgmixed_get_level_params(void *hh, int level, ocu_mg_level_params *out) {
+  MixedHandle *h = (MixedHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return h->s.get_level_params(level, out) ? OCU_OK : OCU_ERR_INVALID;
+}
+
+int ocu_mgmixed_sweep_counts(void *hh, long long *counts) {
+  MixedHandle *h = (MixedHandle *)hh;
+  if (!h || !counts) return OCU_ERR_INVALID;
+  const long long *c = h->s.level0_sweeps();
+  for (int q = 0; q < 3; q++) counts[q] = c[q];
+  return OCU_OK;
+}
 
 // ---------------------------------------------------------------- projection
 

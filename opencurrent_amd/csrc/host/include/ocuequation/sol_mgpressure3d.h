@@ -173,6 +173,10 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   // with use_fused_relax: a prolongation is added by the fused relaxation
   // that follows it instead of a launch of its own
   bool use_fused_prolong;
+  // fused red+black sweeps launched at level 0 since construction: [0]
+  // plain, [1] starting from the deferred zero grid, [2] adding the pending
+  // prolongation (for profiling: the three run at different speeds)
+  long long level0_sweeps[3];
 
   Sol_MultigridPressure3DDevice();
   ~Sol_MultigridPressure3DDevice();
@@ -182,6 +186,14 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
 
   Grid3DDevice<T> &pressure() { return _pressure; }
   const Grid3DDevice<T> &pressure() const { return _pressure; }
+
+  // the relaxation parameters of a level (h, f, omega, diagonal mods) as
+  // the kernels receive them; false for a level that does not exist
+  bool get_level_params(int level, ocu_mg_level_params *out) const {
+    if (level < 0 || level >= _num_levels || !out) return false;
+    level_params(level, out);
+    return true;
+  }
 
   const Grid3DDevice<T> &read_u(int level) const { return *_u_grid[level]; }
   const Grid3DDevice<T> &read_r(int level) const { return *_r_grid[level]; }

@@ -66,6 +66,9 @@ class Sol_MultigridPressureMixed3DDeviceD : public Sol_MultigridPressure3DBase {
 
   Grid3DDevice<double> &pressure() { return _x; }
   const Grid3DDevice<double> &pressure() const { return _x; }
+  // level-0 fused sweeps of the FP32 hierarchy (Sol_MultigridPressure3DDevice::level0_sweeps)
+  const long long *level0_sweeps() const { return _mg_f.level0_sweeps; }
+  bool get_level_params(int level, ocu_mg_level_params *out) const { return _mg_f.get_level_params(level, out); }
 };
 
 }  // namespace ocu

@@ -307,6 +307,7 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   use_bottom = true;
   _bottom_level = -1;
   use_fused_prolong = true;
+  level0_sweeps[0] = level0_sweeps[1] = level0_sweeps[2] = 0;
   for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = _u_pro[l] = _fused_lvl[l] = false;
 }
 
@@ -551,6 +552,8 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
     const double lhx = hx(level), lhy = hy(level), lhz = hz(level);
     int it = 0;
     for (; it + 1 < iterations; it += 2) {
+      if (level == 0) level0_sweeps[_u_pro[0] ? 2 : _u_zero[0] ? 1 : 0]++;
+      if (level == 0) level0_sweeps[0]++;  // the second sweep of the pair
       if (_u_pro[level]) {
         _u_pro[level] = false;
         ocu_grid3d dc = get_u(level + 1).desc();

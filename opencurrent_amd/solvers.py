@@ -13,7 +13,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, OCU_F32, OCU_F64, dtype_code, layout)
+from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, MGLevelParams, OCU_F32, OCU_F64, dtype_code,
+                     layout)
 from .device import desc_to_host, host_to_desc
 
 
@@ -72,6 +73,18 @@ class MultigridPressure3D:
         _lib.check(_lib.host().ocu_mg_stats(self.h, C.byref(nl), C.byref(vc), hist, 4096, C.byref(n)), "stats")
         return {"num_levels": nl.value, "vcycles": vc.value, "history": [hist[i] for i in range(min(n.value, 4096))]}
 
+    def level_params(self, level):
+        """MGLevelParams of a level as the solver passes them to the kernels"""
+        p = MGLevelParams()
+        _lib.check(_lib.host().ocu_mg_get_level_params(self.h, level, C.byref(p)), "level_params")
+        return p
+
+    def sweep_counts(self):
+        """level-0 fused sweeps so far: (plain, from the zero grid, adding a prolongation)"""
+        c = (C.c_longlong * 3)()
+        _lib.check(_lib.host().ocu_mg_sweep_counts(self.h, c), "sweep_counts")
+        return tuple(c)
+
     def __del__(self):
         if getattr(self, "h", None):
             _lib.host().ocu_mg_destroy(self.h)
@@ -117,6 +130,19 @@ class MultigridPressureMixed3D:
                                                  C.byref(n)), "stats")
         return {"ir_iterations": it.value, "vcycles": vc.value, "rhs_rms": rms.value,
                 "history": [hist[i] for i in range(min(n.value, 4096))]}
+
+    def level_params(self, level):
+        """MGLevelParams of a level of the FP32 hierarchy"""
+        p = MGLevelParams()
+        _lib.check(_lib.host().ocu_mgmixed_get_level_params(self.h, level, C.byref(p)), "level_params")
+        return p
+
+    def sweep_counts(self):
+        """level-0 fused sweeps of the FP32 hierarchy so far: (plain, from
+        the zero grid, adding a prolongation)"""
+        c = (C.c_longlong * 3)()
+        _lib.check(_lib.host().ocu_mgmixed_sweep_counts(self.h, c), "sweep_counts")
+        return tuple(c)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -4,6 +4,8 @@
  */
 #include "ocu_oracle.h"
 
+#include <omp.h>
+
 #include "oracle_ns.h"
 
 using namespace orc;
@@ -55,6 +57,13 @@ int orc_copy_convert(const ocu_grid3d *dst, const ocu_grid3d *src) {
 
 double orc_reduce(const ocu_grid3d *g, int op) {
   return g->dtype == OCU_F64 ? reduce<double>(*g, op) : reduce<float>(*g, op);
+}
+
+/* OpenMP threads of the oracle's parallel loops (the CPU baseline's thread
+ * count); returns the count now in effect */
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
 }
 
 int orc_mg_num_levels(int nx, int ny, int nz) { return num_levels(nx, ny, nz); }

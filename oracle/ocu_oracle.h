@@ -20,6 +20,7 @@ int orc_layout(int nx, int ny, int nz, int gx, int gy, int gz, int padx, int pad
 int orc_bc_fill(int kind, const ocu_grid3d *g, const ocu_bc_set *bc, double hx, double hy, double hz);
 int orc_bc_fill_mac(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc);
 
+int orc_set_threads(int n);
 int orc_lincomb1(const ocu_grid3d *r, double a1, const ocu_grid3d *g1);
 int orc_lincomb2(const ocu_grid3d *r, double a1, const ocu_grid3d *g1, double a2, const ocu_grid3d *g2);
 int orc_copy_convert(const ocu_grid3d *dst, const ocu_grid3d *src);

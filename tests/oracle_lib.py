@@ -22,6 +22,7 @@ IP = C.POINTER(C.c_int)
 
 SIGS = {
     "orc_layout": (I, [I, I, I, I, I, I, I, I, I, I, GP]),
+    "orc_set_threads": (I, [I]),
     "orc_bc_fill": (I, [I, GP, BP, D, D, D]),
     "orc_bc_fill_mac": (I, [GP, GP, GP, BP]),
     "orc_lincomb1": (I, [GP, D, GP]),
