@@ -86,6 +86,22 @@ void ocu_mgmixed_destroy(void *h);
 int ocu_mgmixed_sweep_counts(void *h, long long *counts);
 int ocu_mgmixed_get_level_params(void *h, int level, ocu_mg_level_params *out);
 
+/* ---- Sol_PCGPressure3DDevice<T> (ocuequation/sol_pcgpressure3d.h:41-101) ----
+ * T follows rhs->dtype; rhs and coeff are the caller's congruent device grids
+ * (the coefficient's ghosts are filled in place by each solve, as the
+ * reference does).  initialize_storage(nx,ny,nz,hx,hy,hz,&rhs,&coeff) with
+ * the public fields bc (periodic or Neumann faces), convergence,
+ * preconditioner (0 none, 1 Jacobi, 2 multigrid, 3 BFBT), multigrid_cycles
+ * and multigrid_use_fmg.  Each solve starts from the current pressure. */
+void *ocu_pcg_create(const ocu_grid3d *rhs, const ocu_grid3d *coeff, float hx, float hy, float hz,
+                     const ocu_bc_set *bc, int convergence, int preconditioner, int multigrid_cycles,
+                     int multigrid_use_fmg);
+/* solve(residual, tol, max_iter) (sol_pcgpressure3d.cu:550-597) */
+int ocu_pcg_solve(void *h, double tol, int max_iter, double *residual);
+int ocu_pcg_iterations(void *h);
+int ocu_pcg_pressure(void *h, ocu_grid3d *out);
+void ocu_pcg_destroy(void *h);
+
 /* ---- Sol_ProjectDivergence3DDevice<T> (ocuequation/sol_project3d.h:74-93) ----
  * u, v, w: congruent MAC grids (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1).
  * mixed = 1 (FP64 u) uses the mixed-precision pressure solver. */
@@ -110,6 +126,10 @@ typedef struct ocu_ns_params {
   /* not in the reference: FP64 equations projected with FP32 V-cycles in
    * FP64 iterative refinement (dtype OCU_F64 only) */
   int32_t mixed_projection;
+  /* not in the reference: 1 runs the reference's operator-by-operator step;
+   * 0 (default) runs the explicit part as one fused launch where it applies
+   * (same values; ocu_ns_step_fused in ocu_hip.h) */
+  int32_t unfused_step;
 } ocu_ns_params;
 
 /* init_* describe host buffers (base = host pointer) laid out by
@@ -121,7 +141,9 @@ void *ocu_ns_create(int dtype, const ocu_ns_params *p, const ocu_grid3d *init_u,
 int ocu_ns_advance_one_step(void *h, double dt);
 int ocu_ns_advance(void *h, double dt);
 int ocu_ns_max_stable_timestep(void *h, double *dt);
-/* which: 0 u, 1 v, 2 w, 3 temperature, 4 pressure (device descriptors) */
+/* which: 0 u, 1 v, 2 w, 3 temperature, 4 pressure (device descriptors).
+ * A descriptor stays valid until the next step: the fused step writes the
+ * new u, v, w, temperature into a second set of buffers and swaps them in. */
 int ocu_ns_field(void *h, int which, ocu_grid3d *out);
 int ocu_ns_num_steps(void *h);
 void ocu_ns_destroy(void *h);

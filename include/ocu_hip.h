@@ -348,6 +348,52 @@ int ocu_laplacian_acc(const ocu_grid3d *phi, const ocu_grid3d *dphi, int nx, int
 int ocu_thermal_force(const ocu_grid3d *d, const ocu_grid3d *t, int nx, int ny, int nz,
                       int vertical_direction, double coefficient, void *stream);
 
+/* ---------------- PCG pressure solver (ocuequation/sol_pcgpressure3d.cu) -- */
+
+/* Sol_PCGPressure3DDevice_apply_laplacian (sol_pcgpressure3d.cu:88-126):
+ * dst = -div(c grad src) on the interior with face coefficients
+ * (c_i + c_nb)/2, from src's and coeff's ghosts as they are (the caller
+ * fills them).  f*_div_hsq are the T values f/h^2 (passed as double, as the
+ * reference's kernel parameters).  When dot_out is given, also
+ * *dot_out = sum over the interior of (T)(src * dst) in double (device
+ * memory; workspace holds ocu_reduce_workspace_bytes()), the quantity of
+ * invoke_kernel_dot_product(src, dst) (:262-293).  dst must not alias src. */
+int ocu_pcg_apply_laplacian(const ocu_grid3d *dst, const ocu_grid3d *src, const ocu_grid3d *coeff,
+                            double fx_div_hsq, double fy_div_hsq, double fz_div_hsq, void *workspace,
+                            double *dot_out, void *stream);
+/* Sol_PCGPressure3DDevice_diag_preconditioner (sol_pcgpressure3d.cu:45-83):
+ * dst = src / (sum over the six faces of f/h^2 (c_i + c_nb)/2); dst may be
+ * src. */
+int ocu_pcg_diag_precond(const ocu_grid3d *dst, const ocu_grid3d *src, const ocu_grid3d *coeff,
+                         double fx_div_hsq, double fy_div_hsq, double fz_div_hsq, void *stream);
+/* *out = sum over the interior of (T)(a * b), accumulated in double;
+ * invoke_kernel_dot_product (sol_pcgpressure3d.cu:262-293) without its
+ * temporary grid.  workspace: ocu_reduce_workspace_bytes(). */
+int ocu_grid_dot(const ocu_grid3d *a, const ocu_grid3d *b, void *workspace, double *out, void *stream);
+
+/* The explicit part of Eqn_IncompressibleNS3D<T>::advance_one_step
+ * (eqn_incompressns3d.cu:331-393) in one launch: self-advection, viscous
+ * diffusion and thermal force of u, v, w, passive advection and diffusion
+ * of the temperature, the forward-Euler (or Adams-Bashforth) combination
+ * linear_combination(1, x, a_dt, dx) [+ linear_combination(1, x, a_ab,
+ * last_dx)] with the reference's whole-allocation / interior choice
+ * (grid3ddev.cu:316), and the temperature's no-corner ghost fill
+ * (grid3dboundary.cu:28-119).  Reads u, v, w, temp and writes the new
+ * values into u_new .. temp_new (congruent, not aliasing): afterwards the
+ * new buffers hold, element for element, what the reference's in-place
+ * sequence leaves in the old ones, except the normal face n of u, v, w
+ * (overwritten by the MAC boundary fill of the projection that follows;
+ * flow faces periodic or forced-inflow).  keep_tendencies stores the
+ * tendencies in du .. dtemp (the copies into the "last" grids become a
+ * swap); use_last reads last_du .. last_dtemp.  Unused tendency grids may
+ * be NULL.  No reference counterpart as one call. */
+int ocu_ns_step_fused(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_grid3d *temp,
+                      const ocu_grid3d *u_new, const ocu_grid3d *v_new, const ocu_grid3d *w_new,
+                      const ocu_grid3d *temp_new, const ocu_grid3d *du, const ocu_grid3d *dv,
+                      const ocu_grid3d *dw, const ocu_grid3d *dtemp, const ocu_grid3d *last_du,
+                      const ocu_grid3d *last_dv, const ocu_grid3d *last_dw, const ocu_grid3d *last_dtemp,
+                      const ocu_ns_step_params *p, const ocu_bc_set *temp_bc, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

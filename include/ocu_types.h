@@ -146,6 +146,25 @@ typedef struct ocu_mg_config {
   int32_t prolong_quirk;
 } ocu_mg_config;
 
+/* One explicit NS time step done by ocu_ns_step_fused: the tendencies of
+ * Eqn_IncompressibleNS3D<T>::advance_one_step (eqn_incompressns3d.cu:331-393)
+ * and their forward-Euler / Adams-Bashforth combination. */
+typedef struct ocu_ns_step_params {
+  int32_t nx, ny, nz;            /* cells */
+  int32_t advection_scheme;      /* OCU_IT_* */
+  int32_t vertical_direction;    /* OCU_DIR_* (thermal force) */
+  int32_t flow_periodic;         /* periodic-axes mask (bit 0 x, 1 y, 2 z) of the flow BCs */
+  int32_t temp_periodic;         /* the same for the temperature BCs */
+  int32_t keep_tendencies;       /* 1: store this step's tendencies (Adams-Bashforth) */
+  int32_t use_last;              /* 1: add a_ab * the last step's tendencies */
+  double hx, hy, hz;
+  double viscosity;              /* <= 0: no momentum diffusion */
+  double thermal_diffusion;      /* <= 0: no temperature diffusion */
+  double thermal_coef;           /* dirmult * gravity * bouyancy, in T */
+  double a_dt;                   /* dt (Euler) or dt - ab (Adams-Bashforth), in T */
+  double a_ab;                   /* ab = -dt^2 / (2 dt_last), in T */
+} ocu_ns_step_params;
+
 /* Status codes of every ocu_* entry point (0 = success). */
 enum ocu_status {
   OCU_OK = 0,

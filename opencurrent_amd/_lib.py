@@ -11,7 +11,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-from ._types import BCSet, Grid3DDesc, MGConfig, MGLevelParams
+from ._types import BCSet, Grid3DDesc, MGConfig, MGLevelParams, NSStepParams
 
 LIBDIR = Path(__file__).resolve().parent / "lib"
 
@@ -112,6 +112,10 @@ HIP_SIGS = {
     "ocu_passive_advect_ex": (I, [GP, GP, GP, GP, GP, I, I, I, D, D, D, I, I, P]),
     "ocu_laplacian_acc": (I, [GP, GP, I, I, I, D, D, D, D, P]),
     "ocu_thermal_force": (I, [GP, GP, I, I, I, I, D, P]),
+    "ocu_ns_step_fused": (I, [GP] * 16 + [C.POINTER(NSStepParams), BP, P]),
+    "ocu_pcg_apply_laplacian": (I, [GP, GP, GP, D, D, D, P, DP, P]),
+    "ocu_pcg_diag_precond": (I, [GP, GP, GP, D, D, D, P]),
+    "ocu_grid_dot": (I, [GP, GP, P, DP, P]),
 }
 
 
@@ -128,6 +132,7 @@ class NSParams(C.Structure):
         ("advection_scheme", C.c_int32), ("time_step", C.c_int32),
         ("cfl_factor", C.c_double),
         ("mixed_projection", C.c_int32),
+        ("unfused_step", C.c_int32),
     ]
 
 
@@ -166,6 +171,11 @@ HOST_SIGS = {
     "ocu_project_divergence": (I, [P, GP]),
     "ocu_project_pressure": (I, [P, GP]),
     "ocu_project_destroy": (None, [P]),
+    "ocu_pcg_create": (P, [GP, GP, C.c_float, C.c_float, C.c_float, BP, I, I, I, I]),
+    "ocu_pcg_solve": (I, [P, D, I, DP]),
+    "ocu_pcg_iterations": (I, [P]),
+    "ocu_pcg_pressure": (I, [P, GP]),
+    "ocu_pcg_destroy": (None, [P]),
     "ocu_ns_create": (P, [I, C.POINTER(NSParams), GP, GP, GP, GP]),
     "ocu_ns_advance_one_step": (I, [P, D]),
     "ocu_ns_advance": (I, [P, D]),

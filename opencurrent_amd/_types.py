@@ -74,6 +74,17 @@ class MGLevelParams(C.Structure):
                 ("omega", C.c_double), ("mods", C.c_double * 6)]
 
 
+class NSStepParams(C.Structure):
+    """ocu_ns_step_params (include/ocu_types.h)"""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("advection_scheme", C.c_int32), ("vertical_direction", C.c_int32),
+                ("flow_periodic", C.c_int32), ("temp_periodic", C.c_int32),
+                ("keep_tendencies", C.c_int32), ("use_last", C.c_int32),
+                ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+                ("viscosity", C.c_double), ("thermal_diffusion", C.c_double),
+                ("thermal_coef", C.c_double), ("a_dt", C.c_double), ("a_ab", C.c_double)]
+
+
 class MGConfig(C.Structure):
     _fields_ = [("bc", BCSet), ("nu1", C.c_int32), ("nu2", C.c_int32),
                 ("convergence", C.c_int32), ("make_symmetric_operator", C.c_int32),

@@ -2,6 +2,8 @@
 // interior reductions (ocustorage/grid3dops.cu, ocuutil/reduction_op.h).
 #include "ocu_hip_common.h"
 
+#include <climits>
+
 using namespace ocu_hip;
 
 namespace {
@@ -166,6 +168,41 @@ __global__ void k_reduce_stage2(int op, const double *parts, int n, double *out)
 
 }  // namespace
 
+namespace {
+
+// the two launches of an interior reduction: NORM_PARTS partials into
+// `parts`, folded into *out (device memory)
+int check_reduce(const ocu_grid3d *g, int op, const double *out, const char *what) {
+  if (int e = check_grid(g, what)) return e;
+  OCU_REQUIRE(op >= 0 && op <= 5, "%s: bad op %d", what, op);
+  OCU_REQUIRE(out, "%s: null out", what);
+  OCU_REQUIRE(g->dtype != OCU_I32, "%s: int grids are not supported", what);
+  OCU_REQUIRE((int64_t)g->nx * g->ny < INT_MAX, "%s: too many rows", what);
+  return OCU_OK;
+}
+
+int launch_reduce(const ocu_grid3d *g, int op, double *parts, double *out, void *stream, const char *what) {
+#define RMAX(TT, OP) k_reduce_max_rows<TT, OP><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<TT>(g), parts)
+  if (op <= 2 && g->dtype == OCU_F64) {
+    if (op == 0) RMAX(double, 0);
+    else if (op == 1) RMAX(double, 1);
+    else RMAX(double, 2);
+  } else if (op <= 2) {
+    if (op == 0) RMAX(float, 0);
+    else if (op == 1) RMAX(float, 1);
+    else RMAX(float, 2);
+  } else if (g->dtype == OCU_F64) {
+    k_reduce_stage1<double><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<double>(g), op, parts);
+  } else {
+    k_reduce_stage1<float><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<float>(g), op, parts);
+  }
+#undef RMAX
+  k_reduce_stage2<<<1, 512, 0, S(stream)>>>(op, parts, NORM_PARTS, out);
+  OCU_HIP_LAUNCHED(what);
+}
+
+}  // namespace
+
 extern "C" {
 
 int ocu_grid_clear(const ocu_grid3d *g, double value, void *stream) {
@@ -259,29 +296,9 @@ int ocu_grid_add_scalar(const ocu_grid3d *g, double value, void *stream) {
 size_t ocu_reduce_workspace_bytes(void) { return sizeof(double) * NORM_PARTS; }
 
 int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, void *stream) {
-  if (int e = check_grid(g, "ocu_grid_reduce")) return e;
-  OCU_REQUIRE(op >= 0 && op <= 5, "ocu_grid_reduce: bad op %d", op);
-  OCU_REQUIRE(workspace && out, "ocu_grid_reduce: null workspace/out");
-  OCU_REQUIRE(g->dtype != OCU_I32, "ocu_grid_reduce: int grids are not supported");
-  double *parts = (double *)workspace;
-  OCU_REQUIRE((int64_t)g->nx * g->ny < INT_MAX, "ocu_grid_reduce: too many rows");
-#define RMAX(TT, OP) k_reduce_max_rows<TT, OP><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<TT>(g), parts)
-  if (op <= 2 && g->dtype == OCU_F64) {
-    if (op == 0) RMAX(double, 0);
-    else if (op == 1) RMAX(double, 1);
-    else RMAX(double, 2);
-  } else if (op <= 2) {
-    if (op == 0) RMAX(float, 0);
-    else if (op == 1) RMAX(float, 1);
-    else RMAX(float, 2);
-  } else if (g->dtype == OCU_F64) {
-    k_reduce_stage1<double><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<double>(g), op, parts);
-  } else {
-    k_reduce_stage1<float><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<float>(g), op, parts);
-  }
-#undef RMAX
-  k_reduce_stage2<<<1, 512, 0, S(stream)>>>(op, parts, NORM_PARTS, out);
-  OCU_HIP_LAUNCHED("ocu_grid_reduce");
+  if (int e = check_reduce(g, op, out, "ocu_grid_reduce")) return e;
+  OCU_REQUIRE(workspace, "ocu_grid_reduce: null workspace");
+  return launch_reduce(g, op, (double *)workspace, out, stream, "ocu_grid_reduce");
 }
 
 int ocu_grid_sum(const ocu_grid3d *g, double *ws, double *out, void *stream) {

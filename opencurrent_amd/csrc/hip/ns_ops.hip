@@ -4,23 +4,12 @@
 // requires congruent grids, sol_selfadvection3ddev.cu:622-626), so one
 // index serves every array.
 #include "ocu_hip_common.h"
+#include "advect.h"
 #include "rows.h"
 
 using namespace ocu_hip;
 
 namespace {
-
-template <typename T>
-struct Inv {
-  T x, y, z;
-};
-
-// interpolation.h:35-48
-template <typename T>
-__device__ __forceinline__ T interp(int scheme, T u, T x0, T x1) {
-  if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return ((T).5) * (x1 + x0);
-  return u < 0 ? x1 : x0;
-}
 
 // sol_divergence3ddev.cu:23-45
 template <typename T>
@@ -62,42 +51,11 @@ __global__ __launch_bounds__(256) void k_self_advect(const T *__restrict__ U, co
   int i = blockIdx.z;
   if (j >= ny || k >= nz) return;
   const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  const int64_t pi = id + xs, pj = id + ys, pk = id + 1, mi = id - xs, mj = id - ys, mk = id - 1;
-  const T half = (T).5;
-  T u0 = U[id], v0 = V[id], w0 = W[id];
-
-  T u_iph = half * (u0 + U[pi]);
-  T u_imh = half * (u0 + U[mi]);
-  T duu = (u_iph * interp(scheme, u_iph, u0, U[pi])) - (u_imh * interp(scheme, u_imh, U[mi], u0));
-  T v_jph = half * (V[pj] + V[pj - xs]);
-  T v_jmh = half * (v0 + V[mi]);
-  T dvu = (v_jph * interp(scheme, v_jph, u0, U[pj])) - (v_jmh * interp(scheme, v_jmh, U[mj], u0));
-  T w_kph = half * (W[pk] + W[pk - xs]);
-  T w_kmh = half * (w0 + W[mi]);
-  T dwu = (w_kph * interp(scheme, w_kph, u0, U[pk])) - (w_kmh * interp(scheme, w_kmh, U[mk], u0));
-  DU[id] = -duu * ih.x - dvu * ih.y - dwu * ih.z;
-
-  T u_atv_iph = half * (U[pi] + U[pi - ys]);
-  T u_atv_imh = half * (u0 + U[mj]);
-  T duv = (u_atv_iph * interp(scheme, u_atv_iph, v0, V[pi])) - (u_atv_imh * interp(scheme, u_atv_imh, V[mi], v0));
-  T v_iph = half * (v0 + V[pj]);
-  T v_imh = half * (v0 + V[mj]);
-  T dvv = (v_iph * interp(scheme, v_iph, v0, V[pj])) - (v_imh * interp(scheme, v_imh, V[mj], v0));
-  T w_atv_kph = half * (W[pk] + W[pk - ys]);
-  T w_atv_kmh = half * (w0 + W[mj]);
-  T dwv = (w_atv_kph * interp(scheme, w_atv_kph, v0, V[pk])) - (w_atv_kmh * interp(scheme, w_atv_kmh, V[mk], v0));
-  DV[id] = -duv * ih.x - dvv * ih.y - dwv * ih.z;
-
-  T u_atw_iph = half * (U[pi] + U[pi - 1]);
-  T u_atw_imh = half * (u0 + U[mk]);
-  T duw = (u_atw_iph * interp(scheme, u_atw_iph, w0, W[pi])) - (u_atw_imh * interp(scheme, u_atw_imh, W[mi], w0));
-  T v_atw_jph = half * (V[pj] + V[pj - 1]);
-  T v_atw_jmh = half * (v0 + V[mk]);
-  T dvw = (v_atw_jph * interp(scheme, v_atw_jph, w0, W[pj])) - (v_atw_jmh * interp(scheme, v_atw_jmh, W[mj], w0));
-  T w_iph = half * (w0 + W[pk]);
-  T w_imh = half * (w0 + W[mk]);
-  T dww = (w_iph * interp(scheme, w_iph, w0, W[pk])) - (w_imh * interp(scheme, w_imh, W[mk], w0));
-  DW[id] = -duw * ih.x - dvw * ih.y - dww * ih.z;
+  T du, dv, dw;
+  self_advect_cell(U, V, W, id, xs, ys, ih, scheme, du, dv, dw);
+  DU[id] = du;
+  DV[id] = dv;
+  DW[id] = dw;
 }
 
 // Sol_PassiveAdvection3D_apply_interp (sol_passiveadvection3ddev.cu:31-78)
@@ -111,59 +69,11 @@ __global__ __launch_bounds__(256) void k_passive_advect(const T *__restrict__ u,
   int i = blockIdx.z;
   if (j >= ny || k >= nz) return;
   const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  T p = phi[id];
-  T u_pi = u[id + xs], u0 = u[id];
-  T du = u_pi * interp(scheme, u_pi, p, phi[id + xs]) - u0 * interp(scheme, u0, phi[id - xs], p);
-  T v_pj = v[id + ys], v0 = v[id];
-  T dv = v_pj * interp(scheme, v_pj, p, phi[id + ys]) - v0 * interp(scheme, v0, phi[id - ys], p);
-  T w_pk = w[id + 1], w0 = w[id];
-  T dw = w_pk * interp(scheme, w_pk, p, phi[id + 1]) - w0 * interp(scheme, w0, phi[id - 1], p);
-  d[id] = (-du * ih.x) - (dv * ih.y) - (dw * ih.z);
+  d[id] = passive_advect_cell(u, v, w, phi, id, xs, ys, ih, scheme);
 }
 
-// ---- second-order (linear) upwind, OCU_IT_SECOND_ORDER_UPWIND (no reference
-// counterpart) ----
-//
-// The face value between xL and xR is extrapolated from the two upstream
-// samples, xL + .5*(xL - xLL) for vel >= 0 and xR + .5*(xR - xRR) for vel < 0
-// (vel == 0 takes the low side, as the reference's upwind does).  Only the
-// second upstream sample can leave a ghost layer: along axis a the layer of
-// a cell-centred or tangential array spans [-1, n_a], that of the face array
-// normal to a [-1, n_a + 1].  Beyond it the sample wraps (periodic axis) or
-// is missing, and the face falls back to first order.
-template <typename T>
-__device__ __forceinline__ T up2(T vel, T xLL, bool hLL, T xL, T xR, T xRR, bool hRR) {
-  if (vel < 0) return hRR ? xR + ((T).5) * (xR - xRR) : xR;
-  return hLL ? xL + ((T).5) * (xL - xLL) : xL;
-}
+// ---- second-order (linear) upwind (no reference counterpart; advect.h) ----
 
-// Offsets (in elements) of the samples two cells below and above cell c.
-struct Far2 {
-  int64_t lo, hi;
-  bool hlo, hhi;
-};
-
-__device__ __forceinline__ Far2 far2(int c, int n, int top, int64_t s, bool per) {
-  Far2 f;
-  const int wlo = (((c - 2) % n) + n) % n, whi = (c + 2) % n;
-  f.hlo = c >= 1 || per;
-  f.hhi = c + 2 <= top || per;
-  f.lo = (c >= 1 ? (int64_t)-2 : (int64_t)(wlo - c)) * s;
-  f.hi = (c + 2 <= top ? (int64_t)2 : (int64_t)(whi - c)) * s;
-  return f;
-}
-
-// (vh * face_hi) - (vl * face_lo) for the array X along the axis of stride s
-template <typename T>
-__device__ __forceinline__ T dflux2(const T *__restrict__ X, int64_t id, int64_t s, const Far2 &f, T vh, T vl) {
-  const T x0 = X[id], xm = X[id - s], xp = X[id + s];
-  const T xpp = f.hhi ? X[id + f.hi] : (T)0;
-  const T xmm = f.hlo ? X[id + f.lo] : (T)0;
-  return (vh * up2(vh, xm, true, x0, xp, xpp, f.hhi)) - (vl * up2(vl, xmm, f.hlo, xm, x0, xp, true));
-}
-
-// self-advection with second-order upwind face values: the face velocities
-// and flux terms of k_self_advect
 template <typename T>
 __global__ __launch_bounds__(256) void k_self_advect_up2(const T *__restrict__ U, const T *__restrict__ V,
                                                          const T *__restrict__ W, T *__restrict__ DU,
@@ -174,32 +84,13 @@ __global__ __launch_bounds__(256) void k_self_advect_up2(const T *__restrict__ U
   int i = blockIdx.z;
   if (j >= ny || k >= nz) return;
   const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  const int64_t pi = id + xs, pj = id + ys, pk = id + 1, mi = id - xs, mj = id - ys, mk = id - 1;
-  const bool px = pmask & 1, py = pmask & 2, pz = pmask & 4;
-  // tangential (t) and normal (n) ghost extents per axis
-  const Far2 fxt = far2(i, nx, nx, xs, px), fxn = far2(i, nx, nx + 1, xs, px);
-  const Far2 fyt = far2(j, ny, ny, ys, py), fyn = far2(j, ny, ny + 1, ys, py);
-  const Far2 fzt = far2(k, nz, nz, 1, pz), fzn = far2(k, nz, nz + 1, 1, pz);
-  const T half = (T).5;
-  const T u0 = U[id], v0 = V[id], w0 = W[id];
-
-  const T duu = dflux2(U, id, xs, fxn, half * (u0 + U[pi]), half * (u0 + U[mi]));
-  const T dvu = dflux2(U, id, ys, fyt, half * (V[pj] + V[pj - xs]), half * (v0 + V[mi]));
-  const T dwu = dflux2(U, id, (int64_t)1, fzt, half * (W[pk] + W[pk - xs]), half * (w0 + W[mi]));
-  DU[id] = -duu * ih.x - dvu * ih.y - dwu * ih.z;
-
-  const T duv = dflux2(V, id, xs, fxt, half * (U[pi] + U[pi - ys]), half * (u0 + U[mj]));
-  const T dvv = dflux2(V, id, ys, fyn, half * (v0 + V[pj]), half * (v0 + V[mj]));
-  const T dwv = dflux2(V, id, (int64_t)1, fzt, half * (W[pk] + W[pk - ys]), half * (w0 + W[mj]));
-  DV[id] = -duv * ih.x - dvv * ih.y - dwv * ih.z;
-
-  const T duw = dflux2(W, id, xs, fxt, half * (U[pi] + U[pi - 1]), half * (u0 + U[mk]));
-  const T dvw = dflux2(W, id, ys, fyt, half * (V[pj] + V[pj - 1]), half * (v0 + V[mk]));
-  const T dww = dflux2(W, id, (int64_t)1, fzn, half * (w0 + W[pk]), half * (w0 + W[mk]));
-  DW[id] = -duw * ih.x - dvw * ih.y - dww * ih.z;
+  T du, dv, dw;
+  self_advect_up2_cell(U, V, W, id, xs, ys, ih, i, j, k, nx, ny, nz, pmask, du, dv, dw);
+  DU[id] = du;
+  DV[id] = dv;
+  DW[id] = dw;
 }
 
-// passive advection with second-order upwind face values
 template <typename T>
 __global__ __launch_bounds__(256) void k_passive_advect_up2(const T *__restrict__ u, const T *__restrict__ v,
                                                             const T *__restrict__ w, const T *__restrict__ phi,
@@ -210,10 +101,7 @@ __global__ __launch_bounds__(256) void k_passive_advect_up2(const T *__restrict_
   int i = blockIdx.z;
   if (j >= ny || k >= nz) return;
   const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  const T du = dflux2(phi, id, xs, far2(i, nx, nx, xs, pmask & 1), u[id + xs], u[id]);
-  const T dv = dflux2(phi, id, ys, far2(j, ny, ny, ys, pmask & 2), v[id + ys], v[id]);
-  const T dw = dflux2(phi, id, (int64_t)1, far2(k, nz, nz, 1, pmask & 4), w[id + 1], w[id]);
-  d[id] = (-du * ih.x) - (dv * ih.y) - (dw * ih.z);
+  d[id] = passive_advect_up2_cell(u, v, w, phi, id, xs, ys, ih, i, j, k, nx, ny, nz, pmask);
 }
 
 // ---- adjoints (no reference counterpart) ----
@@ -392,11 +280,8 @@ __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ phi, T 
   int i = blockIdx.z;
   if (j >= ny || k >= nz) return;
   const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  T p = phi[id];
   T dp = d[id];
-  T lap = ih2.z * (phi[id + 1] + phi[id - 1] - ((T)2) * p) + ih2.y * (phi[id + ys] + phi[id - ys] - ((T)2) * p) +
-          ih2.x * (phi[id + xs] + phi[id - xs] - ((T)2) * p);
-  d[id] = dp + coef * lap;
+  d[id] = dp + coef * laplacian_cell(phi, id, xs, ys, ih2);
 }
 
 // The same d += coef * lap(phi) by rows: each wave owns a row j (V cells
@@ -488,7 +373,7 @@ __global__ __launch_bounds__(256) void k_thermal(T *__restrict__ d, const T *__r
   int i = blockIdx.z;
   if (j >= ny || k >= nz) return;
   const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  d[id] += ((T).5) * coef * (t[id] + t[id - nbr]);
+  d[id] += thermal_cell(t, id, nbr, coef);
 }
 
 template <typename T>

@@ -10,6 +10,7 @@
 #include "ocuequation/eqn_scalaradvection3d.h"
 #include "ocuequation/sol_mgpressure3d.h"
 #include "ocuequation/sol_mgpressuremixed3d.h"
+#include "ocuequation/sol_pcgpressure3d.h"
 #include "ocuequation/sol_project3d.h"
 #include "ocuutil/thread.h"
 
@@ -122,6 +123,7 @@ bool fill_params(Eqn_IncompressibleNS3DParams<T> &P, const ocu_ns_params *p, con
   P.time_step = (TimeStepType)p->time_step;
   P.cfl_factor = p->cfl_factor;
   P.mixed_precision_projection = p->mixed_projection != 0;
+  P.fused_step = p->unfused_step == 0;
   return host_grid_from(P.init_u, iu, "ocu_ns_create init_u") && host_grid_from(P.init_v, iv, "ocu_ns_create init_v") &&
          host_grid_from(P.init_w, iw, "ocu_ns_create init_w") &&
          host_grid_from(P.init_temp, it, "ocu_ns_create init_temp");
@@ -519,5 +521,86 @@ int ocu_scalar_field(void *hh, int which, ocu_grid3d *out) {
 }
 
 void ocu_scalar_destroy(void *hh) { delete (ScalarHandle *)hh; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- PCG
+
+namespace {
+struct PCGHandle {
+  Sol_PCGPressure3DDevice<double> *d = nullptr;
+  Sol_PCGPressure3DDevice<float> *f = nullptr;
+  Grid3DDevice<double> rhs_d, coeff_d;
+  Grid3DDevice<float> rhs_f, coeff_f;
+  ~PCGHandle() {
+    delete d;
+    delete f;
+  }
+};
+
+template <typename T>
+bool pcg_init(Sol_PCGPressure3DDevice<T> &s, Grid3DDevice<T> &rhs, Grid3DDevice<T> &coeff, const ocu_grid3d *r,
+              const ocu_grid3d *c, float hx, float hy, float hz, const ocu_bc_set *bc, int convergence,
+              int preconditioner, int cycles, int use_fmg) {
+  s.bc = from_c(*bc);
+  s.convergence = (ConvergenceType)convergence;
+  s.preconditioner = (PreconditionerType)preconditioner;
+  s.multigrid_cycles = cycles;
+  s.multigrid_use_fmg = use_fmg != 0;
+  return rhs.init_external(*r) && coeff.init_external(*c) &&
+         s.initialize_storage(r->nx, r->ny, r->nz, hx, hy, hz, &rhs, &coeff);
+}
+}  // namespace
+
+extern "C" {
+
+void *ocu_pcg_create(const ocu_grid3d *rhs, const ocu_grid3d *coeff, float hx, float hy, float hz,
+                     const ocu_bc_set *bc, int convergence, int preconditioner, int multigrid_cycles,
+                     int multigrid_use_fmg) {
+  if (!rhs || !coeff || !bc || (rhs->dtype != OCU_F32 && rhs->dtype != OCU_F64) || coeff->dtype != rhs->dtype)
+    return nullptr;
+  if (preconditioner < 0 || preconditioner > 3) return nullptr;
+  PCGHandle *h = new (std::nothrow) PCGHandle();
+  if (!h) return nullptr;
+  bool ok;
+  if (rhs->dtype == OCU_F64) {
+    h->d = new Sol_PCGPressure3DDevice<double>();
+    ok = pcg_init(*h->d, h->rhs_d, h->coeff_d, rhs, coeff, hx, hy, hz, bc, convergence, preconditioner,
+                  multigrid_cycles, multigrid_use_fmg);
+  } else {
+    h->f = new Sol_PCGPressure3DDevice<float>();
+    ok = pcg_init(*h->f, h->rhs_f, h->coeff_f, rhs, coeff, hx, hy, hz, bc, convergence, preconditioner,
+                  multigrid_cycles, multigrid_use_fmg);
+  }
+  if (!ok) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int ocu_pcg_solve(void *hh, double tol, int max_iter, double *residual) {
+  PCGHandle *h = (PCGHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  double r = 0;
+  bool ok = h->d ? h->d->solve(r, tol, max_iter) : h->f->solve(r, tol, max_iter);
+  if (residual) *residual = r;
+  return ok ? 1 : 0;
+}
+
+int ocu_pcg_iterations(void *hh) {
+  PCGHandle *h = (PCGHandle *)hh;
+  if (!h) return OCU_ERR_INVALID;
+  return h->d ? h->d->iterations() : h->f->iterations();
+}
+
+int ocu_pcg_pressure(void *hh, ocu_grid3d *out) {
+  PCGHandle *h = (PCGHandle *)hh;
+  if (!h || !out) return OCU_ERR_INVALID;
+  *out = h->d ? h->d->pressure().desc() : h->f->pressure().desc();
+  return OCU_OK;
+}
+
+void ocu_pcg_destroy(void *hh) { delete (PCGHandle *)hh; }
 
 }  // extern "C"

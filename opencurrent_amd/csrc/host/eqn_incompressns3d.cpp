@@ -67,7 +67,7 @@ bool Eqn_IncompressibleNS3DBase<T>::set_base_parameters(const Eqn_Incompressible
 }
 
 template <typename T>
-Eqn_IncompressibleNS3D<T>::Eqn_IncompressibleNS3D() : _mixed_projection(false) {}
+Eqn_IncompressibleNS3D<T>::Eqn_IncompressibleNS3D() : _mixed_projection(false), _fused(false) {}
 
 // eqn_incompressns3d.cu:119-155: the vertical component gets
 // .5 * (dirmult * g * beta) * (T + T_{-vertical})
@@ -200,6 +200,24 @@ bool Eqn_IncompressibleNS3D<T>::set_parameters(const Eqn_IncompressibleNS3DParam
     printf("[ERROR] Eqn_IncompressibleNS3D::set_parameters - grid layouts do not all match\n");
     return false;
   }
+  // The fused step needs one congruent layout for the four grids (what
+  // init_grids gives) and flow faces the projection's MAC fill rewrites
+  // (periodic or forced inflow), since it leaves the normal face n of u, v,
+  // w to that fill.
+  _fused = false;
+  if (params.fused_step && params.flow_bc.check_type(BC_PERIODIC, BC_FORCED_INFLOW_VARIABLE_SLIP) &&
+      _u.shift_amount() == _v.shift_amount() && _u.shift_amount() == _w.shift_amount() &&
+      _u.shift_amount() == _temp.shift_amount() && _u.gx() == _temp.gx() && _u.gy() == _temp.gy() &&
+      _u.gz() == _temp.gz() && _u.num_allocated_elements() == _temp.num_allocated_elements() &&
+      _u.gx() == _v.gx() && _u.gy() == _v.gy() && _u.gz() == _v.gz() && _u.gx() == _w.gx() &&
+      _u.gy() == _w.gy() && _u.gz() == _w.gz()) {
+    if (!_u_next.init_congruent(_u) || !_v_next.init_congruent(_v) || !_w_next.init_congruent(_w) ||
+        !_temp_next.init_congruent(_temp)) {
+      printf("[ERROR] Eqn_IncompressibleNS3D::set_parameters - failed on the fused step's buffers\n");
+      return false;
+    }
+    _fused = true;
+  }
   return true;
 }
 
@@ -226,11 +244,73 @@ double Eqn_IncompressibleNS3D<T>::get_max_stable_timestep() const {
   return step;
 }
 
+// The explicit part of the step as one ocu_ns_step_fused launch into the
+// _next buffers, which are then swapped in; the projection follows as in
+// the reference.  Same values as the sequence below (ocu_ns_step_fused).
+template <typename T>
+bool Eqn_IncompressibleNS3D<T>::advance_fused(double dt) {
+  T ab_coeff = -dt * dt / (2 * this->_lastdt);
+  bool ab2 = this->_time_step == TS_ADAMS_BASHFORD2 && this->_lastdt > 0;
+  bool keep = this->_time_step == TS_ADAMS_BASHFORD2;
+  ocu_ns_step_params sp;
+  sp.nx = this->nx(); sp.ny = this->ny(); sp.nz = this->nz();
+  sp.advection_scheme = (int)_advection_solver.interp_type;
+  sp.vertical_direction = (int)this->_vertical_direction;
+  sp.flow_periodic = _advection_solver.periodic_axes;
+  sp.temp_periodic = _thermal_solver.periodic_axes;
+  sp.keep_tendencies = keep;
+  sp.use_last = ab2;
+  sp.hx = this->_hx; sp.hy = this->_hy; sp.hz = this->_hz;
+  sp.viscosity = viscosity_coefficient() > 0 ? (double)viscosity_coefficient() : 0.0;
+  sp.thermal_diffusion = thermal_diffusion_coefficient() > 0 ? (double)thermal_diffusion_coefficient() : 0.0;
+  T direction_mult = (this->_vertical_direction & DIR_NEGATIVE_FLAG) ? 1 : -1;
+  T coef = direction_mult * this->_gravity * this->_bouyancy;
+  sp.thermal_coef = (double)coef;
+  sp.a_dt = ab2 ? (double)(T)(dt - ab_coeff) : (double)(T)dt;
+  sp.a_ab = ab2 ? (double)ab_coeff : 0.0;
+  ocu_grid3d u = _u.desc(), v = _v.desc(), w = _w.desc(), t = _temp.desc();
+  ocu_grid3d un = _u_next.desc(), vn = _v_next.desc(), wn = _w_next.desc(), tn = _temp_next.desc();
+  ocu_grid3d du = _deriv_udt.desc(), dv = _deriv_vdt.desc(), dw = _deriv_wdt.desc(), dt_ = _deriv_tempdt.desc();
+  ocu_grid3d lu = _last_deriv_udt.desc(), lv = _last_deriv_vdt.desc(), lw = _last_deriv_wdt.desc(),
+             lt = _last_deriv_tempdt.desc();
+  ocu_bc_set tbc = to_c_bc(this->_thermalbc);
+  if (!ocu_status_ok(ocu_ns_step_fused(&u, &v, &w, &t, &un, &vn, &wn, &tn, &du, &dv, &dw, &dt_, &lu, &lv, &lw, &lt,
+                                       &sp, &tbc, ThreadManager::get_compute_stream()),
+                     "Eqn_IncompressibleNS3D::advance_one_step (fused)")) {
+    this->add_error();
+    return false;
+  }
+  _u.swap_storage(_u_next);
+  _v.swap_storage(_v_next);
+  _w.swap_storage(_w_next);
+  _temp.swap_storage(_temp_next);
+  if (keep) {
+    // the reference copies the tendencies into the last-tendency grids
+    this->_lastdt = dt;
+    _last_deriv_tempdt.swap_storage(_deriv_tempdt);
+    _last_deriv_udt.swap_storage(_deriv_udt);
+    _last_deriv_vdt.swap_storage(_deriv_vdt);
+    _last_deriv_wdt.swap_storage(_deriv_wdt);
+  }
+  return true;
+}
+
 // eqn_incompressns3d.cu:331-399
 template <typename T>
 bool Eqn_IncompressibleNS3D<T>::advance_one_step(double dt) {
   this->clear_error();
   this->num_steps++;
+  if (_fused) {
+    if (!advance_fused(dt)) return false;
+    if constexpr (std::is_same<T, double>::value) {
+      if (_mixed_projection) {
+        this->check_ok(_mixed_projection_solver.solve(this->_max_divergence));
+        return !this->any_error();
+      }
+    }
+    this->check_ok(_projection_solver.solve(this->_max_divergence));
+    return !this->any_error();
+  }
   this->check_ok(_advection_solver.solve());
   if (viscosity_coefficient() > 0) {
     this->check_ok(_u_diffusion.solve());

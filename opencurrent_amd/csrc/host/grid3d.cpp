@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 
 #include "ocu_hip.h"
 #include "ocustorage/grid3d.h"
@@ -226,6 +227,19 @@ bool Grid3DDevice<T>::init_external(const ocu_grid3d &d) {
   this->_buffer = d.base;
   this->_owns = false;
   this->_shifted_buffer = (T *)d.base + this->_shift_amount;
+  return true;
+}
+
+template <typename T>
+bool Grid3DDevice<T>::swap_storage(Grid3DDevice<T> &other) {
+  if (!this->check_layout_match(other) || this->_shift_amount != other._shift_amount ||
+      this->_allocated_elements != other._allocated_elements || !this->check_interior_dimension_match(other)) {
+    printf("[ERROR] Grid3DDevice::swap_storage - layout mismatch\n");
+    return false;
+  }
+  std::swap(this->_buffer, other._buffer);
+  std::swap(this->_shifted_buffer, other._shifted_buffer);
+  std::swap(this->_owns, other._owns);
   return true;
 }
 

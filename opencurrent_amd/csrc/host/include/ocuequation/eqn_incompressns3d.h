@@ -37,6 +37,10 @@ class Eqn_IncompressibleNS3DParams : public Parameters {
   // not in the reference: project with FP32 V-cycles inside FP64 iterative
   // refinement (Sol_ProjectDivergenceMixed3DDeviceD); T = double only
   bool mixed_precision_projection;
+  // not in the reference: advance with the fused step kernel
+  // (ocu_ns_step_fused) where it applies; false runs the reference's
+  // operator-by-operator sequence.  Both give the same values.
+  bool fused_step;
 
   // defaults of eqn_incompressns3d.h:59-74
   Eqn_IncompressibleNS3DParams() {
@@ -52,6 +56,7 @@ class Eqn_IncompressibleNS3DParams : public Parameters {
     time_step = TS_FORWARD_EULER;
     vertical_direction = DIR_YPOS;
     mixed_precision_projection = false;
+    fused_step = true;
   }
 
   // eqn_incompressns3d.h:78-86: paddings (0,1,1), (1,0,1), (1,1,0), (1,1,1)
@@ -115,8 +120,12 @@ class Eqn_IncompressibleNS3D : public Eqn_IncompressibleNS3DBase<T> {
   Grid3DDevice<T> _deriv_tempdt;
   Grid3DDevice<T> _last_deriv_tempdt;
   Grid3DDevice<T> _last_deriv_udt, _last_deriv_vdt, _last_deriv_wdt;
+  // fused step: the buffers the new u, v, w, T are written to (swapped in)
+  bool _fused;
+  Grid3DDevice<T> _u_next, _v_next, _w_next, _temp_next;
 
   void add_thermal_force();
+  bool advance_fused(double dt);
 
  public:
   Eqn_IncompressibleNS3D();
@@ -138,6 +147,7 @@ class Eqn_IncompressibleNS3D : public Eqn_IncompressibleNS3DBase<T> {
     return _projection_solver.pressure_solver.pressure();
   }
   bool mixed_projection() const { return _mixed_projection; }
+  bool fused_step() const { return _fused; }
   Sol_ProjectDivergence3DDevice<T> &projection_solver() { return _projection_solver; }
 };
 

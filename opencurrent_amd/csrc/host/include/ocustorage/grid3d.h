@@ -199,6 +199,11 @@ class Grid3DDevice : public Grid3DTypedBase<T> {
 
   // reduction in double precision (op as ocu_grid_reduce), blocking
   bool reduce_double(int op, double &result) const;
+
+  // exchange the storage of two grids of one layout (no reference
+  // counterpart): a kernel that writes a new version of a grid out of
+  // place hands it over without a copy
+  bool swap_storage(Grid3DDevice<T> &other);
 };
 
 typedef Grid3DHost<float> Grid3DHostF;

@@ -204,6 +204,48 @@ def _check_init_grids(what, params, dt, grids):
                              f"{(nx, ny, nz)} / dtype {dt}")
 
 
+class PCGPressure3D:
+    """Sol_PCGPressure3DDevice<T> (ocuequation/sol_pcgpressure3d.h:41-101):
+    -div(c grad p) = rhs for device grids rhs and coeff (congruent, kept
+    alive here).  preconditioner: 0 none, 1 Jacobi, 2 multigrid, 3 BFBT
+    (the reference's default)."""
+
+    NONE, JACOBI, MULTIGRID, BFBT = 0, 1, 2, 3
+
+    def __init__(self, rhs, coeff, hx, hy, hz, bc, preconditioner=3, multigrid_cycles=2, use_fmg=True,
+                 convergence=CONVERGENCE_L2):
+        self._keep = (rhs, coeff)
+        self.h = _lib.host().ocu_pcg_create(rhs.ptr(), coeff.ptr(), hx, hy, hz, C.byref(bc), convergence,
+                                            preconditioner, multigrid_cycles, int(use_fmg))
+        if not self.h:
+            raise _lib.OcuError(f"Sol_PCGPressure3DDevice::initialize_storage failed: {_lib.last_error()}")
+
+    def solve(self, tol=1e-6, max_iter=1000):
+        r = C.c_double()
+        ok = _lib.check(_lib.host().ocu_pcg_solve(self.h, tol, max_iter, C.byref(r)), "solve")
+        return bool(ok), r.value
+
+    @property
+    def iterations(self):
+        return _lib.host().ocu_pcg_iterations(self.h)
+
+    def pressure_desc(self):
+        d = Grid3DDesc()
+        _lib.check(_lib.host().ocu_pcg_pressure(self.h, C.byref(d)), "pressure")
+        return d
+
+    def pressure(self):
+        return desc_to_host(self.pressure_desc())
+
+    def set_pressure(self, host_grid):
+        host_to_desc(host_grid, self.pressure_desc())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.host().ocu_pcg_destroy(self.h)
+            self.h = None
+
+
 class IncompressibleNS3D:
     """Eqn_IncompressibleNS3D<T> (ocuequation/eqn_incompressns3d.h:132-175).
 

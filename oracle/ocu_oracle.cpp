@@ -7,6 +7,7 @@
 #include <omp.h>
 
 #include "oracle_ns.h"
+#include "oracle_pcg.h"
 
 using namespace orc;
 
@@ -356,4 +357,56 @@ int orc_gradient_adjoint(const ocu_grid3d *adj_u, const ocu_grid3d *adj_v, const
   return DISPATCH(adj_p->dtype,
                   (gradient_adjoint<float>(*adj_u, *adj_v, *adj_w, *adj_p, (float)coefficient, hx, hy, hz), 1),
                   (gradient_adjoint<double>(*adj_u, *adj_v, *adj_w, *adj_p, (double)coefficient, hx, hy, hz), 1));
+}
+
+int orc_pcg_apply_laplacian(const ocu_grid3d *dst, const ocu_grid3d *src, const ocu_grid3d *coeff, double fxh,
+                            double fyh, double fzh) {
+  if (!dst || !src || !coeff) return -1;
+  return DISPATCH(src->dtype,
+                  (pcg_apply_laplacian<float>(*dst, *src, *coeff, (float)fxh, (float)fyh, (float)fzh), 1),
+                  (pcg_apply_laplacian<double>(*dst, *src, *coeff, fxh, fyh, fzh), 1));
+}
+
+int orc_pcg_diag_precond(const ocu_grid3d *dst, const ocu_grid3d *src, const ocu_grid3d *coeff, double fxh,
+                         double fyh, double fzh) {
+  if (!dst || !src || !coeff) return -1;
+  return DISPATCH(src->dtype, (pcg_diag_precond<float>(*dst, *src, *coeff, (float)fxh, (float)fyh, (float)fzh), 1),
+                  (pcg_diag_precond<double>(*dst, *src, *coeff, fxh, fyh, fzh), 1));
+}
+
+double orc_pcg_dot(const ocu_grid3d *a, const ocu_grid3d *b) {
+  return a->dtype == OCU_F64 ? (double)pcg_dot<double>(*a, *b) : (double)pcg_dot<float>(*a, *b);
+}
+
+template <typename T>
+static int pcg_solve_t(const ocu_grid3d *rhs, const ocu_grid3d *coeff, const ocu_grid3d *pressure, float hx,
+                       float hy, float hz, const ocu_bc_set *bc, int preconditioner, int cycles, int use_fmg,
+                       int convergence, double tol, int max_iter, double *residual, int *iterations) {
+  PCGSolver<T> s;
+  s.bc = *bc;
+  s.preconditioner = preconditioner;
+  s.multigrid_cycles = cycles;
+  s.multigrid_use_fmg = use_fmg != 0;
+  s.convergence = convergence;
+  if (!s.initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, *rhs, *coeff)) return -1;
+  if (!layout_match(*pressure, s.pressure.desc)) return -1;
+  std::memcpy(s.pressure.buf.data(), pressure->base, sizeof(T) * (size_t)pressure->allocated);
+  double res = 0;
+  bool ok = s.solve(res, tol, max_iter);
+  std::memcpy(pressure->base, s.pressure.buf.data(), sizeof(T) * (size_t)pressure->allocated);
+  if (residual) *residual = res;
+  if (iterations) *iterations = s.iterations;
+  return ok ? 1 : 0;
+}
+
+int orc_pcg_solve(const ocu_grid3d *rhs, const ocu_grid3d *coeff, const ocu_grid3d *pressure, float hx, float hy,
+                  float hz, const ocu_bc_set *bc, int preconditioner, int multigrid_cycles, int use_fmg,
+                  int convergence, double tol, int max_iter, double *residual, int *iterations) {
+  if (!rhs || !coeff || !pressure || !bc) return -1;
+  if (rhs->dtype != coeff->dtype || rhs->dtype != pressure->dtype) return -1;
+  return DISPATCH(rhs->dtype,
+                  pcg_solve_t<float>(rhs, coeff, pressure, hx, hy, hz, bc, preconditioner, multigrid_cycles, use_fmg,
+                                     convergence, tol, max_iter, residual, iterations),
+                  pcg_solve_t<double>(rhs, coeff, pressure, hx, hy, hz, bc, preconditioner, multigrid_cycles,
+                                      use_fmg, convergence, tol, max_iter, residual, iterations));
 }

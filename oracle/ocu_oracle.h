@@ -120,6 +120,20 @@ int orc_scalar_advect_step(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_g
                            int nx, int ny, int nz, double hx, double hy, double hz, int scheme,
                            double dt);
 
+/* Sol_PCGPressure3DDevice<T> (sol_pcgpressure3d.cu): fxh/fyh/fzh are the
+ * T values f/h^2 the kernels receive */
+int orc_pcg_apply_laplacian(const ocu_grid3d *dst, const ocu_grid3d *src, const ocu_grid3d *coeff, double fxh,
+                            double fyh, double fzh);
+int orc_pcg_diag_precond(const ocu_grid3d *dst, const ocu_grid3d *src, const ocu_grid3d *coeff, double fxh,
+                         double fyh, double fzh);
+double orc_pcg_dot(const ocu_grid3d *a, const ocu_grid3d *b);
+/* solve(residual, tol, max_iter) from the guess in pressure (overwritten by
+ * the result); the coefficient's ghosts are filled in place as the reference
+ * does.  preconditioner: 0 none, 1 Jacobi, 2 multigrid, 3 BFBT. */
+int orc_pcg_solve(const ocu_grid3d *rhs, const ocu_grid3d *coeff, const ocu_grid3d *pressure, float hx, float hy,
+                  float hz, const ocu_bc_set *bc, int preconditioner, int multigrid_cycles, int use_fmg,
+                  int convergence, double tol, int max_iter, double *residual, int *iterations);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,10 @@ SIGS = {
     "orc_ns_max_stable_timestep": (D, [C.c_void_p]),
     "orc_ns_get": (I, [C.c_void_p, I, GP]),
     "orc_ns_destroy": (None, [C.c_void_p]),
+    "orc_pcg_apply_laplacian": (I, [GP, GP, GP, D, D, D]),
+    "orc_pcg_diag_precond": (I, [GP, GP, GP, D, D, D]),
+    "orc_pcg_dot": (D, [GP, GP]),
+    "orc_pcg_solve": (I, [GP, GP, GP, C.c_float, C.c_float, C.c_float, BP, I, I, I, I, D, I, DP, IP]),
     "orc_scalar_advect_step": (I, [GP, GP, GP, GP, GP, BP, I, I, I, D, D, D, I, D]),
 }
 
@@ -100,3 +104,13 @@ def level_params(bc, level, hx, hy, hz):
     p = MGLevelParams()
     lib().orc_mg_level_params(C.byref(bc), level, hx, hy, hz, C.byref(p))
     return p
+
+
+def pcg_solve(pressure, rhs, coeff, h, bc, preconditioner, cycles=2, use_fmg=True, convergence=5, tol=1e-6,
+              max_iter=1000):
+    """The oracle's Sol_PCGPressure3DDevice solve from the guess in HostGrid
+    `pressure` (overwritten); returns (ok, residual, iterations)."""
+    res, it = C.c_double(), C.c_int()
+    ok = lib().orc_pcg_solve(rhs.ptr(), coeff.ptr(), pressure.ptr(), h[0], h[1], h[2], C.byref(bc), preconditioner,
+                             cycles, int(use_fmg), convergence, tol, max_iter, C.byref(res), C.byref(it))
+    return ok, res.value, it.value

@@ -418,14 +418,18 @@ def rb_initial(n, dtype):
     return u, v, w, t
 
 
+@pytest.mark.parametrize("unfused", [0, 1])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("scheme", [T.IT_FIRST_ORDER_UPWIND, T.IT_SECOND_ORDER_CENTERED, T.IT_SECOND_ORDER_UPWIND])
 @pytest.mark.parametrize("ts", [T.TS_FORWARD_EULER, T.TS_ADAMS_BASHFORD2])
-def test_ns_steps_match_oracle(gpu, dtype, scheme, ts):
+def test_ns_steps_match_oracle(gpu, dtype, scheme, ts, unfused):
     """Rayleigh-Benard steps (rayleightiming.cpp set-up at 16^3): the GPU
-    Eqn_IncompressibleNS3D and the oracle's restatement stay bit-identical."""
+    Eqn_IncompressibleNS3D - with the fused step and with the reference's
+    operator sequence - and the oracle's restatement stay bit-identical
+    (whole buffers, ghosts and padding included)."""
     n = (16, 16, 16)
     p = ns_params(n, dtype, scheme, ts)
+    p.unfused_step = unfused
     u, v, w, t = rb_initial(n, dtype)
     from opencurrent_amd._types import dtype_code
     ho = O.lib().orc_ns_create(dtype_code(dtype), C.byref(p), u.ptr(), v.ptr(), w.ptr(), t.ptr())

@@ -40,6 +40,9 @@ def test_struct_sizes_match_the_c_layout():
     assert C.sizeof(T.BCSet) == 72
     assert C.sizeof(T.Grid3DDesc) == 8 + 10 * 4 + 8 + 8
     assert C.sizeof(T.MGLevelParams) == 11 * 8
+    # ocu_ns_step_params: nine int32 (padded to 40 bytes), eight doubles
+    assert C.sizeof(T.NSStepParams) == 40 + 8 * 8
+    assert T.NSStepParams.hx.offset == 40
 
 
 def test_host_only_entry_points_without_gpu():
