@@ -103,6 +103,42 @@ int ocu_bc_fill_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, double hx, 
 int ocu_bc_fill_mac_level1(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
                            const ocu_bc_set *bc, void *stream);
 
+/* apply_3d_boundary_conditions_level2_nocorners (grid3dboundary.cu:122-232,
+ * 904-938): the two ghost layers of each face (not edges or corners), from
+ * the interior: periodic copies, Dirichlet -phi0 + 2*value in both layers,
+ * Neumann phi0 -/+ h*value and phi0 -/+ 2*h*value.  Needs 2 ghost layers. */
+int ocu_bc_fill_level2_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, double hx, double hy,
+                                 double hz, void *stream);
+/* apply_3d_mac_boundary_conditions_level2 (grid3dboundary.cu:350-549,
+ * 940-1016): x, y, z passes over two ghost layers (periodic or forced
+ * inflow / variable slip).  nx is u->nx - 1: the reference computes
+ * u.nx() - u.gz(), which with the two ghost layers it requires rejects
+ * every congruent MAC triple. */
+int ocu_bc_fill_mac_level2(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w,
+                           const ocu_bc_set *bc, void *stream);
+
+/* ---------------- point sampling (ocustorage/grid3dsample.cu) ------------- */
+
+/* sample_points_3d (grid3dsample.cu:91-171): trilinear interpolation of phi
+ * (ghosts filled) at n float positions (device arrays, physical units,
+ * shifted by shift* cells), with periodic wrap (period_n* cells) or the
+ * clamp / Neumann extrapolation of the faces (periodic, Neumann, Dirichlet,
+ * scalar slip).  phi_sampled: n values of phi's type (device), each the
+ * float result of the lookup.  The cell index is kept in the first ghost
+ * layer (the reference reads one past it for a position exactly at the upper
+ * periodic end). */
+int ocu_sample_points(void *phi_sampled, const float *position_x, const float *position_y,
+                      const float *position_z, int n, const ocu_grid3d *phi, const ocu_bc_set *bc,
+                      float period_nx, float period_ny, float period_nz, float hx, float hy, float hz,
+                      float shiftx, float shifty, float shiftz, void *stream);
+/* sample_points_mac_grid_3d (grid3dsample.cu:174-263): u, v, w of a MAC
+ * velocity at n positions, with the reference's translation of the MAC
+ * faces (periodic or forced inflow) into sampling conditions. */
+int ocu_sample_points_mac(void *vx, void *vy, void *vz, const float *position_x, const float *position_y,
+                          const float *position_z, int n, const ocu_grid3d *u, const ocu_grid3d *v,
+                          const ocu_grid3d *w, const ocu_bc_set *bc, float hx, float hy, float hz,
+                          void *stream);
+
 /* ---------------- multigrid (ocuequation/sol_mgpressure3ddev*.cu) -------- */
 
 /* One colour (cells with (i+j+k)%2 == color) of red-black Gauss-Seidel,

@@ -70,6 +70,10 @@ HIP_SIGS = {
     "ocu_bc_fill_level1": (I, [GP, BP, D, D, D, P]),
     "ocu_bc_fill_nocorners": (I, [GP, BP, D, D, D, P]),
     "ocu_bc_fill_mac_level1": (I, [GP, GP, GP, BP, P]),
+    "ocu_bc_fill_level2_nocorners": (I, [GP, BP, D, D, D, P]),
+    "ocu_bc_fill_mac_level2": (I, [GP, GP, GP, BP, P]),
+    "ocu_sample_points": (I, [P, P, P, P, I, GP, BP] + [C.c_float] * 9 + [P]),
+    "ocu_sample_points_mac": (I, [P] * 6 + [I, GP, GP, GP, BP] + [C.c_float] * 3 + [P]),
     "ocu_mg_relax": (I, [GP, GP, I, MP, I, P]),
     "ocu_mg_relax_fused": (I, [GP, GP, GP, MP, BP, I, P]),
     "ocu_mg_relax_bc": (I, [GP, GP, I, MP, BP, D, D, D, P]),

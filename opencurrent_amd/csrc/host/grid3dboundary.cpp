@@ -45,6 +45,28 @@ bool apply_3d_mac_boundary_conditions_level1(Grid3DDevice<T> &u, Grid3DDevice<T>
                        "apply_3d_mac_boundary_conditions_level1");
 }
 
+template <typename T>
+bool apply_3d_boundary_conditions_level2_nocorners(Grid3DDevice<T> &grid, const BoundaryConditionSet &bc,
+                                                   double hx, double hy, double hz) {
+  ocu_grid3d d = grid.desc();
+  ocu_bc_set b = to_c_bc(bc);
+  return ocu_status_ok(ocu_bc_fill_level2_nocorners(&d, &b, hx, hy, hz, ThreadManager::get_compute_stream()),
+                       "apply_3d_boundary_conditions_level2_nocorners");
+}
+
+template <typename T>
+bool apply_3d_mac_boundary_conditions_level2(Grid3DDevice<T> &u, Grid3DDevice<T> &v, Grid3DDevice<T> &w,
+                                             const BoundaryConditionSet &bc, double, double, double) {
+  ocu_grid3d du = u.desc(), dv = v.desc(), dw = w.desc();
+  ocu_bc_set b = to_c_bc(bc);
+  return ocu_status_ok(ocu_bc_fill_mac_level2(&du, &dv, &dw, &b, ThreadManager::get_compute_stream()),
+                       "apply_3d_mac_boundary_conditions_level2");
+}
+
+template bool apply_3d_boundary_conditions_level2_nocorners(Grid3DDevice<float> &, const BoundaryConditionSet &, double, double, double);
+template bool apply_3d_boundary_conditions_level2_nocorners(Grid3DDevice<double> &, const BoundaryConditionSet &, double, double, double);
+template bool apply_3d_mac_boundary_conditions_level2(Grid3DDevice<float> &, Grid3DDevice<float> &, Grid3DDevice<float> &, const BoundaryConditionSet &, double, double, double);
+template bool apply_3d_mac_boundary_conditions_level2(Grid3DDevice<double> &, Grid3DDevice<double> &, Grid3DDevice<double> &, const BoundaryConditionSet &, double, double, double);
 template bool apply_3d_boundary_conditions_level1(Grid3DDevice<float> &, const BoundaryConditionSet &, double, double, double);
 template bool apply_3d_boundary_conditions_level1(Grid3DDevice<double> &, const BoundaryConditionSet &, double, double, double);
 template bool apply_3d_boundary_conditions_level1_nocorners(Grid3DDevice<float> &, const BoundaryConditionSet &, double, double, double);

@@ -19,6 +19,14 @@ bool apply_3d_boundary_conditions_level1_nocorners(Grid3DDevice<T> &grid, const 
 template <typename T>
 bool apply_3d_mac_boundary_conditions_level1(Grid3DDevice<T> &u, Grid3DDevice<T> &v, Grid3DDevice<T> &w,
                                              const BoundaryConditionSet &bc, double hx, double hy, double hz);
+// grid3dboundary.cu:904-938: two ghost layers per face (not edges or corners)
+template <typename T>
+bool apply_3d_boundary_conditions_level2_nocorners(Grid3DDevice<T> &grid, const BoundaryConditionSet &bc,
+                                                   double hx, double hy, double hz);
+// grid3dboundary.cu:940-1016, with nx = u.nx() - 1 (see ocu_bc_fill_mac_level2)
+template <typename T>
+bool apply_3d_mac_boundary_conditions_level2(Grid3DDevice<T> &u, Grid3DDevice<T> &v, Grid3DDevice<T> &w,
+                                             const BoundaryConditionSet &bc, double hx, double hy, double hz);
 
 }  // namespace ocu
 

@@ -8,6 +8,7 @@
 
 #include "oracle_ns.h"
 #include "oracle_pcg.h"
+#include "oracle_bc2.h"
 
 using namespace orc;
 
@@ -409,4 +410,38 @@ int orc_pcg_solve(const ocu_grid3d *rhs, const ocu_grid3d *coeff, const ocu_grid
                                      convergence, tol, max_iter, residual, iterations),
                   pcg_solve_t<double>(rhs, coeff, pressure, hx, hy, hz, bc, preconditioner, multigrid_cycles,
                                       use_fmg, convergence, tol, max_iter, residual, iterations));
+}
+
+int orc_bc_fill_level2_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, double hx, double hy, double hz) {
+  if (!g || !bc) return -1;
+  return DISPATCH(g->dtype, bc_fill_level2_nocorners<float>(*g, *bc, (float)hx, (float)hy, (float)hz) ? 1 : 0,
+                  bc_fill_level2_nocorners<double>(*g, *bc, hx, hy, hz) ? 1 : 0);
+}
+
+int orc_bc_fill_mac_level2(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc) {
+  if (!u || !v || !w || !bc) return -1;
+  return DISPATCH(u->dtype, bc_fill_mac_level2<float>(*u, *v, *w, *bc) ? 1 : 0,
+                  bc_fill_mac_level2<double>(*u, *v, *w, *bc) ? 1 : 0);
+}
+
+int orc_sample_points(void *out, const float *px, const float *py, const float *pz, int n, const ocu_grid3d *phi,
+                      const ocu_bc_set *bc, float period_nx, float period_ny, float period_nz, float hx, float hy,
+                      float hz, float shiftx, float shifty, float shiftz) {
+  if (!out || !phi || !bc) return -1;
+  return DISPATCH(phi->dtype,
+                  sample_points<float>((float *)out, px, py, pz, n, *phi, *bc, period_nx, period_ny, period_nz, hx,
+                                       hy, hz, shiftx, shifty, shiftz) ? 1 : 0,
+                  sample_points<double>((double *)out, px, py, pz, n, *phi, *bc, period_nx, period_ny, period_nz, hx,
+                                        hy, hz, shiftx, shifty, shiftz) ? 1 : 0);
+}
+
+int orc_sample_points_mac(void *vx, void *vy, void *vz, const float *px, const float *py, const float *pz, int n,
+                          const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc,
+                          float hx, float hy, float hz) {
+  if (!vx || !vy || !vz || !u || !v || !w || !bc) return -1;
+  return DISPATCH(u->dtype,
+                  sample_points_mac<float>((float *)vx, (float *)vy, (float *)vz, px, py, pz, n, *u, *v, *w, *bc, hx,
+                                           hy, hz) ? 1 : 0,
+                  sample_points_mac<double>((double *)vx, (double *)vy, (double *)vz, px, py, pz, n, *u, *v, *w, *bc,
+                                            hx, hy, hz) ? 1 : 0);
 }

@@ -134,6 +134,17 @@ int orc_pcg_solve(const ocu_grid3d *rhs, const ocu_grid3d *coeff, const ocu_grid
                   float hz, const ocu_bc_set *bc, int preconditioner, int multigrid_cycles, int use_fmg,
                   int convergence, double tol, int max_iter, double *residual, int *iterations);
 
+/* two-layer ghost fills (grid3dboundary.cu:904-1016) and point sampling
+ * (grid3dsample.cu); positions are float host arrays, out has phi's type */
+int orc_bc_fill_level2_nocorners(const ocu_grid3d *g, const ocu_bc_set *bc, double hx, double hy, double hz);
+int orc_bc_fill_mac_level2(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc);
+int orc_sample_points(void *out, const float *px, const float *py, const float *pz, int n, const ocu_grid3d *phi,
+                      const ocu_bc_set *bc, float period_nx, float period_ny, float period_nz, float hx, float hy,
+                      float hz, float shiftx, float shifty, float shiftz);
+int orc_sample_points_mac(void *vx, void *vy, void *vz, const float *px, const float *py, const float *pz, int n,
+                          const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d *w, const ocu_bc_set *bc,
+                          float hx, float hy, float hz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,10 @@ SIGS = {
     "orc_pcg_dot": (D, [GP, GP]),
     "orc_pcg_solve": (I, [GP, GP, GP, C.c_float, C.c_float, C.c_float, BP, I, I, I, I, D, I, DP, IP]),
     "orc_scalar_advect_step": (I, [GP, GP, GP, GP, GP, BP, I, I, I, D, D, D, I, D]),
+    "orc_bc_fill_level2_nocorners": (I, [GP, BP, D, D, D]),
+    "orc_bc_fill_mac_level2": (I, [GP, GP, GP, BP]),
+    "orc_sample_points": (I, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, I, GP, BP] + [C.c_float] * 9),
+    "orc_sample_points_mac": (I, [C.c_void_p] * 6 + [I, GP, GP, GP, BP] + [C.c_float] * 3),
 }
 
 _lib = None

@@ -1,5 +1,6 @@
 """The reference's C++ API as a drop-in: callers written against the ocu
-headers (tests/cpp/mg_dropin.cpp, tests/cpp/advection_dropin.cpp) compile
+headers (tests/cpp/mg_dropin.cpp, tests/cpp/advection_dropin.cpp,
+tests/cpp/pcg_sample_dropin.cpp) compile
 and link against libocu.so + libocu_hip.so unchanged, and on the GPU solve
 their Poisson problem and pass the reference advection tests' symmetry,
 conservation and band checks."""
@@ -22,7 +23,7 @@ def build(tmp_path, name):
     return exe
 
 
-@pytest.mark.parametrize("name", ["mg_dropin", "advection_dropin"])
+@pytest.mark.parametrize("name", ["mg_dropin", "advection_dropin", "pcg_sample_dropin"])
 def test_dropin_caller_compiles_and_links(tmp_path, name):
     assert build(tmp_path, name).exists()
 
@@ -47,3 +48,14 @@ def test_dropin_scalar_advection_matches_reference_checks(gpu, tmp_path, scheme)
     p = subprocess.run([str(exe), str(scheme)], capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "ok=0" not in p.stdout
+
+
+@pytest.mark.gpu
+def test_dropin_pcg_and_sampling(gpu, tmp_path):
+    """Sol_PCGPressure3DDeviceD (multigrid preconditioner) returns minus the
+    eigenmode behind its rhs; a two-layer Neumann fill of a linear field
+    followed by sample_points_3d returns the field anywhere in the box."""
+    exe = build(tmp_path, "pcg_sample_dropin")
+    p = subprocess.run([str(exe), "32"], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.count("ok=1") == 2, p.stdout
