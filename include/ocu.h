@@ -102,6 +102,28 @@ int ocu_pcg_iterations(void *h);
 int ocu_pcg_pressure(void *h, ocu_grid3d *out);
 void ocu_pcg_destroy(void *h);
 
+/* ---- NetCDFGrid3DWriter / NetCDFGrid3DReader (ocustorage/gridnetcdf.h:67-183) ----
+ * Grid fields in NetCDF classic files (64-bit offset variant written; both
+ * variants read).  Grids are host descriptors (base = host pointer); the
+ * element type picks the variable type (OCU_F32 -> NC_FLOAT 5, OCU_F64 ->
+ * NC_DOUBLE 6, OCU_I32 -> NC_INT 4).  staggering: 0 cell centre, 1 u face,
+ * 2 v face, 3 w face, 4 custom (the offsets given).  The writer's file is
+ * complete after ocu_nc_writer_close, which also frees the handle. */
+void *ocu_nc_writer_open(const char *filename, int nx, int ny, int nz, float hx, float hy, float hz);
+int ocu_nc_writer_define_variable(void *h, const char *name, int nc_type, int staggering, float offsetx,
+                                  float offsety, float offsetz);
+int ocu_nc_writer_add_time_level(void *h, float time, int *level);
+int ocu_nc_writer_add_data(void *h, const char *name, const ocu_grid3d *host_grid, int time_level);
+int ocu_nc_writer_close(void *h);
+void *ocu_nc_reader_open(const char *filename);
+int ocu_nc_reader_info(void *h, int *nx, int *ny, int *nz, float *hx, float *hy, float *hz, int *num_time_levels);
+int ocu_nc_reader_time(void *h, int time_level, float *time);
+/* index < 0 returns the number of 4-D variables; else its name (NUL
+ * terminated, name_cap bytes available), type and sizes */
+int ocu_nc_reader_variable(void *h, int index, char *name, int name_cap, int *nc_type, int *nx, int *ny, int *nz);
+int ocu_nc_reader_read(void *h, const char *name, const ocu_grid3d *host_grid, int time_level);
+void ocu_nc_reader_close(void *h);
+
 /* ---- Sol_ProjectDivergence3DDevice<T> (ocuequation/sol_project3d.h:74-93) ----
  * u, v, w: congruent MAC grids (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1).
  * mixed = 1 (FP64 u) uses the mixed-precision pressure solver. */

@@ -175,6 +175,17 @@ HOST_SIGS = {
     "ocu_project_divergence": (I, [P, GP]),
     "ocu_project_pressure": (I, [P, GP]),
     "ocu_project_destroy": (None, [P]),
+    "ocu_nc_writer_open": (P, [C.c_char_p, I, I, I, C.c_float, C.c_float, C.c_float]),
+    "ocu_nc_writer_define_variable": (I, [P, C.c_char_p, I, I, C.c_float, C.c_float, C.c_float]),
+    "ocu_nc_writer_add_time_level": (I, [P, C.c_float, IP]),
+    "ocu_nc_writer_add_data": (I, [P, C.c_char_p, GP, I]),
+    "ocu_nc_writer_close": (I, [P]),
+    "ocu_nc_reader_open": (P, [C.c_char_p]),
+    "ocu_nc_reader_info": (I, [P, IP, IP, IP, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), IP]),
+    "ocu_nc_reader_time": (I, [P, I, C.POINTER(C.c_float)]),
+    "ocu_nc_reader_variable": (I, [P, I, C.c_char_p, I, IP, IP, IP, IP]),
+    "ocu_nc_reader_read": (I, [P, C.c_char_p, GP, I]),
+    "ocu_nc_reader_close": (None, [P]),
     "ocu_pcg_create": (P, [GP, GP, C.c_float, C.c_float, C.c_float, BP, I, I, I, I]),
     "ocu_pcg_solve": (I, [P, D, I, DP]),
     "ocu_pcg_iterations": (I, [P]),

@@ -59,13 +59,15 @@ __device__ __forceinline__ T untouched(const NSStep<T> &s, int grid, T old) {
   return ((s.flat >> grid) & 1) ? combine(s, old, (T)0, (T)0) : old;
 }
 
-// T's tendency at interior cell c: passive advection, then + kappa*lap(T)
+// T's tendency at interior cell c (offset from cell (0,0,0)): passive
+// advection, then + kappa*lap(T)
 template <typename T>
 __device__ __forceinline__ T temp_tendency(const NSStep<T> &s, int64_t c, int i, int j, int k) {
+  const T *U = s.u + s.shift, *V = s.v + s.shift, *W = s.w + s.shift, *Tt = s.t + s.shift;
   T d = s.scheme == OCU_IT_SECOND_ORDER_UPWIND
-            ? passive_advect_up2_cell(s.u, s.v, s.w, s.t, c, s.xs, s.ys, s.ih, i, j, k, s.nx, s.ny, s.nz, s.tper)
-            : passive_advect_cell(s.u, s.v, s.w, s.t, c, s.xs, s.ys, s.ih, s.scheme);
-  if (s.tdiff) d = d + s.kappa * laplacian_cell(s.t, c, s.xs, s.ys, s.ih2);
+            ? passive_advect_up2_cell(U, V, W, Tt, c, s.xs, s.ys, s.ih, i, j, k, s.nx, s.ny, s.nz, s.tper)
+            : passive_advect_cell(U, V, W, Tt, c, s.xs, s.ys, s.ih, s.scheme);
+  if (s.tdiff) d = d + s.kappa * laplacian_cell(Tt, c, s.xs, s.ys, s.ih2);
   return d;
 }
 

@@ -11,6 +11,7 @@
 #include "ocuequation/sol_mgpressure3d.h"
 #include "ocuequation/sol_mgpressuremixed3d.h"
 #include "ocuequation/sol_pcgpressure3d.h"
+#include "ocustorage/gridnetcdf.h"
 #include "ocuequation/sol_project3d.h"
 #include "ocuutil/thread.h"
 
@@ -602,5 +603,135 @@ int ocu_pcg_pressure(void *hh, ocu_grid3d *out) {
 }
 
 void ocu_pcg_destroy(void *hh) { delete (PCGHandle *)hh; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- NetCDF
+
+namespace {
+// the layout of a descriptor, for the writer's and reader's strided access
+struct DescLayout : public Grid3DUntyped {
+  explicit DescLayout(const ocu_grid3d &d) {
+    _nx = d.nx, _ny = d.ny, _nz = d.nz;
+    _gx = d.gx, _gy = d.gy, _gz = d.gz;
+    _pnx = d.pnx, _pny = d.pny, _pnz = d.pnz;
+    _pnzpny = d.pnz * d.pny;
+    _shift_amount = (int)d.shift;
+    _allocated_elements = (int)d.allocated;
+  }
+};
+nc_type nc_of(int dtype) { return dtype == OCU_F64 ? NC_DOUBLE : dtype == OCU_F32 ? NC_FLOAT : NC_INT; }
+size_t atom_of(int dtype) { return dtype == OCU_F64 ? 8 : 4; }
+}  // namespace
+
+extern "C" {
+
+void *ocu_nc_writer_open(const char *filename, int nx, int ny, int nz, float hx, float hy, float hz) {
+  NetCDFGrid3DWriter *w = new (std::nothrow) NetCDFGrid3DWriter();
+  if (!w) return nullptr;
+  if (!w->open(filename, nx, ny, nz, hx, hy, hz)) {
+    delete w;
+    return nullptr;
+  }
+  return w;
+}
+
+int ocu_nc_writer_define_variable(void *h, const char *name, int nc_type_, int staggering, float offsetx,
+                                  float offsety, float offsetz) {
+  if (!h || staggering < 0 || staggering > 4) return OCU_ERR_INVALID;
+  return ((NetCDFGrid3DWriter *)h)->define_variable(name, nc_type_, (GridStaggering)staggering, offsetx, offsety,
+                                                    offsetz)
+             ? OCU_OK
+             : OCU_ERR_INVALID;
+}
+
+int ocu_nc_writer_add_time_level(void *h, float time, int *level) {
+  if (!h) return OCU_ERR_INVALID;
+  size_t l = 0;
+  if (!((NetCDFGrid3DWriter *)h)->add_time_level(time, l)) return OCU_ERR_INVALID;
+  if (level) *level = (int)l;
+  return OCU_OK;
+}
+
+int ocu_nc_writer_add_data(void *h, const char *name, const ocu_grid3d *host_grid, int time_level) {
+  if (!h || !host_grid || !host_grid->base) return OCU_ERR_INVALID;
+  DescLayout L(*host_grid);
+  const void *interior = (const char *)host_grid->base + atom_of(host_grid->dtype) * (size_t)host_grid->shift;
+  return ((NetCDFGrid3DWriter *)h)->add_data_layout(name, L, nc_of(host_grid->dtype), interior, time_level)
+             ? OCU_OK
+             : OCU_ERR_INVALID;
+}
+
+int ocu_nc_writer_close(void *h) {
+  if (!h) return OCU_ERR_INVALID;
+  NetCDFGrid3DWriter *w = (NetCDFGrid3DWriter *)h;
+  const bool ok = w->close();
+  delete w;
+  return ok ? OCU_OK : OCU_ERR_RUNTIME;
+}
+
+void *ocu_nc_reader_open(const char *filename) {
+  NetCDFGrid3DReader *r = new (std::nothrow) NetCDFGrid3DReader();
+  if (!r) return nullptr;
+  if (!r->open(filename)) {
+    delete r;
+    return nullptr;
+  }
+  return r;
+}
+
+int ocu_nc_reader_info(void *h, int *nx, int *ny, int *nz, float *hx, float *hy, float *hz, int *num_time_levels) {
+  if (!h) return OCU_ERR_INVALID;
+  NetCDFGrid3DReader *r = (NetCDFGrid3DReader *)h;
+  if (nx) *nx = r->nx();
+  if (ny) *ny = r->ny();
+  if (nz) *nz = r->nz();
+  if (hx) *hx = r->hx();
+  if (hy) *hy = r->hy();
+  if (hz) *hz = r->hz();
+  if (num_time_levels) *num_time_levels = r->num_time_levels();
+  return OCU_OK;
+}
+
+int ocu_nc_reader_time(void *h, int time_level, float *time) {
+  NetCDFGrid3DReader *r = (NetCDFGrid3DReader *)h;
+  if (!r || !time || time_level < 0 || time_level >= r->num_time_levels()) return OCU_ERR_INVALID;
+  *time = r->get_time(time_level);
+  return OCU_OK;
+}
+
+int ocu_nc_reader_variable(void *h, int index, char *name, int name_cap, int *nc_type_, int *nx, int *ny, int *nz) {
+  NetCDFGrid3DReader *r = (NetCDFGrid3DReader *)h;
+  if (!r) return OCU_ERR_INVALID;
+  const std::vector<std::string> names = r->list_variables();
+  if (index < 0) return (int)names.size();
+  if (index >= (int)names.size() || !name || name_cap <= (int)names[index].size()) return OCU_ERR_INVALID;
+  std::memcpy(name, names[index].c_str(), names[index].size() + 1);
+  nc_type t = 0;
+  int a = 0, b = 0, c = 0;
+  r->variable_type(names[index].c_str(), t);
+  r->variable_size(names[index].c_str(), a, b, c);
+  if (nc_type_) *nc_type_ = t;
+  if (nx) *nx = a;
+  if (ny) *ny = b;
+  if (nz) *nz = c;
+  return OCU_OK;
+}
+
+int ocu_nc_reader_read(void *h, const char *name, const ocu_grid3d *host_grid, int time_level) {
+  if (!h || !host_grid || !host_grid->base) return OCU_ERR_INVALID;
+  DescLayout L(*host_grid);
+  void *interior = (char *)host_grid->base + atom_of(host_grid->dtype) * (size_t)host_grid->shift;
+  return ((NetCDFGrid3DReader *)h)->read_variable_layout(name, L, nc_of(host_grid->dtype), interior, time_level)
+             ? OCU_OK
+             : OCU_ERR_INVALID;
+}
+
+void ocu_nc_reader_close(void *h) {
+  NetCDFGrid3DReader *r = (NetCDFGrid3DReader *)h;
+  if (!r) return;
+  if (r->is_open()) r->close();
+  delete r;
+}
 
 }  // extern "C"

@@ -352,6 +352,7 @@ template <typename T> bool Grid3DDevice<T>::reduce_checknan(T &r) const { double
 
 template class Grid3DHost<float>;
 template class Grid3DHost<double>;
+template class Grid3DHost<int>;
 template class Grid3DDevice<float>;
 template class Grid3DDevice<double>;
 

@@ -4,11 +4,13 @@
 // runs on the GPU.
 //
 // 1. Sol_PCGPressure3DDeviceD with the multigrid preconditioner on an n^3
-//    all-Neumann problem with coefficient 1: the answer must be minus the
-//    eigenmode the rhs was built from (the operator is -Laplacian).
+//    all-Neumann problem with coefficient 1: the operator is -Laplacian, so
+//    a rhs lam * mode (lam the positive eigenvalue) gives back the mode.
 // 2. apply_3d_boundary_conditions_level2_nocorners on a linear field with
-//    matching Neumann gradients, then sample_points_3d anywhere in the box
-//    (near a face the lookup reads the ghosts): the linear field comes back.
+//    matching Neumann gradients, then sample_points_3d across the whole box
+//    in x (near the x faces the lookup reads the face ghosts) and inside the
+//    cell-centre hull in y and z (the fill leaves edges): the linear field
+//    comes back.
 // Prints one line per part ("... ok=1") and exits 0 when both pass.
 #include <cmath>
 #include <cstdio>
@@ -51,7 +53,7 @@ static int pcg_part(int n) {
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++)
       for (int k = 0; k < n; k++)
-        err = fmax(err, fabs(hp.at(i, j, k) - mean + cos(pi * (i + .5) / n) * cos(2 * pi * (j + .5) / n)));
+        err = fmax(err, fabs(hp.at(i, j, k) - mean - cos(pi * (i + .5) / n) * cos(2 * pi * (j + .5) / n)));
   printf("pcg n=%d ok=%d residual=%.3e iterations=%d max_err=%.3e\n", n, (int)(ok && err < 1e-6), residual,
          solver.iterations(), err);
   return ok && err < 1e-6 ? 0 : 1;
@@ -79,11 +81,9 @@ static int sample_part() {
   if (!px.init(m, 0) || !py.init(m, 0) || !pz.init(m, 0) || !out.init(m, 0)) return 5;
   srand(11);
   for (int q = 0; q < m; q++) {
-    // anywhere in the box: within half a cell of a face the lookup reads
-    // the first ghost layer, which the Neumann fill continued linearly
     px.at(q) = (float)(nx * (rand() / (double)RAND_MAX) * hx);
-    py.at(q) = (float)(ny * (rand() / (double)RAND_MAX) * hy);
-    pz.at(q) = (float)(nz * (rand() / (double)RAND_MAX) * hz);
+    py.at(q) = (float)((.5 + (ny - 1) * (rand() / (double)RAND_MAX)) * hy);
+    pz.at(q) = (float)((.5 + (nz - 1) * (rand() / (double)RAND_MAX)) * hz);
   }
   Grid1DDevice<float> dx, dy, dz;
   Grid1DDeviceD dout;

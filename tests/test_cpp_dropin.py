@@ -52,9 +52,10 @@ def test_dropin_scalar_advection_matches_reference_checks(gpu, tmp_path, scheme)
 
 @pytest.mark.gpu
 def test_dropin_pcg_and_sampling(gpu, tmp_path):
-    """Sol_PCGPressure3DDeviceD (multigrid preconditioner) returns minus the
+    """Sol_PCGPressure3DDeviceD (multigrid preconditioner) returns the
     eigenmode behind its rhs; a two-layer Neumann fill of a linear field
-    followed by sample_points_3d returns the field anywhere in the box."""
+    followed by sample_points_3d returns the field, also within half a cell
+    of the x faces."""
     exe = build(tmp_path, "pcg_sample_dropin")
     p = subprocess.run([str(exe), "32"], capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stdout + p.stderr
