@@ -1308,6 +1308,37 @@ __global__ __launch_bounds__(256) void k_coarse(DGrid<TS> ug, DGrid<TS> bg, Coar
   }
 }
 
+// A coarsest level whose boxes exceed one workgroup's LDS (a grid that
+// coarsens little or not at all): the same iterations by one workgroup on
+// global memory.  MODE 0 works on the grids in place (u and b share one
+// layout); MODE 1 on double boxes wu, wb of a temporary, loaded from and
+// stored back to the float grids as k_coarse does with LDS.  A workgroup
+// runs on one CU, so its barriers order its global accesses as they order
+// LDS ones.
+template <typename TS, typename TC, int MODE>
+__global__ __launch_bounds__(1024) void k_coarse_global(DGrid<TS> ug, DGrid<TS> bg, TC *wu, TC *wb, CoarseGeom g,
+                                                        HostC<TC> c, FacesS F, TC hx, TC hy, TC hz, int iterations,
+                                                        int order) {
+  const BlockTeam tm;
+  const int box = (g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  if (MODE == 1) {
+    for (int t = tm.rank(); t < box; t += tm.size()) {
+      int k = t % (g.nz + 2) - 1, j = (t / (g.nz + 2)) % (g.ny + 2) - 1, i = t / ((g.nz + 2) * (g.ny + 2)) - 1;
+      wu[t] = (TC)ug.at(i, j, k);
+      wb[t] = (TC)bg.at(i, j, k);
+    }
+    tm.sync();
+  }
+  coarse_iterations<TC, MODE>(tm, wu, wb, g, c, F, hx, hy, hz, iterations, order);
+  if (MODE == 1) {
+    tm.sync();
+    for (int t = tm.rank(); t < box; t += tm.size()) {
+      int k = t % (g.nz + 2) - 1, j = (t / (g.nz + 2)) % (g.ny + 2) - 1, i = t / ((g.nz + 2) * (g.ny + 2)) - 1;
+      ug.at(i, j, k) = (TS)wu[t];
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // the bottom of a V-cycle in one workgroup (LDS resident)
 // ------------------------------------------------------------------------
@@ -2309,6 +2340,39 @@ int ocu_mg_prolong_bc(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int 
   return prolong(u_coarse, u_fine, quirk, bc, hx, hy, hz, stream);
 }
 
+// ocu_mg_coarse_relax for boxes beyond one workgroup's LDS
+static int coarse_relax_global(const ocu_grid3d *u, const ocu_grid3d *b, CoarseGeom g, size_t box, int iterations,
+                               int order, const ocu_mg_level_params *p, const FacesS &F, double hx, double hy,
+                               double hz, int mode, void *stream) {
+  OCU_REQUIRE(box < (1u << 30), "ocu_mg_coarse_relax: grid %dx%dx%d too large", g.nx, g.ny, g.nz);
+  if (mode == 1) {
+    double *w = nullptr;
+    OCU_HIP_TRY(hipMallocAsync((void **)&w, 2 * box * sizeof(double), S(stream)));
+    k_coarse_global<float, double, 1><<<1, 1024, 0, S(stream)>>>(dgrid<float>(u), dgrid<float>(b), w, w + box, g,
+                                                                 host_c<double>(p), F, hx, hy, hz, iterations, order);
+    const hipError_t le = hipGetLastError();
+    OCU_HIP_TRY(hipFreeAsync(w, S(stream)));
+    OCU_HIP_TRY(le);
+    return OCU_OK;
+  }
+  OCU_REQUIRE(layout_match(u, b), "ocu_mg_coarse_relax: u and b layouts differ (needed beyond LDS size)");
+  const int64_t xs = (int64_t)u->pny * u->pnz;
+  OCU_REQUIRE(xs < (1ll << 30), "ocu_mg_coarse_relax: grid %dx%dx%d too large", g.nx, g.ny, g.nz);
+  g.lxs = (int)xs;
+  g.lys = u->pnz;
+  g.lshift = 0;
+  if (u->dtype == OCU_F64) {
+    DGrid<double> du = dgrid<double>(u), db = dgrid<double>(b);
+    k_coarse_global<double, double, 0><<<1, 1024, 0, S(stream)>>>(du, db, du.p, db.p, g, host_c<double>(p), F, hx,
+                                                                  hy, hz, iterations, order);
+  } else {
+    DGrid<float> du = dgrid<float>(u), db = dgrid<float>(b);
+    k_coarse_global<float, float, 0><<<1, 1024, 0, S(stream)>>>(du, db, du.p, db.p, g, host_c<float>(p), F,
+                                                                (float)hx, (float)hy, (float)hz, iterations, order);
+  }
+  OCU_HIP_LAUNCHED("ocu_mg_coarse_relax");
+}
+
 int ocu_mg_coarse_relax(const ocu_grid3d *u, const ocu_grid3d *b, int iterations, int order,
                         const ocu_mg_level_params *p, const ocu_bc_set *bc, double hx, double hy, double hz,
                         int mode, void *stream) {
@@ -2328,9 +2392,8 @@ int ocu_mg_coarse_relax(const ocu_grid3d *u, const ocu_grid3d *b, int iterations
   size_t box = (size_t)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
   size_t tsize = (mode == 1 || u->dtype == OCU_F64) ? 8 : 4;
   size_t shmem = 2 * box * tsize;
-  OCU_REQUIRE(shmem <= BOT_LDS_MAX, "ocu_mg_coarse_relax: grid %dx%dx%d too large for one workgroup", g.nx, g.ny,
-              g.nz);
   FacesS F = faces_s(bc);
+  if (shmem > BOT_LDS_MAX) return coarse_relax_global(u, b, g, box, iterations, order, p, F, hx, hy, hz, mode, stream);
   // dynamic LDS beyond 64 KiB is requested explicitly
   const void *kc = mode == 1 ? reinterpret_cast<const void *>(&k_coarse<float, double, 1>)
                    : u->dtype == OCU_F64 ? reinterpret_cast<const void *>(&k_coarse<double, double, 0>)

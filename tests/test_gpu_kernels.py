@@ -481,14 +481,26 @@ def test_relax_fused_prolong_rejects(gpu):
         D().mg_relax_fused_prolong_bc(uf, uf, b, uc, p, bc, .1, .1, .1)
 
 
+def coarse_colors(iters, order):
+    it_n = max(4, iters + (iters % 2)) if order == 2 else iters
+    for it in range(it_n):
+        for rb in (0, 1):
+            yield 1 - rb if order == 1 or (order == 2 and it >= it_n // 2) else rb
+
+
+# (4, 6, 4): boxes in one workgroup's LDS; (30, 28, 24): boxes of 200 KB in
+# FP32 and twice that in FP64, relaxed in global memory
+COARSE_SIZES = [(4, 6, 4), (30, 28, 24)]
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("case", ["neumann0", "dirichlet", "mixed", "neumann_v"])
 @pytest.mark.parametrize("order", [0, 1, 2])
-def test_coarse_relax_reference_semantics(gpu, dtype, case, order):
+@pytest.mark.parametrize("cn", COARSE_SIZES)
+def test_coarse_relax_reference_semantics(gpu, dtype, case, order, cn):
     """relax_on_host (sol_mgpressure3ddev.cu:134-188): host arithmetic, host
     no-corner fill after each colour, level-1 fill at the end."""
     bc = BC_CASES[case]
-    cn = (4, 6, 4)
     h = .5
     u = filled(*cn, dtype, 71, bc, h)
     b = random_grid(*cn, dtype, 72)
@@ -509,6 +521,32 @@ def test_coarse_relax_reference_semantics(gpu, dtype, case, order):
     gu, gb = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(b)
     D().mg_coarse_relax(gu, gb, iters, order, p, bc, h, h, h, mode=0)
     assert_bitexact(gu.to_host(), want, f"coarse {case} order {order}")
+
+
+@pytest.mark.parametrize("case", ["neumann0", "dirichlet", "mixed"])
+@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("cn", COARSE_SIZES)
+def test_coarse_relax_double_on_float(gpu, case, order, cn):
+    """mode 1 (the mixed solver's coarsest level): FP32 grids relaxed in FP64
+    with the host arithmetic and a level-1 fill after each colour, rounded
+    back to FP32 once at the end - the oracle's FP64 relaxation of the
+    widened grids, rounded."""
+    bc = BC_CASES[case]
+    h = .5
+    u = filled(*cn, np.float32, 73, bc, h)
+    b = random_grid(*cn, np.float32, 74)
+    p = O.level_params(bc, 0, h, h, h)
+    iters = 6
+    ud, bd = HostGrid.congruent(u, dtype=np.float64), HostGrid.congruent(b, dtype=np.float64)
+    ud.buf[:], bd.buf[:] = u.buf, b.buf
+    for color in coarse_colors(iters, order):
+        O.lib().orc_mg_relax(ud.ptr(), bd.ptr(), color, C.byref(p), T.RELAX_HOST)
+        O.lib().orc_bc_fill(0, ud.ptr(), C.byref(bc), h, h, h)
+    want = u.copy()
+    want.buf[:] = ud.buf.astype(np.float32)
+    gu, gb = D().DeviceGrid.from_host(u), D().DeviceGrid.from_host(b)
+    D().mg_coarse_relax(gu, gb, iters, order, p, bc, h, h, h, mode=1)
+    assert_bitexact(gu.to_host(), want, f"coarse mode 1 {case} order {order}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
