@@ -88,6 +88,15 @@ int ocu_grid_add_scalar(const ocu_grid3d *g, double value, void *stream);
  * hold ocu_reduce_workspace_bytes() bytes of device memory. */
 size_t ocu_reduce_workspace_bytes(void);
 int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, void *stream);
+/* ocu_grid_reduce without the workspace argument and without any allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP-graph capture, tight serving loops).  The partials
+ * live in a library-owned array of NORM_PARTS = 2048 doubles (16 KiB) per
+ * GPU, held until the process ends; every grid ocu_grid_reduce accepts fits
+ * (the partial count does not depend on the grid size; grids with nx*ny >=
+ * INT_MAX rows are rejected as there).  Calls that overlap on one GPU (two
+ * streams or two host threads) race on that array. */
+int ocu_grid_reduce_noalloc(const ocu_grid3d *g, int op, double *out, void *stream);
 
 /* ---------------- ghost-cell fills (ocustorage/grid3dboundary.cu) -------- */
 

@@ -67,6 +67,7 @@ HIP_SIGS = {
     "ocu_grid_add_scalar": (I, [GP, D, P]),
     "ocu_reduce_workspace_bytes": (C.c_size_t, []),
     "ocu_grid_reduce": (I, [GP, I, P, DP, P]),
+    "ocu_grid_reduce_noalloc": (I, [GP, I, DP, P]),
     "ocu_bc_fill_level1": (I, [GP, BP, D, D, D, P]),
     "ocu_bc_fill_nocorners": (I, [GP, BP, D, D, D, P]),
     "ocu_bc_fill_mac_level1": (I, [GP, GP, GP, BP, P]),

@@ -3,6 +3,7 @@
 #include "ocu_hip_common.h"
 
 #include <climits>
+#include <mutex>
 
 using namespace ocu_hip;
 
@@ -168,7 +169,39 @@ __global__ void k_reduce_stage2(int op, const double *parts, int n, double *out)
 
 }  // namespace
 
+// The partials of ocu_grid_reduce_noalloc: NORM_PARTS doubles (16 KiB) per
+// GPU, held until the process ends.  On the host this name is only a
+// stand-in for the symbol; kernels get the device address that
+// scratch_on_current_device() looked up.  Every call on one GPU reuses the
+// same array, so calls that overlap on that GPU (two streams, two host
+// threads) race on it.
+static __device__ __align__(256) double scratch[NORM_PARTS];
+
 namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// device address of `scratch` on the current device, looked up once per
+// device with that device current (each GPU holds its own copy)
+int scratch_on_current_device(double **out) {
+  int dev = 0;
+  OCU_HIP_TRY(hipGetDevice(&dev));
+  OCU_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "no-allocation scratch: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+      return OCU_ERR_RUNTIME;
+    }
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return OCU_OK;
+}
 
 // the two launches of an interior reduction: NORM_PARTS partials into
 // `parts`, folded into *out (device memory)
@@ -299,6 +332,13 @@ int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, v
   if (int e = check_reduce(g, op, out, "ocu_grid_reduce")) return e;
   OCU_REQUIRE(workspace, "ocu_grid_reduce: null workspace");
   return launch_reduce(g, op, (double *)workspace, out, stream, "ocu_grid_reduce");
+}
+
+int ocu_grid_reduce_noalloc(const ocu_grid3d *g, int op, double *out, void *stream) {
+  if (int e = check_reduce(g, op, out, "ocu_grid_reduce_noalloc")) return e;
+  double *parts = nullptr;
+  if (int e = scratch_on_current_device(&parts)) return e;
+  return launch_reduce(g, op, parts, out, stream, "ocu_grid_reduce_noalloc");
 }
 
 int ocu_grid_sum(const ocu_grid3d *g, double *ws, double *out, void *stream) {
