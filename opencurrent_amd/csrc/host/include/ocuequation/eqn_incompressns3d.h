@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/eqn_incompressns3d.h: Boussinesq incompressible Navier-Stokes
 // on a MAC grid (eqn_incompressns3d.cu)
 #ifndef __OCU_EQUATION_EQN_INCOMPRESS_NS_3D_H__

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/eqn_scalaradvection3d.h: forward-Euler passive advection of a
 // scalar by a fixed MAC velocity (eqn_scalaradvection3d.cpp)
 #ifndef __OCU_EQUATION_EQN_SCALAR_ADVECTION_3D_H__

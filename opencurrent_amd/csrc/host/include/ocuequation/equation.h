@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/equation.h: Equation::advance substeps by the stable step
 #ifndef __OCU_EQUATION_EQUATION_H__
 #define __OCU_EQUATION_EQUATION_H__

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/error_handler.h:27-59: bool returns plus a sticky error flag
 #ifndef __OCU_EQUATION_ERROR_HANDLER_H__
 #define __OCU_EQUATION_ERROR_HANDLER_H__

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/sol_laplaciancent3d.h: dphidt += c * Laplacian(phi)
 #ifndef __OCU_EQUATION_LAPLACIANCENT_3D_H__
 #define __OCU_EQUATION_LAPLACIANCENT_3D_H__

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Multigrid pressure solver, the drop-in for ocuequation/sol_mgpressure3d.h.
 //
 // Sol_MultigridPressure3DBase keeps the reference's FMG / V-cycle driver

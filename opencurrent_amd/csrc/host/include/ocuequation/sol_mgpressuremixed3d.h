@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Mixed-precision pressure solver with the public API of
 // ocuequation/sol_mgpressuremixed3d.h (a drop-in for the FP64 solver).
 //

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/sol_passiveadvection3d.h: flux-form advection of a cell
 // scalar by a MAC velocity (sol_passiveadvection3ddev.cu)
 #ifndef __OCU_EQUATION_PASSIVEADVECTION_3D_H__

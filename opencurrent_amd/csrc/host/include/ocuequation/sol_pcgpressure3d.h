@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Preconditioned conjugate-gradient pressure solver, the drop-in for
 // ocuequation/sol_pcgpressure3d.h: solves -div(c grad p) = rhs for a
 // variable coefficient c with periodic or Neumann faces, preconditioned by

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/sol_project3d.h: Helmholtz projection of a MAC velocity
 // (sol_project3d.cpp, sol_project3ddev.cu).  Sol_ProjectDivergence3DDevice<T>
 // uses Sol_MultigridPressure3DDevice<T>; Sol_ProjectDivergenceMixed3DDeviceD

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/sol_selfadvection3d.h: flux-form self-advection of a MAC
 // velocity (sol_selfadvection3ddev.cu)
 #ifndef __OCU_EQUATION_SELFADVECTION_3D_H__

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuequation/solver.h: the Solver base.  Launch checking goes through the
 // C ABI status codes; PreKernel/PostKernel are kept for source compatibility
 // and never synchronise.

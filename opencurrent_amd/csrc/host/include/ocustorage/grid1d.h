@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // One-dimensional grids, the drop-in for ocustorage/grid1d.h: nx cells with
 // gx ghost cells on each side, padded to a multiple of 16 (4-byte types) or
 // 8 (8-byte types) elements (grid1ddev.cu:306-330), element i at

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Grid3D storage with OpenCurrent's padded ghost-cell layout
 // (ocustorage/grid3d.h:29-264, grid3ddev.cu:358-400): pnz rounded up to a
 // multiple of 16, a (16 - gz)-element pre-padding, cell (i,j,k) at

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Ghost-cell fills, ocustorage/grid3dboundary.h.  Enqueued on the compute
 // stream; each returns false on invalid input (same checks as the reference).
 #ifndef __OCU_STORAGE_GRID3D_BOUNDARY_H__

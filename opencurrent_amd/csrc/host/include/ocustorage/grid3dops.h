@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocustorage/grid3dops.h: check_valid_mac_dimensions (grid3dops.cpp:23-45)
 #ifndef __OCU_STORAGE_GRID3D_OPS_H__
 #define __OCU_STORAGE_GRID3D_OPS_H__

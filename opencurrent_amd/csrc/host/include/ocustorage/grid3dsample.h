@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Point sampling of grids, the drop-in for ocustorage/grid3dsample.h:
 // trilinear interpolation at float positions (physical units) with the
 // faces' periodic wrap or clamp / Neumann extrapolation.  Enqueued on the

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Grid files, the drop-in for ocustorage/gridnetcdf.h: NetCDFGrid3DWriter /
 // NetCDFGrid3DReader write and read Grid3DHost fields as 4-D variables
 // (Time, X, Y, Z) of a NetCDF file, with coordinate variables per

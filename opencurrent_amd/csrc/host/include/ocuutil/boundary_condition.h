@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // BoundaryCondition / BoundaryConditionSet with the field order and widths of
 // ocuutil/boundary_condition.h:26-112, so they are interchangeable with the
 // C ABI's ocu_bc / ocu_bc_set (static_asserts in grid3dboundary.cpp).

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuutil/float_routines.h: check_float, min3, max3 (max2 added)
 #ifndef __OCU_UTIL_FLOAT_ROUTINES_H__
 #define __OCU_UTIL_FLOAT_ROUTINES_H__

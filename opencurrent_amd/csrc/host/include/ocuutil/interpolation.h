@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuutil/interpolation.h:26-31 (the functors themselves live in the kernels)
 #ifndef __OCU_UTIL_INTERPOLATION_H__
 #define __OCU_UTIL_INTERPOLATION_H__

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // Kernel error checking.  The reference's KernelWrapper synchronises the
 // device after every launch (ocuutil/kernel_wrapper.cu:94-103); here a launch
 // is only checked for an immediate error status (the C ABI's return code)

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ThreadManager for one process per GPU (the reference's co-array images,
 // ocuutil/thread.h:33-103, are not part of this build): it owns the compute
 // stream that every solver enqueues on.  The stream is created on first use

@@ -1,3 +1,4 @@
+// Interface after OpenCurrent (Copyright 2008-2009 NVIDIA Corporation, Apache License 2.0; see NOTICE).
 // ocuutil/timestep.h:26-30
 #ifndef __OCU_UTIL_TIMESTEP_H__
 #define __OCU_UTIL_TIMESTEP_H__
