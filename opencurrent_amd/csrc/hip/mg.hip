@@ -309,43 +309,42 @@ __device__ __forceinline__ void write_ghost_row(const DGrid<T> &o, const GhostL1
   if (k0 + V == o.nz) row[o.nz] = gw.xf(4, w.v[V - 1]);
 }
 
-// diagonal of row (i,j) before the z terms: diag_dev's x and y mods in order
-template <typename T, typename C>
-__device__ __forceinline__ T diag_xy(const C &c, int i, int j, int nx, int ny) {
-  T d = c.ld;
-  if (i == nx - 1) d += c.m[0];
-  if (i == 0) d += c.m[1];
-  if (j == ny - 1) d += c.m[2];
-  if (j == 0) d += c.m[3];
-  return d;
-}
-
 // One colour on a row held as V cells per lane (k = k0..k0+V-1): the cells
-// with k%2 == kpar and k < kupd get relax_pt (diag_dev's z mods added to dxy
-// in order), the others keep their value in out.  k0 is even for V >= 2, so
-// the colour of a cell depends on e alone and the branches are uniform.
-template <typename T, int V, typename C>
-__device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, int kupd, int nz, const Vec<T, V> &cen,
-                                          T lo, T hi, const Vec<T, V> &ym, const Vec<T, V> &yp,
-                                          const Vec<T, V> &xm, const Vec<T, V> &xp, const Vec<T, V> &b,
-                                          Vec<T, V> &out) {
+// with k%2 == kpar and k < kupd get the relax_DIAG update of cell (i, j, k)
+// (diag_dev's diagonal), the others keep their value (lanes past the row
+// too: one of them carries the k = nz ghost for its neighbour).  Even V: k0
+// is even, so the relaxed elements are those with e%2 == kpar (P below, a
+// uniform choice between two straight-line bodies).  V = 1: the cell is
+// relaxed on every lane and kept where its colour is kpar.  The row is
+// returned by value: stores through a reference from the two bodies would be
+// merged into one with a computed element index, which sends the caller's
+// row to scratch memory.
+template <int P, typename T, int V, typename C>
+__device__ __forceinline__ Vec<T, V> relax_row_elems(const C &c, int i, int j, int nx, int ny, int kpar, int k0,
+                                                     int kupd, int nz, const Vec<T, V> &cen, T lo, T hi,
+                                                     const Vec<T, V> &ym, const Vec<T, V> &yp, const Vec<T, V> &xm,
+                                                     const Vec<T, V> &xp, const Vec<T, V> &b, Vec<T, V> out) {
+  constexpr int S = V == 1 ? 1 : 2;  // element stride
+  constexpr int NE = V == 1 ? 1 : V / 2;
 #pragma unroll
-  for (int e = 0; e < V; e++) {
-    const int k = k0 + e;
-    if (V % 2 == 0) {
-      if ((e & 1) != kpar) continue;
-    } else if ((k & 1) != kpar) {
-      continue;
-    }
-    T d = dxy;
-    if (k == nz - 1) d += c.m[4];
-    if (k == 0) d += c.m[5];
-    const T nv = relax_pt(c, cen.v[e], e > 0 ? cen.v[e - 1] : lo, e < V - 1 ? cen.v[e + 1] : hi, ym.v[e],
-                          yp.v[e], xm.v[e], xp.v[e], b.v[e], d);
-    // lanes past the row (and k >= 2*(nz/2)) keep their value: one of them
-    // carries the k = nz ghost for its neighbour
-    out.v[e] = k < kupd ? nv : out.v[e];
+  for (int m = 0; m < NE; m++) {
+    const int e = P + S * m, k = k0 + e;
+    const T nv = relax_pt(c, cen.v[e], e > 0 ? cen.v[e - 1] : lo, e < V - 1 ? cen.v[e + 1] : hi, ym.v[e], yp.v[e],
+                          xm.v[e], xp.v[e], b.v[e], diag_dev<T>(c, i, j, k, nx, ny, nz));
+    const bool keep = V == 1 ? ((k & 1) == kpar && k < kupd) : k < kupd;
+    out.v[e] = keep ? nv : out.v[e];
   }
+  return out;
+}
+template <typename T, int V, typename C>
+__device__ __forceinline__ Vec<T, V> relax_row(const C &c, int i, int j, int nx, int ny, int kpar, int k0, int kupd,
+                                               int nz, const Vec<T, V> &cen, T lo, T hi, const Vec<T, V> &ym,
+                                               const Vec<T, V> &yp, const Vec<T, V> &xm, const Vec<T, V> &xp,
+                                               const Vec<T, V> &b, const Vec<T, V> &out) {
+  static_assert(V == 1 || V % 2 == 0, "rows hold 1 or an even number of cells per lane");
+  if (V == 1 || kpar == 0)
+    return relax_row_elems<0, T, V>(c, i, j, nx, ny, kpar, k0, kupd, nz, cen, lo, hi, ym, yp, xm, xp, b, out);
+  return relax_row_elems<1, T, V>(c, i, j, nx, ny, kpar, k0, kupd, nz, cen, lo, hi, ym, yp, xm, xp, b, out);
 }
 
 // Rows of a block: NW waves of R rows each; the outer two rows on each side
@@ -365,6 +364,12 @@ __device__ __forceinline__ void relax_row(const C &c, T dxy, int kpar, int k0, i
 template <typename T>
 constexpr int fused_rows_min_waves() { return sizeof(T) == 4 ? 6 : 1; }
 
+// block barriers per plane of the whole-row sweep (1 or 2; the macro exists
+// for tuning builds)
+#ifndef OCU_SWEEP_BARRIERS
+#define OCU_SWEEP_BARRIERS 1
+#endif
+
 template <typename T, int V, int R, int NW, bool ZERO, bool PRO, bool PER>
 __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
                                                                typename DevCFor<T>::type c, int first_color,
@@ -373,8 +378,13 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
                                                                int perx, int pery) {
   typedef Vec<T, V> VT;
   constexpr int NR = R * NW;
-  __shared__ VT sU0[NR][64];
+  // U0 and U1 rows of planes p and p-1 (slot = plane & 1): with both
+  // double-buffered one barrier per plane orders every LDS access - a wave
+  // past the barrier of plane p+1 writes only slots that no wave still in
+  // plane p reads (U0 of p+1, the coarse ring's plane ahead)
+  __shared__ VT sU0[OCU_SWEEP_BARRIERS == 1 ? 2 : 1][NR][64];
   __shared__ VT sU1[2][NR][64];
+  auto u0slot = [](int p) { return OCU_SWEEP_BARRIERS == 1 ? (p & 1) : 0; };
 
   int per = gridDim.x / FXCD;
   int logical = (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD);
@@ -630,7 +640,7 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       }
       if constexpr (AHEAD2) bnext[r] = loadb(r, min(p + 1, i1));
       else bcur[r] = loadb(r, p);
-      sU0[t0 + r][lane] = cc[r];
+      sU0[u0slot(p)][t0 + r][lane] = cc[r];
     }
     __syncthreads();
     if constexpr (PRO) {
@@ -646,14 +656,16 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
       u1[r] = cc[r];
       if (ring1(r) && (inner_j(r) || py) && (p_in || px)) {
         const int j = jw[r], pw = px ? wrap1(p, nx) : p;
-        const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[t0 - 1][lane];
-        const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[t0 + R][lane];
-        relax_row<T, V>(c, diag_xy<T>(c, pw, j, nx, ny), (first_color ^ pw ^ j) & 1, k0, kupd, nz, cc[r], lo, hi,
-                        ym, yp, u1m1[r], n[r], bcur[r], u1[r]);
+        const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[u0slot(p)][t0 - 1][lane];
+        const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[u0slot(p)][t0 + R][lane];
+        u1[r] = relax_row<T, V>(c, pw, j, nx, ny, (first_color ^ pw ^ j) & 1, k0, kupd, nz, cc[r], lo, hi, ym, yp,
+                                u1m1[r], n[r], bcur[r], u1[r]);
       }
       sU1[p & 1][t0 + r][lane] = u1[r];
     }
-    __syncthreads();
+    // the second colour reads U1 of plane p-1, written before this plane's
+    // barrier; a second barrier is needed only when U0 has a single slot
+    if (OCU_SWEEP_BARRIERS != 1) __syncthreads();
 
     // second colour on plane q = p-1 (output rows); U1 == U0 on ghosts
     const int q = p - 1;
@@ -665,9 +677,8 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
         const int j = jb + r;
         const VT ym = r > 0 ? u1m1[r > 0 ? r - 1 : 0] : sU1[q & 1][t0 - 1][lane];
         const VT yp = r < R - 1 ? u1m1[r < R - 1 ? r + 1 : 0] : sU1[q & 1][t0 + R][lane];
-        VT v = u1m1[r];
-        relax_row<T, V>(c, diag_xy<T>(c, q, j, nx, ny), (first_color ^ q ^ j ^ 1) & 1, k0, kupd, nz, u1m1[r], lo, hi,
-                        ym, yp, u1m2[r], u1[r], bprev[r], v);
+        const VT v = relax_row<T, V>(c, q, j, nx, ny, (first_color ^ q ^ j ^ 1) & 1, k0, kupd, nz, u1m1[r], lo, hi,
+                                     ym, yp, u1m2[r], u1[r], bprev[r], u1m1[r]);
         if (active) {
           *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
           if (wghost) {  // ghosts of this row: z ends, x and y face rows
