@@ -361,8 +361,14 @@ __device__ __forceinline__ Vec<T, V> relax_row(const C &c, int i, int j, int nx,
 // FP32 sweeps keep to 80 VGPRs (six waves per SIMD: two 12-wave blocks per
 // CU, each hiding the other's barriers); the prolongation variant would
 // otherwise take 81
-template <typename T>
-constexpr int fused_rows_min_waves() { return sizeof(T) == 4 ? 6 : 1; }
+#ifndef OCU_ROWS_F32_WAVES
+#define OCU_ROWS_F32_WAVES 6
+#endif
+#ifndef OCU_ROWS_PRO_WAVES
+#define OCU_ROWS_PRO_WAVES 6
+#endif
+template <typename T, bool PRO>
+constexpr int fused_rows_min_waves() { return sizeof(T) == 4 ? (PRO ? OCU_ROWS_PRO_WAVES : OCU_ROWS_F32_WAVES) : 1; }
 
 // block barriers per plane of the whole-row sweep (1 or 2; the macro exists
 // for tuning builds)
@@ -371,11 +377,9 @@ constexpr int fused_rows_min_waves() { return sizeof(T) == 4 ? 6 : 1; }
 #endif
 
 template <typename T, int V, int R, int NW, bool ZERO, bool PRO, bool PER>
-__global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fused_rows(DGrid<T> uin, DGrid<T> uout, DGrid<T> b,
-                                                               typename DevCFor<T>::type c, int first_color,
-                                                               int tiles_j, int ichunk, int ntiles,
-                                                               GhostL1<T> gw, int wghost, DGrid<T> uc, int quirk,
-                                                               int perx, int pery) {
+__global__ __launch_bounds__(64 * NW, (fused_rows_min_waves<T, PRO>())) void k_relax_fused_rows(
+    DGrid<T> uin, DGrid<T> uout, DGrid<T> b, typename DevCFor<T>::type c, int first_color, int tiles_j, int ichunk,
+    int ntiles, GhostL1<T> gw, int wghost, DGrid<T> uc, int quirk, int perx, int pery) {
   typedef Vec<T, V> VT;
   constexpr int NR = R * NW;
   // U0 and U1 rows of planes p and p-1 (slot = plane & 1): with both
@@ -731,9 +735,11 @@ __global__ __launch_bounds__(64 * NW, fused_rows_min_waves<T>()) void k_relax_fu
 #define OCU_ROWS_F64 1, 12
 #endif
 template <int R_, int NW_> struct Shape { static constexpr int R = R_, NW = NW_; };
-template <typename T, int V> struct RowsShape : Shape<OCU_ROWS_F32> {};
-template <int V> struct RowsShape<double, V> : Shape<OCU_ROWS_F32> {};
-template <> struct RowsShape<double, 4> : Shape<OCU_ROWS_F64> {};
+template <typename T, int V, bool PRO> struct RowsShape : Shape<OCU_ROWS_F32> {};
+template <> struct RowsShape<double, 4, false> : Shape<OCU_ROWS_F64> {};
+// the sweep that adds the prolongation stages a coarse row per wave
+template <> struct RowsShape<float, 4, true> : Shape<1, 12> {};
+template <> struct RowsShape<double, 4, true> : Shape<1, 12> {};
 
 
 // ------------------------------------------------------------------------
@@ -1752,7 +1758,7 @@ int launch_fused_rows(const ocu_grid3d *u_in, const ocu_grid3d *u_out, const ocu
                       const typename DevCFor<T>::type &c, int first_color, const GhostL1<T> &gw, int wghost,
                       hipStream_t stream, const ocu_grid3d *uc = nullptr, int quirk = 0, int perx = 0,
                       int pery = 0) {
-  typedef RowsShape<T, V> RS;
+  typedef RowsShape<T, V, PRO> RS;
   constexpr int NR = RS::R * RS::NW;
   // the periodic variant only where a face pair is periodic: the plain
   // sweep carries none of its wrap logic

@@ -84,18 +84,10 @@ __device__ __forceinline__ bool tf_writes(const TFace &f) {
   return f.type == OCU_BC_PERIODIC || f.type == OCU_BC_DIRICHELET || f.type == OCU_BC_NEUMANN;
 }
 
+// One element (ip, jp, kp) of the padded box (indices from the box corner,
+// ghosts included): the new value of each grid there.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ns_step(NSStep<T> s) {
-  const int kp = blockIdx.x * 64 + threadIdx.x, jp = blockIdx.y * 4 + threadIdx.y, ip = blockIdx.z;
-  // the pre-padding before the padded box: untouched elements
-  if (ip == 0 && jp == 0 && kp < s.pre) {
-    const int64_t e = kp;
-    s.un[e] = untouched(s, 0, s.u[e]);
-    s.vn[e] = untouched(s, 1, s.v[e]);
-    s.wn[e] = untouched(s, 2, s.w[e]);
-    s.tn[e] = untouched(s, 3, s.t[e]);
-  }
-  if (kp >= s.pnz || jp >= s.pny) return;
+__device__ __forceinline__ void ns_element(const NSStep<T> &s, int ip, int jp, int kp) {
   const int i = ip - s.gx, j = jp - s.gy, k = kp - s.gz;
   const int64_t e = (int64_t)s.pre + ((int64_t)ip * s.pny + jp) * s.pnz + kp;  // element of the allocation
   const int64_t c = e - s.shift;                                               // offset from cell (0,0,0)
@@ -156,25 +148,240 @@ __global__ __launch_bounds__(256) void k_ns_step(NSStep<T> s) {
   // no-corner fill writes: the fill of the new value of its source cell
   const bool gx_ = (i == -1 || i == s.nx) && cj && ck, gy_ = ci && (j == -1 || j == s.ny) && ck,
              gz_ = ci && cj && (k == -1 || k == s.nz);
-  int face = -1, si = i, sj = j, sk = k, ax = 0;
-  bool neg = false;
+  // (the face and spacing picked by selects: an indexed read of the
+  // parameter arrays would copy them to scratch memory)
+  int si = i, sj = j, sk = k;
+  bool neg = false, has = false;
+  TFace f{OCU_BC_NONE, 0.f};
+  T h = 0;
   if (gx_) {
-    neg = i < 0, face = neg ? 1 : 0, ax = 0;
-    si = s.tf[face].type == OCU_BC_PERIODIC ? (neg ? s.nx - 1 : 0) : (neg ? 0 : s.nx - 1);
+    neg = i < 0, f = neg ? s.tf[1] : s.tf[0], h = s.th[0], has = true;
+    si = f.type == OCU_BC_PERIODIC ? (neg ? s.nx - 1 : 0) : (neg ? 0 : s.nx - 1);
   } else if (gy_) {
-    neg = j < 0, face = neg ? 3 : 2, ax = 1;
-    sj = s.tf[face].type == OCU_BC_PERIODIC ? (neg ? s.ny - 1 : 0) : (neg ? 0 : s.ny - 1);
+    neg = j < 0, f = neg ? s.tf[3] : s.tf[2], h = s.th[1], has = true;
+    sj = f.type == OCU_BC_PERIODIC ? (neg ? s.ny - 1 : 0) : (neg ? 0 : s.ny - 1);
   } else if (gz_) {
-    neg = k < 0, face = neg ? 5 : 4, ax = 2;
-    sk = s.tf[face].type == OCU_BC_PERIODIC ? (neg ? s.nz - 1 : 0) : (neg ? 0 : s.nz - 1);
+    neg = k < 0, f = neg ? s.tf[5] : s.tf[4], h = s.th[2], has = true;
+    sk = f.type == OCU_BC_PERIODIC ? (neg ? s.nz - 1 : 0) : (neg ? 0 : s.nz - 1);
   }
-  if (face >= 0 && tf_writes(s.tf[face])) {
+  if (has && tf_writes(f)) {
     const int64_t cs = (int64_t)si * s.xs + (int64_t)sj * s.ys + sk;
     const T dt = temp_tendency(s, cs, si, sj, sk);
     const T tnew = combine(s, Tt[cs], dt, s.use_last ? s.lt[cs + s.shift] : (T)0);
-    s.tn[e] = nocorner_xform(s.tf[face], neg, s.th[ax], tnew);
+    s.tn[e] = nocorner_xform(f, neg, h, tnew);
   } else {
     s.tn[e] = untouched(s, 3, Tt[c]);
+  }
+}
+
+// the pre-padding before the padded box: untouched elements
+template <typename T>
+__device__ __forceinline__ void ns_prepad(const NSStep<T> &s, int64_t e) {
+  s.un[e] = untouched(s, 0, s.u[e]);
+  s.vn[e] = untouched(s, 1, s.v[e]);
+  s.wn[e] = untouched(s, 2, s.w[e]);
+  s.tn[e] = untouched(s, 3, s.t[e]);
+}
+
+// The interior core is the box [bx, nx-bx) x [by, ny-by) x [bz, nz-bz) of
+// T's interior (a band of b = 2 cells is left out along a periodic axis when
+// second-order upwind samples would wrap around it); k_ns_march takes the
+// core.  Every other element - the bands, the u, v and w faces at i = nx,
+// j = ny and k = nz, the ghost layers, the padding - goes here, one thread
+// each, after the pre-padding: the planes outside the core's x range (count
+// na), then the rows outside its y range of the core's planes (nb), then the
+// row ends outside its z range of the core's rows (nc).
+struct Core {
+  int bx, by, bz;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ns_shell(NSStep<T> s, Core b, int64_t na, int64_t nb, int64_t nc) {
+  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < s.pre) {
+    ns_prepad(s, q);
+    return;
+  }
+  q -= s.pre;
+  const int cx = s.nx - 2 * b.bx, cy = s.ny - 2 * b.by, cz = s.nz - 2 * b.bz;  // core extents
+  const int lx = s.gx + b.bx, ly = s.gy + b.by, lz = s.gz + b.bz;             // its low corner (box indices)
+  int ip, jp, kp;
+  if (q < na) {
+    const int64_t plane = (int64_t)s.pny * s.pnz, pl = q / plane, r = q % plane;
+    ip = pl < lx ? (int)pl : (int)pl + cx;
+    jp = (int)(r / s.pnz);
+    kp = (int)(r % s.pnz);
+  } else if ((q -= na) < nb) {
+    const int64_t per = (int64_t)(s.pny - cy) * s.pnz, pl = q / per, r = q % per;
+    const int rr = (int)(r / s.pnz);
+    ip = (int)pl + lx;
+    jp = rr < ly ? rr : rr + cy;
+    kp = (int)(r % s.pnz);
+  } else if ((q -= nb) < nc) {
+    const int per = s.pnz - cz, t = (int)(q % per);
+    const int64_t rowi = q / per;
+    ip = (int)(rowi / cy) + lx;
+    jp = (int)(rowi % cy) + ly;
+    kp = t < lz ? t : t + cz;
+  } else {
+    return;
+  }
+  ns_element(s, ip, jp, kp);
+}
+
+// ------------------------------------------------------------------------
+// the interior core, marching along x through an LDS window
+// ------------------------------------------------------------------------
+//
+// A block owns a tile of MTJ rows x MTK cells (j, k) of T's interior (its
+// cells in the core) and an x-chunk of planes.  It keeps a window of planes p-2..p+2 of u, v, w and T
+// in LDS, each plane with a two-cell halo in j and k (every sample the
+// cell functions read: second-order upwind reaches two cells, the
+// self-advection velocities one diagonal), in six slots, so the plane
+// loading for p+1 goes to the slot no thread reads in plane p and one
+// barrier per plane orders the accesses.  The cell functions read the
+// window through WAt, whose offsets use a virtual x stride of 2^20 elements
+// (no sample of a core cell wraps around a periodic axis).  Each cell's
+// tendencies and new values are those of ns_element, in the same order.
+constexpr int MTK = 64, MTJ = 8, MWK = MTK + 4, MWJ = MTJ + 4, MPLANE = MWJ * MWK, MSLOTS = 6;
+constexpr int MXSH = 20;
+constexpr int64_t MXS = (int64_t)1 << MXSH;
+constexpr int MLOADS = (MPLANE + MTK * MTJ - 1) / (MTK * MTJ);  // window elements per thread and plane
+
+template <typename T>
+struct WAt {
+  const T *base;               // this field's slot 0
+  int cell;                    // the cell within a window plane
+  int sm2, sm1, s0, sp1, sp2;  // element offsets of the slots of planes p-2..p+2
+  __device__ __forceinline__ T operator()(int64_t o) const {
+    const int di = (int)((o + (MXS >> 1)) >> MXSH);
+    const int r = (int)(o - ((int64_t)di << MXSH));
+    // a chain of selects, not an array: constant offsets fold to one slot,
+    // and a computed one keeps the slots in registers
+    const int sl = di == -2 ? sm2 : di == -1 ? sm1 : di == 0 ? s0 : di == 1 ? sp1 : sp2;
+    return base[sl + cell + r];
+  }
+};
+
+template <typename T>
+struct Tend {
+  T du, dv, dw, dt;
+};
+
+// the tendencies of interior cell (i, j, k) through accessors A (window or
+// global): the sequence of ns_element for a cell of T's interior (returned
+// by value, so that the two callers' results stay in registers)
+template <bool WRAP, typename T, typename A>
+__device__ __forceinline__ Tend<T> ns_tendencies(const NSStep<T> &s, const A &U, const A &V, const A &W, const A &Tt,
+                                                 int64_t xs, int64_t ys, int64_t vnbr, int i, int j, int k) {
+  T du, dv, dw, dt;
+  if (s.scheme == OCU_IT_SECOND_ORDER_UPWIND)
+    self_advect_up2_at<WRAP, T>(U, V, W, xs, ys, s.ih, i, j, k, s.nx, s.ny, s.nz, s.fper, du, dv, dw);
+  else
+    self_advect_at<T>(U, V, W, xs, ys, s.ih, s.scheme, du, dv, dw);
+  if (s.visc) {
+    du = du + s.nu * laplacian_at<T>(U, xs, ys, s.ih2);
+    dv = dv + s.nu * laplacian_at<T>(V, xs, ys, s.ih2);
+    dw = dw + s.nu * laplacian_at<T>(W, xs, ys, s.ih2);
+  }
+  const T f = thermal_at<T>(Tt, vnbr, s.tcoef);
+  if (s.vaxis == 0) du = du + f;
+  else if (s.vaxis == 1) dv = dv + f;
+  else dw = dw + f;
+  dt = s.scheme == OCU_IT_SECOND_ORDER_UPWIND
+           ? passive_advect_up2_at<WRAP, T>(U, V, W, Tt, xs, ys, s.ih, i, j, k, s.nx, s.ny, s.nz, s.tper)
+           : passive_advect_at<T>(U, V, W, Tt, xs, ys, s.ih, s.scheme);
+  if (s.tdiff) dt = dt + s.kappa * laplacian_at<T>(Tt, xs, ys, s.ih2);
+  return Tend<T>{du, dv, dw, dt};
+}
+
+template <typename T>
+__global__ __launch_bounds__(MTK *MTJ) void k_ns_march(NSStep<T> s, Core b, int tiles_k, int tiles_j, int ichunk,
+                                                       int ntiles) {
+  extern __shared__ unsigned char ns_smem[];
+  T *win = reinterpret_cast<T *>(ns_smem);  // [4 fields][MSLOTS][MPLANE]
+  // blocks are dealt round-robin to the 8 XCDs: a contiguous run of tiles
+  // per XCD
+  const int per = gridDim.x / 8;
+  const int logical = (blockIdx.x % 8) * per + (blockIdx.x / 8);
+  if (logical >= ntiles) return;  // uniform per block
+  const int tk = logical % tiles_k, rem = logical / tiles_k, tj = rem % tiles_j, ti = rem / tiles_j;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MTK + tx;
+  const int k0 = tk * MTK, j0 = tj * MTJ, i0 = ti * ichunk, i1 = min(s.nx, i0 + ichunk);
+  const int k = k0 + tx, j = j0 + ty;
+  const bool mine = k >= b.bz && k < s.nz - b.bz && j >= b.by && j < s.ny - b.by;
+  const T *src[4] = {s.u, s.v, s.w, s.t};
+
+  // element offsets (from element 0) of this thread's window positions in
+  // plane 0 (-1: outside the padded box in j or k)
+  int64_t wpos[MLOADS];
+#pragma unroll
+  for (int m = 0; m < MLOADS; m++) {
+    const int t = tid + m * MTK * MTJ;
+    const int wj = t / MWK, wk = t % MWK;
+    const int jp = j0 - 2 + wj + s.gy, kp = k0 - 2 + wk + s.gz;
+    wpos[m] = (t < MPLANE && jp >= 0 && jp < s.pny && kp >= 0 && kp < s.pnz)
+                  ? (int64_t)s.pre + (int64_t)jp * s.pnz + kp
+                  : (int64_t)-1;
+  }
+  auto slot = [](int q) { return ((q % MSLOTS) + MSLOTS) % MSLOTS; };
+  auto fetch = [&](int q, T (&v)[4][MLOADS]) {
+    const int ip = q + s.gx;
+    const bool in = ip >= 0 && ip < s.pnx;
+#pragma unroll
+    for (int m = 0; m < MLOADS; m++) {
+      const bool ok = in && wpos[m] >= 0;
+      const int64_t e = wpos[m] + (int64_t)ip * s.pny * s.pnz;
+#pragma unroll
+      for (int f = 0; f < 4; f++) v[f][m] = ok ? src[f][e] : (T)0;
+    }
+  };
+  auto store = [&](int q, const T (&v)[4][MLOADS]) {
+    T *dst = win + slot(q) * MPLANE;
+#pragma unroll
+    for (int m = 0; m < MLOADS; m++) {
+      const int t = tid + m * MTK * MTJ;
+      if (t < MPLANE) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) dst[f * MSLOTS * MPLANE + t] = v[f][m];
+      }
+    }
+  };
+  T v[4][MLOADS];
+  for (int q = i0 - 2; q <= i0 + 2; q++) {
+    fetch(q, v);
+    store(q, v);
+  }
+  __syncthreads();
+
+  const int cell = (ty + 2) * MWK + tx + 2;
+  const int64_t xs = s.xs, ys = s.ys;
+  const int64_t vnbr_w = s.vaxis == 0 ? MXS : s.vaxis == 1 ? (int64_t)MWK : (int64_t)1;
+  for (int p = i0; p < i1; p++) {
+    fetch(p + 3, v);  // in flight while plane p is computed
+    if (mine && p >= b.bx && p < s.nx - b.bx) {
+      const int64_t c = (int64_t)p * xs + (int64_t)j * ys + k;  // offset from cell (0,0,0)
+      const int sm2 = slot(p - 2) * MPLANE, sm1 = slot(p - 1) * MPLANE, s0 = slot(p) * MPLANE,
+                sp1 = slot(p + 1) * MPLANE, sp2 = slot(p + 2) * MPLANE;
+      auto acc = [&](int f) { return WAt<T>{win + f * MSLOTS * MPLANE, cell, sm2, sm1, s0, sp1, sp2}; };
+      const Tend<T> d = ns_tendencies<false, T>(s, acc(0), acc(1), acc(2), acc(3), MXS, (int64_t)MWK, vnbr_w, p, j, k);
+      const int64_t e = c + s.shift;
+      const T *olds = win + slot(p) * MPLANE + cell;
+      const T ou = olds[0], ov = olds[MSLOTS * MPLANE], ow = olds[2 * MSLOTS * MPLANE],
+              ot = olds[3 * MSLOTS * MPLANE];
+      s.un[e] = combine(s, ou, d.du, s.use_last ? s.lu[e] : (T)0);
+      s.vn[e] = combine(s, ov, d.dv, s.use_last ? s.lv[e] : (T)0);
+      s.wn[e] = combine(s, ow, d.dw, s.use_last ? s.lw[e] : (T)0);
+      s.tn[e] = combine(s, ot, d.dt, s.use_last ? s.lt[e] : (T)0);
+      if (s.keep) {
+        s.du[e] = d.du;
+        s.dv[e] = d.dv;
+        s.dw[e] = d.dw;
+        s.dtt[e] = d.dt;
+      }
+    }
+    store(p + 3, v);
+    __syncthreads();
   }
 }
 
@@ -225,8 +432,49 @@ int launch_ns_step(const ocu_grid3d *const *in, const ocu_grid3d *const *out, co
   s.flat = 0;
   for (int q = 0; q < 4; q++) s.flat |= whole_allocation(in[q]) << q;
   s.visc = p->viscosity > 0, s.tdiff = p->thermal_diffusion > 0;
-  const dim3 grid((g->pnz + 63) / 64, (g->pny + 3) / 4, g->pnx);
-  k_ns_step<T><<<grid, dim3(64, 4, 1), 0, stream>>>(s);
+  // the core: bands of two cells along the periodic axes whose second-order
+  // upwind samples wrap (flow or temperature)
+  Core b{0, 0, 0};
+  if (s.scheme == OCU_IT_SECOND_ORDER_UPWIND) {
+    const int per = s.fper | s.tper;
+    b.bx = (per & 1) ? 2 : 0, b.by = (per & 2) ? 2 : 0, b.bz = (per & 4) ? 2 : 0;
+  }
+  const bool core = s.nx > 2 * b.bx && s.ny > 2 * b.by && s.nz > 2 * b.bz;
+  if (!core) b = Core{(s.nx + 1) / 2, (s.ny + 1) / 2, (s.nz + 1) / 2};  // all to the shell
+  const int cx = max(s.nx - 2 * b.bx, 0), cy = max(s.ny - 2 * b.by, 0), cz = max(s.nz - 2 * b.bz, 0);
+  // the shell: everything outside the core
+  const int64_t na = (int64_t)(s.pnx - cx) * s.pny * s.pnz, nb = (int64_t)cx * (s.pny - cy) * s.pnz,
+                nc = (int64_t)cx * cy * (s.pnz - cz);
+  const int64_t nshell = s.pre + na + nb + nc;
+  k_ns_shell<T><<<(unsigned)((nshell + 255) / 256), 256, 0, stream>>>(s, b, na, nb, nc);
+  OCU_HIP_TRY(hipGetLastError());
+  if (!core) return OCU_OK;
+  // the interior
+  const int tiles_k = (s.nz + MTK - 1) / MTK, tiles_j = (s.ny + MTJ - 1) / MTJ;
+  const size_t lds = (size_t)4 * MSLOTS * MPLANE * sizeof(T);
+  OCU_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ns_march<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int dev = 0, cus = 256, per = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_march<T>, MTK * MTJ, lds) != hipSuccess || per < 1)
+    per = 1;
+  // planes per block: the fewest rounds of resident blocks, each block
+  // loading four planes beyond its own
+  const int64_t slots = (int64_t)cus * per;
+  int ichunk = s.nx;
+  int64_t best = INT64_MAX;
+  for (int ic = 1;; ic = ic < 8 ? ic + 1 : ic * 2) {
+    const int c = min(ic, s.nx);  // the last candidate is the whole extent
+    const int64_t nb_ = (int64_t)tiles_k * tiles_j * ((s.nx + c - 1) / c);
+    const int64_t cost = ((nb_ + slots - 1) / slots) * (c + 4);
+    if (cost < best) best = cost, ichunk = c;
+    if (c == s.nx) break;
+  }
+  const int ntiles = tiles_k * tiles_j * ((s.nx + ichunk - 1) / ichunk);
+  const int nblocks = ((ntiles + 7) / 8) * 8;
+  k_ns_march<T><<<nblocks, dim3(MTK, MTJ), lds, stream>>>(s, b, tiles_k, tiles_j, ichunk, ntiles);
   return OCU_OK;
 }
 
@@ -286,8 +534,9 @@ int ocu_ns_step_fused(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_grid3d
                     b->type == OCU_BC_NEUMANN,
                 "%s - invalid temperature boundary condition type %d", what, b->type);
   if ((int64_t)nx * ny * nz == 0) return OCU_OK;
-  if (u->dtype == OCU_F64) launch_ns_step<double>(in, out, d, last, p, temp_bc, S(stream));
-  else launch_ns_step<float>(in, out, d, last, p, temp_bc, S(stream));
+  const int e = u->dtype == OCU_F64 ? launch_ns_step<double>(in, out, d, last, p, temp_bc, S(stream))
+                                    : launch_ns_step<float>(in, out, d, last, p, temp_bc, S(stream));
+  if (e != OCU_OK) return e;
   OCU_HIP_LAUNCHED(what);
 }
 
