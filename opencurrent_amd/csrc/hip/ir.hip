@@ -78,9 +78,12 @@ __global__ __launch_bounds__(64 * IRR_NW) void k_ir_residual_rows(DGrid<double> 
   const int gk = lane == 0 ? -1 : nz;
   const bool g_in_data = lane == hi_lane && nz < 64 * V;
   double s2 = 0, mx = 0, sm = 0, sx = 0;
-  const int per = gridDim.x / 8;
-  const int first = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  for (int t = first; t < ntiles; t += gridDim.x) {
+  // tiles dealt over the XCDs in contiguous runs: the first nb blocks (the
+  // tiles rounded up to the XCDs, at most the grid) take them
+  const int nb = min((int)gridDim.x, ((ntiles + 7) / 8) * 8);
+  const int per = nb / 8;
+  const int first = blockIdx.x < nb ? (blockIdx.x % 8) * per + blockIdx.x / 8 : ntiles;
+  for (int t = first; t < ntiles; t += nb) {
     const int tj = t % tiles_j, ti = t / tiles_j;
     const int j = tj * IRR_NW + w;
     if (j >= x.ny) continue;  // per wave; the only barrier is after the loop

@@ -863,12 +863,16 @@ __global__ __launch_bounds__(64 * NW) void k_res_restrict_rows(DGrid<T> u, DGrid
                                                                 int tiles_j, int ichunk, int ntiles, double *parts) {
   typedef Vec<T, V> VT;
   typedef Vec<T, V / 2> VC;
-  const int per = gridDim.x / FXCD;
-  const int logical = (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD);
+  // the first nb blocks (ntiles rounded up to the XCDs) take the tiles, each
+  // XCD a contiguous run of them; with norms every block of the NORM_PARTS
+  // grid writes its partial pair (zeros past the tiles)
+  const int nb = ((ntiles + FXCD - 1) / FXCD) * FXCD;
+  const int per = nb / FXCD;
+  const int logical = blockIdx.x < nb ? (blockIdx.x % FXCD) * per + (blockIdx.x / FXCD) : ntiles;
   if (logical >= ntiles) {  // uniform per block
     if (NORM && threadIdx.x == 0 && threadIdx.y == 0) {
-      parts[2 * logical] = 0.0;
-      parts[2 * logical + 1] = 0.0;
+      parts[2 * blockIdx.x] = 0.0;
+      parts[2 * blockIdx.x + 1] = 0.0;
     }
     return;
   }
@@ -969,8 +973,8 @@ __global__ __launch_bounds__(64 * NW) void k_res_restrict_rows(DGrid<T> u, DGrid
         a += sh[0][q];
         m = fmax(m, sh[1][q]);
       }
-      parts[2 * logical] = a;
-      parts[2 * logical + 1] = m;
+      parts[2 * blockIdx.x] = a;
+      parts[2 * blockIdx.x + 1] = m;
     }
   }
 }
