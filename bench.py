@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # passes (FETCH_SIZE, WRITE_SIZE) and scaled by what the same counters read
 # for a plain copy of known size; recorded with the profile summary
 # profiles/level0_kernels_256_hbm_traffic.md
-SWEEP_TRAFFIC_256 = {"plain": 241_400_000, "zero": 152_200_000, "pro": 254_400_000}
+SWEEP_TRAFFIC_256 = {"plain": 241_500_000, "zero": 152_200_000, "pro": 254_500_000}
 
 
 def parse():
