@@ -1711,46 +1711,55 @@ int row_vec_width(const ocu_grid3d *u, const ocu_grid3d *o, const ocu_grid3d *b)
   return V;
 }
 
-// Concurrent blocks of a kernel on the current device (CUs x blocks per CU),
+// Resident blocks of a kernel on the current device: CUs and blocks per CU,
 // cached per device, kernel and block size.
+struct Slots {
+  int cus, per;
+};
 template <typename K>
-int device_slots(K kern, int threads) {
+Slots device_slots(K kern, int threads) {
   struct Entry {
     const void *k;
-    int dev, threads, slots;
+    int dev, threads;
+    Slots s;
   };
   static std::mutex mu;
   static Entry cache[128];
   static int used = 0;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return Slots{256, 1};
   const void *key = reinterpret_cast<const void *>(kern);
   std::lock_guard<std::mutex> lock(mu);
   for (int q = 0; q < used; q++)
-    if (cache[q].k == key && cache[q].dev == dev && cache[q].threads == threads) return cache[q].slots;
+    if (cache[q].k == key && cache[q].dev == dev && cache[q].threads == threads) return cache[q].s;
   int cus = 0, per = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per <= 0) per = 1;
-  const int slots = cus * per;
-  if (used < 128) cache[used++] = Entry{key, dev, threads, slots};
-  return slots;
+  const Slots s{cus, per};
+  if (used < 128) cache[used++] = Entry{key, dev, threads, s};
+  return s;
 }
 
 // i-planes per block of the whole-row kernel.  A block reads ichunk+3
-// planes to write ichunk (the two-colour halo), and blocks run in rounds of
-// `slots`: pick the chunk that minimises rounds x planes per block.  The
-// OCU_FUSED_ICHUNK environment variable overrides the choice (tuning).
-int fused_ichunk(int nx, int tiles_j, int slots) {
+// planes to write ichunk (the two-colour halo).  Two costs compete: the
+// march of one block (rounds of resident blocks x planes per block, latency
+// bound) and the planes every CU streams in total (nblocks x planes / CUs,
+// bandwidth bound); the chunk minimising 3 x march + 2 x stream is taken,
+// weights fitted to 32^3..256^3 sweeps.  The OCU_FUSED_ICHUNK environment
+// variable overrides the choice (tuning).
+int fused_ichunk(int nx, int tiles_j, Slots sl) {
   if (const char *s = getenv("OCU_FUSED_ICHUNK")) {
     int v = atoi(s);
     if (v > 0) return v;
   }
   static const int cand[] = {1, 2, 4, 6, 8, 12, 16, 24, 32, 48, 64};
+  const int64_t slots = (int64_t)sl.cus * sl.per;
   int best = 1;
   int64_t best_cost = INT64_MAX;
   for (int ic : cand) {
     const int64_t nb = (int64_t)tiles_j * ((nx + ic - 1) / ic);
-    const int64_t cost = ((nb + slots - 1) / slots) * (min(ic, nx) + 3);
+    const int64_t planes = min(ic, nx) + 3;
+    const int64_t cost = 3 * ((nb + slots - 1) / slots) * planes * sl.cus + 2 * nb * planes;
     if (cost < best_cost) best_cost = cost, best = ic;
     if (ic >= nx) break;
   }
