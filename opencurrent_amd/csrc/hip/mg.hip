@@ -110,13 +110,22 @@ __device__ __forceinline__ float relax_pt(const DevC32 &c, float uc, float zm, f
 // do_cpu_solve point update (sol_mgpressure3ddev.cu:106-113): neighbour
 // products in double, each partial sum rounded back to T
 template <typename T>
-__device__ __forceinline__ T relax_pt_host(const HostC<T> &c, T uc, T zm, T zp, T ym, T yp, T xm, T xp,
-                                           T b, T x_mod, T y_mod, T z_mod) {
+__device__ __forceinline__ T relax_pt_host_d(const HostC<T> &c, T uc, T zm, T zp, T ym, T yp, T xm, T xp, T b,
+                                             T diag) {
   T r = (-c.h * c.h * b - c.ld * uc);
   r += c.fz * (zm + zp);
   r += c.fy * (yp + ym);
   r += c.fx * (xp + xm);
-  return uc + c.omega * r / (c.ld + x_mod + y_mod + z_mod);
+  return uc + c.omega * r / diag;
+}
+template <typename T>
+__device__ __forceinline__ T diag_host(const HostC<T> &c, T x_mod, T y_mod, T z_mod) {
+  return c.ld + x_mod + y_mod + z_mod;
+}
+template <typename T>
+__device__ __forceinline__ T relax_pt_host(const HostC<T> &c, T uc, T zm, T zp, T ym, T yp, T xm, T xp,
+                                           T b, T x_mod, T y_mod, T z_mod) {
+  return relax_pt_host_d(c, uc, zm, zp, ym, yp, xm, xp, b, diag_host(c, x_mod, y_mod, z_mod));
 }
 
 // ------------------------------------------------------------------------
@@ -1461,41 +1470,125 @@ __device__ __forceinline__ void face_ghosts(T *u, const BotGeom &g, const GhostL
   }
 }
 
-// one colour with the device arithmetic (k < 2*(nz/2)), threads over the
-// colour's cells only
-template <typename T>
-__device__ void bot_relax_dev(T *u, const T *b, const BotGeom &g, const typename DevCFor<T>::type &c,
-                              const GhostL1<T> &gw, const BotPer &per, int color) {
-  const int h = (int)g.mhd.n, n = g.nx * g.ny * h;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    const int row = mdiv(t, g.mhd), i = mdiv(row, g.my), j = row - i * g.ny;
-    const int k = 2 * (t - row * h) + ((i + j + color) & 1);
-    const int id = g.lshift + i * g.lxs + j * g.lys + k;
-    const T d = diag_dev<T>(c, i, j, k, g.nx, g.ny, g.nz);
-    const T v = relax_pt(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs], u[id + g.lxs],
-                         b[id], d);
-    u[id] = v;
-    face_ghosts(u, g, gw, per, id, i, j, k, v);
-  }
-}
+// A cell of a colour's compacted numbering t (threads over the colour's
+// cells only): its LDS index, its face bits (x high, x low, y high, y low,
+// z high, z low; bit 6: no cell) and its diagonal.  HOST: the
+// relax_on_host numbering (every k, rows of ceil(nz/2) slots) and diagonal;
+// else the device one (k < 2*(nz/2)).
+template <typename D>
+struct BotCell {
+  int id, fl;
+  D d;
+};
+constexpr int BOT_NONE = 64;
 
-// one colour with the relax_on_host arithmetic (every k)
-template <typename T>
-__device__ void bot_relax_host(T *u, const T *b, const BotGeom &g, const HostC<T> &c, const GhostL1<T> &gw,
-                               const BotPer &per, int color) {
-  const int h = (int)g.mhh.n, n = g.nx * g.ny * h;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    const int row = mdiv(t, g.mhh), i = mdiv(row, g.my), j = row - i * g.ny;
-    const int k = 2 * (t - row * h) + ((i + j + color) & 1);
-    if (k >= g.nz) continue;
-    const int id = g.lshift + i * g.lxs + j * g.lys + k;
+template <bool HOST, typename D, typename C>
+__device__ __forceinline__ BotCell<D> bot_cell(const BotGeom &g, const C &c, int color, int t) {
+  Mag mh;
+  if constexpr (HOST) mh = g.mhh;
+  else mh = g.mhd;
+  const int h = (int)mh.n;
+  const int row = mdiv(t, mh), i = mdiv(row, g.my), j = row - i * g.ny;
+  const int k = 2 * (t - row * h) + ((i + j + color) & 1);
+  BotCell<D> s;
+  s.id = g.lshift + i * g.lxs + j * g.lys + k;
+  s.fl = (i == g.nx - 1) | (i == 0) << 1 | (j == g.ny - 1) << 2 | (j == 0) << 3 | (k == g.nz - 1) << 4 |
+         (k == 0) << 5 | (t >= g.nx * g.ny * h || k >= g.nz ? BOT_NONE : 0);
+  if constexpr (HOST) {
+    typedef decltype(c.omega) T;
     const T xm_ = (i == 0) ? c.m[1] : (i == g.nx - 1) ? c.m[0] : (T)0;
     const T ym_ = (j == 0) ? c.m[3] : (j == g.ny - 1) ? c.m[2] : (T)0;
     const T zm_ = (k == 0) ? c.m[5] : (k == g.nz - 1) ? c.m[4] : (T)0;
-    const T v = relax_pt_host(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs],
-                              u[id + g.lxs], b[id], xm_, ym_, zm_);
-    u[id] = v;
-    face_ghosts(u, g, gw, per, id, i, j, k, v);
+    s.d = diag_host(c, xm_, ym_, zm_);
+  } else {
+    s.d = diag_dev<D>(c, i, j, k, g.nx, g.ny, g.nz);
+  }
+  return s;
+}
+
+// face ghosts of a cell from its face bits (face_ghosts' writes)
+template <typename T>
+__device__ __forceinline__ void face_ghosts_fl(T *u, const BotGeom &g, const GhostL1<T> &gw, const BotPer &per,
+                                               int id, int fl, T v) {
+  if (fl & 2) {
+    if (per.a[0]) u[id + g.nx * g.lxs] = v;
+    else u[id - g.lxs] = gw.xf(1, v);
+  }
+  if (fl & 1) {
+    if (per.a[0]) u[id - g.nx * g.lxs] = v;
+    else u[id + g.lxs] = gw.xf(0, v);
+  }
+  if (fl & 8) {
+    if (per.a[1]) u[id + g.ny * g.lys] = v;
+    else u[id - g.lys] = gw.xf(3, v);
+  }
+  if (fl & 4) {
+    if (per.a[1]) u[id - g.ny * g.lys] = v;
+    else u[id + g.lys] = gw.xf(2, v);
+  }
+  if (fl & 32) {
+    if (per.a[2]) u[id + g.nz] = v;
+    else u[id - 1] = gw.xf(5, v);
+  }
+  if (fl & 16) {
+    if (per.a[2]) u[id - g.nz] = v;
+    else u[id + 1] = gw.xf(4, v);
+  }
+}
+
+// relax one cell (device or relax_on_host arithmetic) and its face ghosts
+template <bool HOST, typename T, typename D, typename C>
+__device__ __forceinline__ void bot_relax_cell(T *u, const T *b, const BotGeom &g, const C &c, const GhostL1<T> &gw,
+                                               const BotPer &per, const BotCell<D> &s) {
+  const int id = s.id;
+  T v;
+  if constexpr (HOST)
+    v = relax_pt_host_d(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs], u[id + g.lxs],
+                        b[id], s.d);
+  else
+    v = relax_pt(c, u[id], u[id - 1], u[id + 1], u[id - g.lys], u[id + g.lys], u[id - g.lxs], u[id + g.lxs], b[id],
+                 s.d);
+  u[id] = v;
+  if (s.fl) face_ghosts_fl(u, g, gw, per, id, s.fl, v);
+}
+
+// iters red-black sweeps of a level (HOST: the coarsest's relax_on_host
+// arithmetic).  Up to BOT_SLOTS cells of each colour per thread are decoded
+// once and kept in registers for all sweeps; larger levels decode per
+// sweep.  The face ghosts follow every update, so the 7-point reads of the
+// level need no fill (edges and corners are left to the fill before
+// anything reads them).
+constexpr int BOT_SLOTS = 2;
+template <bool HOST, typename T, typename C>
+__device__ __forceinline__ void bot_sweeps(T *u, const T *b, const BotGeom &g, const C &c, const GhostL1<T> &gw,
+                                           const BotPer &per, int iters) {
+  int n;
+  if constexpr (HOST) n = g.nx * g.ny * (int)g.mhh.n;
+  else n = g.nx * g.ny * (int)g.mhd.n;
+  if (n <= BOT_SLOTS * (int)blockDim.x) {
+    BotCell<T> cell[2][BOT_SLOTS];
+#pragma unroll
+    for (int col = 0; col < 2; col++)
+#pragma unroll
+      for (int q = 0; q < BOT_SLOTS; q++) cell[col][q] = bot_cell<HOST, T>(g, c, col, threadIdx.x + q * blockDim.x);
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+      for (int col = 0; col < 2; col++) {
+#pragma unroll
+        for (int q = 0; q < BOT_SLOTS; q++)
+          if (!(cell[col][q].fl & BOT_NONE)) bot_relax_cell<HOST>(u, b, g, c, gw, per, cell[col][q]);
+        __syncthreads();
+      }
+    }
+  } else {
+    for (int it = 0; it < iters; it++)
+      for (int col = 0; col < 2; col++) {
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+          const BotCell<T> s = bot_cell<HOST, T>(g, c, col, t);
+          if (!(s.fl & BOT_NONE)) bot_relax_cell<HOST>(u, b, g, c, gw, per, s);
+        }
+        __syncthreads();
+      }
   }
 }
 
@@ -1541,15 +1634,8 @@ __device__ void bot_fill(T *u, const BotGeom &g, const GhostL1<T> &gw, const Bot
 }
 
 template <typename T>
-__device__ void bot_relax(T *u, const T *b, const BotLevel<T> &L, const BotPer &per, int iters) {
-  for (int it = 0; it < iters; it++) {
-    bot_relax_dev(u, b, L.g, L.dc, L.gw, per, 0);
-    __syncthreads();
-    bot_relax_dev(u, b, L.g, L.dc, L.gw, per, 1);
-    __syncthreads();
-  }
-  bot_fill(u, L.g, L.gw, per);
-  __syncthreads();
+__device__ __forceinline__ void bot_relax(T *u, const T *b, const BotLevel<T> &L, const BotPer &per, int iters) {
+  bot_sweeps<false>(u, b, L.g, L.dc, L.gw, per, iters);
 }
 
 // b_c = restrict(b_f - A u_f) over the coarse interior (k_res_restrict)
@@ -1569,9 +1655,12 @@ __device__ void bot_res_restrict(const T *uf, const T *bf, const BotGeom &gf, co
   }
 }
 
-// u_f += trilinear prolongation of u_c (k_prolong, reference corner choice)
+// u_f += trilinear prolongation of u_c (k_prolong, reference corner choice);
+// the face ghosts of every updated cell follow it (what the level-1 fill
+// after the prolongation would give them)
 template <typename T>
-__device__ void bot_prolong(const T *uc, const BotGeom &gc, T *uf, const BotGeom &gf) {
+__device__ void bot_prolong(const T *uc, const BotGeom &gc, T *uf, const BotGeom &gf, const GhostL1<T> &gw,
+                            const BotPer &per) {
   const int n = (int)gc.mn.n;
   for (int t = threadIdx.x; t < 8 * n; t += blockDim.x) {
     const int q = mdiv(t, gc.mn), ct = t - q * n;  // child-major: consecutive threads walk coarse cells
@@ -1584,8 +1673,11 @@ __device__ void bot_prolong(const T *uc, const BotGeom &gc, T *uf, const BotGeom
     const T X = uc[id + sx], Y = uc[id + sy], Z = uc[id + sz];
     const T XY = uc[id + sx + sy], XZ = uc[id + sx + sz], YZ = uc[id + sy + sz];
     const T XYZ = uc[id + cx + sy + sz];
-    T &f = uf[gf.lshift + (2 * ic + dx) * gf.lxs + (2 * jc + dy) * gf.lys + 2 * kc + dz];
-    f = f + PW<T>::c0 * C + PW<T>::c1 * (X + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+    const int i = 2 * ic + dx, j = 2 * jc + dy, k = 2 * kc + dz;
+    const int idf = gf.lshift + i * gf.lxs + j * gf.lys + k;
+    const T v = uf[idf] + PW<T>::c0 * C + PW<T>::c1 * (X + Y + Z) + PW<T>::c2 * (XY + XZ + YZ) + PW<T>::c3 * XYZ;
+    uf[idf] = v;
+    face_ghosts(uf, gf, gw, per, idf, i, j, k, v);
   }
 }
 
@@ -1613,25 +1705,35 @@ __device__ void bot_zero(T *s, int n) {
 }
 
 template <typename T, typename TC, int MODE>
-__global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> A) {
+__global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> args) {
+  // the arguments are read where they lie, in the kernel-argument segment
+  // (the by-value parameter is its first and only explicit argument): a
+  // private copy of a struct this large ends up in scratch memory
+  (void)args;
+  const BotArgs<T, TC> &A = *(const BotArgs<T, TC> *)__builtin_amdgcn_kernarg_segment_ptr();
   extern __shared__ unsigned char smem[];
   auto U = [&](int l) { return (T *)(smem + A.L[l].off); };
   auto B = [&](int l) { return (T *)(smem + A.L[l].off) + box_cells(A.L[l].g); };
   const int C = A.nlev - 1;
 
-  // top level: u (or the zero grid with its zero ghosts) and b
+  // top level: u (or the zero grid with its zero ghosts) and b; the u and
+  // b boxes below start at zero (the b ghosts stay zero)
   if (A.zero_top)
     bot_zero(U(0), box_cells(A.L[0].g));
   else
     bot_box<T, true>(U(0), A.L[0].u, A.L[0].g);
   bot_box<T, true>(B(0), A.L[0].b, A.L[0].g);
+  for (int l = 1; l <= C; l++) bot_zero(U(l), 2 * box_cells(A.L[l].g));
   __syncthreads();
 
-  // down: relax, zero start below, residual -> restriction
+  // down: relax, residual -> restriction (the residual reads face ghosts
+  // only, which the relaxation keeps; without a sweep they are refilled)
   for (int l = 0; l < C; l++) {
     bot_relax(U(l), B(l), A.L[l], A.per, A.nu1);
-    bot_zero(U(l + 1), 2 * box_cells(A.L[l + 1].g));  // u and b boxes (b ghosts stay zero)
-    __syncthreads();
+    if (A.nu1 == 0) {
+      bot_fill(U(l), A.L[l].g, A.L[l].gw, A.per);
+      __syncthreads();
+    }
     bot_res_restrict(U(l), B(l), A.L[l].g, A.L[l].rc, B(l + 1), A.L[l + 1].g);
     __syncthreads();
   }
@@ -1651,11 +1753,7 @@ __global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> A) {
       }
       __syncthreads();
     }
-    for (int it = 0; it < A.coarse_iters; it++)
-      for (int rb = 0; rb < 2; rb++) {
-        bot_relax_host(uc, bc, g, A.hc, A.gwc, A.per, rb);
-        __syncthreads();
-      }
+    bot_sweeps<true>(uc, bc, g, A.hc, A.gwc, A.per, A.coarse_iters);
     bot_fill(uc, g, A.gwc, A.per);
     __syncthreads();
     if (MODE == 1) {
@@ -1664,13 +1762,14 @@ __global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> A) {
     }
   }
 
-  // up: prolongation + level-1 fill, relax
+  // up: prolongation (with its face ghosts), relax, level-1 fill (the
+  // next prolongation reads edge and corner ghosts of this level)
   for (int l = C - 1; l >= 0; l--) {
-    bot_prolong(U(l + 1), A.L[l + 1].g, U(l), A.L[l].g);
-    __syncthreads();
-    bot_fill(U(l), A.L[l].g, A.L[l].gw, A.per);
+    bot_prolong(U(l + 1), A.L[l + 1].g, U(l), A.L[l].g, A.L[l].gw, A.per);
     __syncthreads();
     bot_relax(U(l), B(l), A.L[l], A.per, A.nu2);
+    bot_fill(U(l), A.L[l].g, A.L[l].gw, A.per);
+    __syncthreads();
   }
 
   // write back every level (u boxes, b interiors below the top)
