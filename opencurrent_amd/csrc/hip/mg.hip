@@ -1506,33 +1506,26 @@ __device__ __forceinline__ BotCell<D> bot_cell(const BotGeom &g, const C &c, int
   return s;
 }
 
-// face ghosts of a cell from its face bits (face_ghosts' writes)
+// face ghosts of a cell from its face bits (face_ghosts' writes): values
+// and targets are computed for every face, so the transform coefficients
+// are read before the branches, which only store
 template <typename T>
 __device__ __forceinline__ void face_ghosts_fl(T *u, const BotGeom &g, const GhostL1<T> &gw, const BotPer &per,
                                                int id, int fl, T v) {
-  if (fl & 2) {
-    if (per.a[0]) u[id + g.nx * g.lxs] = v;
-    else u[id - g.lxs] = gw.xf(1, v);
+  const int st[3] = {g.lxs, g.lys, 1}, nn[3] = {g.nx, g.ny, g.nz};
+  T lv[3], hv[3];
+  int lt[3], ht[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    lv[a] = per.a[a] ? v : gw.xf(2 * a + 1, v);
+    hv[a] = per.a[a] ? v : gw.xf(2 * a, v);
+    lt[a] = per.a[a] ? id + nn[a] * st[a] : id - st[a];
+    ht[a] = per.a[a] ? id - nn[a] * st[a] : id + st[a];
   }
-  if (fl & 1) {
-    if (per.a[0]) u[id - g.nx * g.lxs] = v;
-    else u[id + g.lxs] = gw.xf(0, v);
-  }
-  if (fl & 8) {
-    if (per.a[1]) u[id + g.ny * g.lys] = v;
-    else u[id - g.lys] = gw.xf(3, v);
-  }
-  if (fl & 4) {
-    if (per.a[1]) u[id - g.ny * g.lys] = v;
-    else u[id + g.lys] = gw.xf(2, v);
-  }
-  if (fl & 32) {
-    if (per.a[2]) u[id + g.nz] = v;
-    else u[id - 1] = gw.xf(5, v);
-  }
-  if (fl & 16) {
-    if (per.a[2]) u[id - g.nz] = v;
-    else u[id + 1] = gw.xf(4, v);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (fl & (2 << (2 * a))) u[lt[a]] = lv[a];
+    if (fl & (1 << (2 * a))) u[ht[a]] = hv[a];
   }
 }
 
@@ -1559,9 +1552,22 @@ __device__ __forceinline__ void bot_relax_cell(T *u, const T *b, const BotGeom &
 // level need no fill (edges and corners are left to the fill before
 // anything reads them).
 constexpr int BOT_SLOTS = 2;
+// a uniform value held in scalar registers from here on (a kernel-argument
+// load would otherwise be repeated inside every branch that uses it)
+__device__ __forceinline__ float held(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double held(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 template <bool HOST, typename T, typename C>
-__device__ __forceinline__ void bot_sweeps(T *u, const T *b, const BotGeom &g, const C &c, const GhostL1<T> &gw,
+__device__ __forceinline__ void bot_sweeps(T *u, const T *b, const BotGeom &g, const C &c, const GhostL1<T> &gw_arg,
                                            const BotPer &per, int iters) {
+  GhostL1<T> gw;
+#pragma unroll
+  for (int f = 0; f < 6; f++) gw.s[f] = held(gw_arg.s[f]), gw.add[f] = held(gw_arg.add[f]);
   int n;
   if constexpr (HOST) n = g.nx * g.ny * (int)g.mhh.n;
   else n = g.nx * g.ny * (int)g.mhd.n;
@@ -1681,16 +1687,40 @@ __device__ void bot_prolong(const T *uc, const BotGeom &gc, T *uf, const BotGeom
   }
 }
 
-// LDS box <-> global grid (ghosts included), LDS interior -> global
-template <typename T, bool LOAD>
-__device__ void bot_box(T *s, const DGrid<T> &gr, const BotGeom &g) {
-  for (int t = threadIdx.x; t < box_cells(g); t += blockDim.x) {
-    const int row = mdiv(t, g.mpz), i = mdiv(row, g.mpy), j = row - i * (g.ny + 2), k = t - row * (g.nz + 2);
-    if (LOAD)
-      s[t] = gr.at(i - 1, j - 1, k - 1);
-    else
-      gr.at(i - 1, j - 1, k - 1) = s[t];
+// Global boxes (ghosts included) -> LDS: NG grids of the same geometry
+// (u and b of the top level, or b alone), BOT_LOADS cells per thread and
+// grid at a time, every global load in flight before the first LDS store
+// waits for one.  And LDS box -> global grid.
+constexpr int BOT_LOADS = 8;
+template <typename T>
+__device__ __forceinline__ T &box_at(const DGrid<T> &gr, const BotGeom &g, int t) {
+  const int row = mdiv(t, g.mpz), i = mdiv(row, g.mpy), j = row - i * (g.ny + 2), k = t - row * (g.nz + 2);
+  return gr.at(i - 1, j - 1, k - 1);
+}
+template <int NG, typename T>
+__device__ void bot_load(T *const (&s)[NG], const DGrid<T> (&gr)[NG], const BotGeom &g) {
+  const int n = box_cells(g);
+  for (int t0 = threadIdx.x; t0 < n; t0 += BOT_LOADS * blockDim.x) {
+    T v[NG][BOT_LOADS];
+#pragma unroll
+    for (int q = 0; q < BOT_LOADS; q++) {
+      const int t = t0 + q * blockDim.x;
+      const int tc = min(t, n - 1);  // past the box: a redundant load, no store
+#pragma unroll
+      for (int a = 0; a < NG; a++) v[a][q] = box_at(gr[a], g, tc);
+    }
+#pragma unroll
+    for (int q = 0; q < BOT_LOADS; q++) {
+      const int t = t0 + q * blockDim.x;
+#pragma unroll
+      for (int a = 0; a < NG; a++)
+        if (t < n) s[a][t] = v[a][q];
+    }
   }
+}
+template <typename T>
+__device__ void bot_store_box(const T *s, const DGrid<T> &gr, const BotGeom &g) {
+  for (int t = threadIdx.x; t < box_cells(g); t += blockDim.x) box_at(gr, g, t) = s[t];
 }
 template <typename T>
 __device__ void bot_store_interior(const T *s, const DGrid<T> &gr, const BotGeom &g) {
@@ -1718,11 +1748,16 @@ __global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> args) 
 
   // top level: u (or the zero grid with its zero ghosts) and b; the u and
   // b boxes below start at zero (the b ghosts stay zero)
-  if (A.zero_top)
+  if (A.zero_top) {
     bot_zero(U(0), box_cells(A.L[0].g));
-  else
-    bot_box<T, true>(U(0), A.L[0].u, A.L[0].g);
-  bot_box<T, true>(B(0), A.L[0].b, A.L[0].g);
+    T *const s[1] = {B(0)};
+    const DGrid<T> gr[1] = {A.L[0].b};
+    bot_load<1>(s, gr, A.L[0].g);
+  } else {
+    T *const s[2] = {U(0), B(0)};
+    const DGrid<T> gr[2] = {A.L[0].u, A.L[0].b};
+    bot_load<2>(s, gr, A.L[0].g);
+  }
   for (int l = 1; l <= C; l++) bot_zero(U(l), 2 * box_cells(A.L[l].g));
   __syncthreads();
 
@@ -1774,7 +1809,7 @@ __global__ __launch_bounds__(BOT_THREADS) void k_mg_bottom(BotArgs<T, TC> args) 
 
   // write back every level (u boxes, b interiors below the top)
   for (int l = 0; l <= C; l++) {
-    bot_box<T, false>(U(l), A.L[l].u, A.L[l].g);
+    bot_store_box(U(l), A.L[l].u, A.L[l].g);
     if (l > 0) bot_store_interior(B(l), A.L[l].b, A.L[l].g);
   }
 }
