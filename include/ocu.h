@@ -27,7 +27,8 @@ extern "C" {
 int ocu_set_compute_stream(void *stream);
 void *ocu_get_compute_stream(void);
 
-/* options of the multigrid solvers (OR-ed; OCU_MG_DEFAULT = all):
+/* options of the multigrid solvers (OR-ed; OCU_MG_DEFAULT = all but
+ * OCU_MG_GRAPHS):
  *   OCU_MG_FUSED_RELAX: one launch per red+black iteration, ghosts written by
  *     the sweep, deferred zero clears (Neumann/Dirichlet faces);
  *   OCU_MG_LDS_BOTTOM: the coarse levels of each V-cycle as one launch of a
@@ -35,7 +36,12 @@ void *ocu_get_compute_stream(void);
  *   OCU_MG_FUSED_PROLONG (with OCU_MG_FUSED_RELAX): a prolongation into a
  *     level with rows of 129..256 cells is added by the fused sweep that
  *     follows it instead of a launch of its own;
- *   (these three give results bit-identical to the plain per-level path)
+ *   OCU_MG_GRAPHS (FP64/FP32 solver): the launches between two residual
+ *     read-backs of the finest V-cycles are recorded once as a HIP graph
+ *     and replayed while they repeat (the compute stream must not be the
+ *     default stream; opt-in: the replay measured no faster than the
+ *     launches on MI355X);
+ *   (these four give results bit-identical to the plain per-level path)
  *   OCU_MG_ADAPTIVE_IR (mixed solver): after the first refinement iteration,
  *     as many V-cycles (at most `cycles`) as the observed contraction per
  *     cycle needs to reach half the tolerance. */
@@ -44,6 +50,7 @@ enum ocu_mg_options {
   OCU_MG_LDS_BOTTOM = 2,
   OCU_MG_ADAPTIVE_IR = 4,
   OCU_MG_FUSED_PROLONG = 8,
+  OCU_MG_GRAPHS = 16,
   OCU_MG_DEFAULT = 15
 };
 
@@ -65,6 +72,9 @@ void ocu_mg_destroy(void *h);
  * adding the pending prolongation (profiling: the kinds run at different
  * speeds); ocu_mgmixed_sweep_counts: those of the FP32 hierarchy */
 int ocu_mg_sweep_counts(void *h, long long *counts);
+/* HIP graphs of V-cycle segments since the handle was created: counts[0]
+ * recorded, counts[1] replayed */
+int ocu_mg_graph_counts(void *h, long long *counts);
 /* relaxation parameters of a level as the solver hands them to the kernels
  * (ocu_mgmixed_get_level_params: the FP32 hierarchy's) */
 int ocu_mg_get_level_params(void *h, int level, ocu_mg_level_params *out);

@@ -160,6 +160,7 @@ HOST_SIGS = {
     "ocu_mg_run_vcycles": (I, [P, I]),
     "ocu_mg_run_fmg": (I, [P, I]),
     "ocu_mg_stats": (I, [P, IP, IP, DP, I, IP]),
+    "ocu_mg_graph_counts": (I, [P, C.POINTER(C.c_longlong)]),
     "ocu_mg_destroy": (None, [P]),
     "ocu_mg_sweep_counts": (I, [P, C.POINTER(C.c_longlong)]),
     "ocu_mg_get_level_params": (I, [P, I, C.POINTER(MGLevelParams)]),

@@ -179,6 +179,7 @@ void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, cons
     h->d->use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
     h->d->use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
     h->d->use_fused_prolong = (options & OCU_MG_FUSED_PROLONG) != 0;
+    h->d->use_graphs = (options & OCU_MG_GRAPHS) != 0;
     ok = h->rhs_d.init_external(*rhs) &&
          h->d->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_d);
   } else {
@@ -188,6 +189,7 @@ void *ocu_mg_create(const ocu_grid3d *rhs, double hx, double hy, double hz, cons
     h->f->use_fused_relax = (options & OCU_MG_FUSED_RELAX) != 0;
     h->f->use_bottom = (options & OCU_MG_LDS_BOTTOM) != 0;
     h->f->use_fused_prolong = (options & OCU_MG_FUSED_PROLONG) != 0;
+    h->f->use_graphs = (options & OCU_MG_GRAPHS) != 0;
     ok = h->rhs_f.init_external(*rhs) &&
          h->f->initialize_storage(rhs->nx, rhs->ny, rhs->nz, hx, hy, hz, &h->rhs_f);
   }
@@ -250,6 +252,14 @@ int ocu_mg_sweep_counts(void *hh, long long *counts) {
   if (!h || !counts) return OCU_ERR_INVALID;
   const long long *c = h->d ? h->d->level0_sweeps : h->f->level0_sweeps;
   for (int q = 0; q < 3; q++) counts[q] = c[q];
+  return OCU_OK;
+}
+
+int ocu_mg_graph_counts(void *hh, long long *counts) {
+  MGHandle *h = (MGHandle *)hh;
+  if (!h || !counts) return OCU_ERR_INVALID;
+  counts[0] = h->d ? h->d->graph_records : h->f->graph_records;
+  counts[1] = h->d ? h->d->graph_replays : h->f->graph_replays;
   return OCU_OK;
 }
 

@@ -17,6 +17,7 @@
 #define __OCU_EQUATION_MULTIGRID_PRESSURE_3D_DEV_H__
 
 #include <cmath>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,20 @@ class Sol_MultigridPressure3DBase : public Solver {
   // in one workgroup), or -1; bottom_vcycle(top) runs that part from `top`
   virtual int bottom_start() const { return -1; }
   virtual void bottom_vcycle(int /*top*/) {}
+  // restrict_residuals with norms in two halves: the launches, then the
+  // wait for the norms and their values (the default runs it whole)
+  virtual void restrict_residuals_launch(int fine_level, int coarse_level, bool norms) {
+    restrict_residuals(fine_level, coarse_level, norms ? &_pend_l2 : 0, norms ? &_pend_linf : 0);
+  }
+  virtual void norms_read(int /*fine_level*/, double *l2, double *linf) {
+    if (l2) *l2 = _pend_l2;
+    if (linf) *linf = _pend_linf;
+  }
+  // the launches between two norm read-backs of do_fmg's finest cycles (the
+  // rest of one V-cycle and the next one's level-0 smoothing and residual);
+  // an implementation may record them once and replay them
+  virtual void run_segment(const std::function<void()> &body) { body(); }
+  double _pend_l2 = 0, _pend_linf = 0;
 
  public:
   BoundaryConditionSet bc;
@@ -163,6 +178,24 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   virtual int bottom_start() const;
   virtual void bottom_vcycle(int top);
 
+  virtual void restrict_residuals_launch(int fine_level, int coarse_level, bool norms);
+  virtual void norms_read(int fine_level, double *l2, double *linf);
+  virtual void run_segment(const std::function<void()> &body);
+  bool _norms_pending;  // norm kernels launched, values not read yet
+  // recorded segments: the solver state they start from (options, stream,
+  // deferred clears and prolongations per level), the flags they leave, the
+  // level-0 sweep counts they add, and the executable graph
+  struct Segment {
+    std::vector<long long> key;
+    std::vector<char> out;
+    long long sweeps[3];
+    void *exec;
+  };
+  std::vector<Segment> _segments;
+  bool _graphs_failed;  // a capture failed: run segments directly from then on
+  std::vector<long long> segment_key(void *stream) const;
+  void drop_segments();
+
  public:
   bool disable_relax_on_host;
   // one launch per red+black iteration when no face is periodic
@@ -174,6 +207,12 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   // with use_fused_relax: a prolongation is added by the fused relaxation
   // that follows it instead of a launch of its own
   bool use_fused_prolong;
+  // replay the launches between two norm read-backs of the finest V-cycles
+  // as a HIP graph once they repeat (the same kernels with the same
+  // arguments, so bit-identical to launching them one by one; off by
+  // default: no faster than the launches themselves on MI355X)
+  bool use_graphs;
+  long long graph_records, graph_replays;  // segments recorded / replayed since construction
   // fused red+black sweeps launched at level 0 since construction: [0]
   // plain, [1] starting from the deferred zero grid, [2] adding the pending
   // prolongation (for profiling: the three run at different speeds)

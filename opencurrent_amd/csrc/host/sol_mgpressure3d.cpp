@@ -13,6 +13,13 @@
 #include "ocuutil/kernel_wrapper.h"
 #include "ocuutil/thread.h"
 
+// whether a solver records and replays its V-cycle segments unless told
+// otherwise: off, since replaying the graph measured no faster than the
+// launches themselves on MI355X (the macro exists for tuning builds)
+#ifndef OCU_MG_GRAPHS_DEFAULT
+#define OCU_MG_GRAPHS_DEFAULT 0
+#endif
+
 namespace ocu {
 
 // ------------------------------------------------------------------------
@@ -129,32 +136,23 @@ bool Sol_MultigridPressure3DBase::do_fmg(double tolerance, int max_iter, double 
   }
 
   const int bottom = bottom_start();
+  const bool want_l2 = (convergence & CONVERGENCE_CALC_L2) != 0, want_linf = (convergence & CONVERGENCE_CALC_LINF) != 0;
   for (int fine_level = _num_levels - 1; fine_level >= 0; fine_level--) {
     int level_ncyc = (fine_level == 0) ? max_iter + 1 : 1;
     // levels top..coarse_level run as one launch when a bottom exists
     const int top = (bottom >= 1 && fine_level < coarse_level) ? max2(bottom, fine_level) : -1;
     const int stop = top >= 0 ? top : coarse_level;
-    for (int i_cyc = 0; i_cyc < level_ncyc; i_cyc++) {
-      if (fine_level == 0) num_vcycles++;
-      for (int level = fine_level; level < stop; level++) {
+    // pre-smoothing and restriction from level `from` down, the bottom (or
+    // the coarsest relaxation), and the way back up
+    auto down_from = [&](int from) {
+      for (int level = from; level < stop; level++) {
         relax(level, nu1, RO_RED_BLACK);
         clear_zero(level + 1);
         apply_boundary_conditions(level + 1);
-        if (level == 0) {
-          restrict_residuals(0, 1, (convergence & CONVERGENCE_CALC_L2) ? &result_l2 : 0,
-                             (convergence & CONVERGENCE_CALC_LINF) ? &result_linf : 0);
-          double residual = pick(convergence, result_l2, result_linf);
-          residual_history.push_back(residual);
-          if (verbose) printf("%d: residual = %.12f,%.12f\n", i_cyc, result_linf, result_l2);
-          if (residual < tolerance) {
-            // ghosts of u[0] are stale after the pre-smoothing
-            apply_boundary_conditions(0);
-            return !any_error();
-          }
-        } else {
-          restrict_residuals(level, level + 1, 0, 0);
-        }
+        restrict_residuals(level, level + 1, 0, 0);
       }
+    };
+    auto bottom_and_up = [&]() {
       if (top >= 0) {
         bottom_vcycle(top);
       } else {
@@ -165,6 +163,48 @@ bool Sol_MultigridPressure3DBase::do_fmg(double tolerance, int max_iter, double 
       for (int level = stop - 1; level >= fine_level; level--) {
         prolong(level + 1, level);
         if (level > 0) relax(level, nu2, make_symmetric_operator ? RO_BLACK_RED : RO_RED_BLACK);
+      }
+    };
+    if (fine_level > 0 || stop < 1) {
+      for (int i_cyc = 0; i_cyc < level_ncyc; i_cyc++) {
+        if (fine_level == 0) num_vcycles++;
+        down_from(fine_level);
+        bottom_and_up();
+      }
+    } else {
+      // the finest cycles: level 0's pre-smoothing ends in the residual with
+      // its norms, which are read before the rest of the cycle runs; what is
+      // launched between two read-backs (the rest of one cycle and the start
+      // of the next) is one segment
+      auto head = [&]() {
+        relax(0, nu1, RO_RED_BLACK);
+        clear_zero(1);
+        apply_boundary_conditions(1);
+        restrict_residuals_launch(0, 1, want_l2 || want_linf);
+      };
+      num_vcycles++;
+      head();
+      for (int i_cyc = 0;; i_cyc++) {
+        norms_read(0, want_l2 ? &result_l2 : 0, want_linf ? &result_linf : 0);
+        double residual = pick(convergence, result_l2, result_linf);
+        residual_history.push_back(residual);
+        if (verbose) printf("%d: residual = %.12f,%.12f\n", i_cyc, result_linf, result_l2);
+        if (residual < tolerance) {
+          // ghosts of u[0] are stale after the pre-smoothing
+          apply_boundary_conditions(0);
+          return !any_error();
+        }
+        if (i_cyc + 1 >= level_ncyc) {
+          down_from(1);
+          bottom_and_up();
+          break;
+        }
+        run_segment([&]() {
+          down_from(1);
+          bottom_and_up();
+          head();
+        });
+        num_vcycles++;
       }
     }
     if (fine_level > 0) prolong(fine_level, fine_level - 1);
@@ -307,12 +347,17 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   use_bottom = true;
   _bottom_level = -1;
   use_fused_prolong = true;
+  use_graphs = OCU_MG_GRAPHS_DEFAULT;
+  graph_records = graph_replays = 0;
+  _graphs_failed = false;
+  _norms_pending = false;
   level0_sweeps[0] = level0_sweeps[1] = level0_sweeps[2] = 0;
   for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = _u_pro[l] = _fused_lvl[l] = false;
 }
 
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::free_storage() {
+  drop_segments();
   if (_u_grid) {
     for (int l = 0; l < _num_levels; l++) {
       if (l > 0) {
@@ -601,10 +646,18 @@ void Sol_MultigridPressure3DDevice<T>::relax(int level, int iterations, RelaxOrd
 template <typename T>
 void Sol_MultigridPressure3DDevice<T>::restrict_residuals(int fine_level, int coarse_level, double *l2,
                                                           double *linf) {
+  restrict_residuals_launch(fine_level, coarse_level, l2 || linf);
+  if (l2 || linf) norms_read(fine_level, l2, linf);
+  if (any_error())
+    printf("[ERROR] Sol_MultigridPressure3DDevice::restrict_residuals - failed at level %d -> %d\n", fine_level,
+           coarse_level);
+}
+
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::restrict_residuals_launch(int fine_level, int coarse_level, bool norms) {
   materialize(fine_level);
   void *s = ThreadManager::get_compute_stream();
   ocu_grid3d u = get_u(fine_level).desc(), b = get_b(fine_level).desc(), r = get_r(fine_level).desc();
-  bool norms = l2 || linf;
   double h = get_h(fine_level);
   if (coarse_level != fine_level) {
     ocu_grid3d bc = get_b(coarse_level).desc();
@@ -617,16 +670,108 @@ void Sol_MultigridPressure3DDevice<T>::restrict_residuals(int fine_level, int co
   }
   if (norms) {
     // the finalize kernel writes the two results straight into pinned host
-    // memory: no copy launch, one stream synchronisation
+    // memory: no copy launch, one stream synchronisation when they are read
     ok_status(ocu_norm_finalize(_norm_ws, _norm_dev, s), "Sol_MultigridPressure3DDevice::restrict_residuals");
-    ok_status(ocu_stream_synchronize(s), "Sol_MultigridPressure3DDevice::restrict_residuals");
-    double n = (double)nx(fine_level) * ny(fine_level) * nz(fine_level);
-    if (l2) *l2 = sqrt(_norm_host[0] / n);
-    if (linf) *linf = _norm_host[1];
+    _norms_pending = true;
   }
-  if (any_error())
-    printf("[ERROR] Sol_MultigridPressure3DDevice::restrict_residuals - failed at level %d -> %d\n", fine_level,
-           coarse_level);
+}
+
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::norms_read(int fine_level, double *l2, double *linf) {
+  if (!_norms_pending) return;
+  _norms_pending = false;
+  ok_status(ocu_stream_synchronize(ThreadManager::get_compute_stream()),
+            "Sol_MultigridPressure3DDevice::restrict_residuals");
+  double n = (double)nx(fine_level) * ny(fine_level) * nz(fine_level);
+  if (l2) *l2 = sqrt(_norm_host[0] / n);
+  if (linf) *linf = _norm_host[1];
+}
+
+// what a segment's launches depend on besides the grids (fixed from
+// initialize_storage on): options, faces, the stream, and the deferred
+// clears and prolongations of every level
+template <typename T>
+std::vector<long long> Sol_MultigridPressure3DDevice<T>::segment_key(void *stream) const {
+  std::vector<long long> k = {nu1, nu2, use_fused_relax, use_bottom, use_fused_prolong, store_residuals,
+                              disable_relax_on_host, make_symmetric_operator, (long long)convergence, _coarse_mode,
+                              (long long)(intptr_t)stream, _num_levels};
+  const BoundaryCondition *f[6] = {&bc.xpos, &bc.xneg, &bc.ypos, &bc.yneg, &bc.zpos, &bc.zneg};
+  for (const BoundaryCondition *c : f) {
+    long long v, a;
+    memcpy(&v, &c->value, sizeof(v));
+    memcpy(&a, &c->aux_value, sizeof(a));
+    k.push_back(c->type);
+    k.push_back(v);
+    k.push_back(a);
+  }
+  for (int l = 0; l < _num_levels; l++) k.push_back(_u_zero[l] | _u_pro[l] << 1);
+  return k;
+}
+
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::drop_segments() {
+  for (Segment &g : _segments) ocu_graph_destroy(g.exec);
+  _segments.clear();
+}
+
+// The segment's launches go to a HIP graph the first time the solver is in
+// a given state at the start of a segment; later segments from that state
+// launch the graph and take the flags the recording left.  A failed
+// recording restores the state and runs the launches directly.
+template <typename T>
+void Sol_MultigridPressure3DDevice<T>::run_segment(const std::function<void()> &body) {
+  void *s = ThreadManager::get_compute_stream();
+  if (!use_graphs || _graphs_failed || !s) {
+    body();
+    return;
+  }
+  const std::vector<long long> key = segment_key(s);
+  for (Segment &g : _segments) {
+    if (g.key != key) continue;
+    if (!ok_status(ocu_graph_launch(g.exec, s), "Sol_MultigridPressure3DDevice::run_segment")) return;
+    for (int l = 0; l < _num_levels; l++) {
+      _u_zero[l] = g.out[l] & 1;
+      _u_pro[l] = (g.out[l] >> 1) & 1;
+    }
+    for (int q = 0; q < 3; q++) level0_sweeps[q] += g.sweeps[q];
+    _norms_pending = true;
+    graph_replays++;
+    return;
+  }
+  // the state to return to if the recording fails
+  std::vector<char> z(_num_levels), p(_num_levels);
+  for (int l = 0; l < _num_levels; l++) z[l] = _u_zero[l], p[l] = _u_pro[l];
+  long long sw[3] = {level0_sweeps[0], level0_sweeps[1], level0_sweeps[2]};
+  const bool pending = _norms_pending, err = any_error();
+  void *exec = nullptr;
+  bool ok = ocu_stream_begin_capture(s) == OCU_OK;
+  if (ok) {
+    body();
+    ok = ocu_stream_end_capture(s, &exec) == OCU_OK && exec && (err || !any_error());
+  }
+  if (!ok) {
+    if (exec) ocu_graph_destroy(exec);
+    for (int l = 0; l < _num_levels; l++) _u_zero[l] = z[l], _u_pro[l] = p[l];
+    for (int q = 0; q < 3; q++) level0_sweeps[q] = sw[q];
+    _norms_pending = pending;
+    if (!err) clear_error();
+    _graphs_failed = true;
+    body();
+    return;
+  }
+  Segment g;
+  g.key = key;
+  g.out.resize(_num_levels);
+  for (int l = 0; l < _num_levels; l++) g.out[l] = (char)(_u_zero[l] | _u_pro[l] << 1);
+  for (int q = 0; q < 3; q++) g.sweeps[q] = level0_sweeps[q] - sw[q];
+  g.exec = exec;
+  if (_segments.size() >= 4) {
+    ocu_graph_destroy(_segments.front().exec);
+    _segments.erase(_segments.begin());
+  }
+  _segments.push_back(g);
+  graph_records++;
+  ok_status(ocu_graph_launch(exec, s), "Sol_MultigridPressure3DDevice::run_segment");
 }
 
 template <typename T>

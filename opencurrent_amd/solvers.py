@@ -18,12 +18,12 @@ from ._types import (BCSet, CONVERGENCE_L2, Grid3DDesc, MGConfig, MGLevelParams,
 from .device import desc_to_host, host_to_desc
 
 
-MG_FUSED_RELAX, MG_LDS_BOTTOM, MG_ADAPTIVE_IR, MG_FUSED_PROLONG = 1, 2, 4, 8
+MG_FUSED_RELAX, MG_LDS_BOTTOM, MG_ADAPTIVE_IR, MG_FUSED_PROLONG, MG_GRAPHS = 1, 2, 4, 8, 16
 
 
-def _options(use_fused_relax, use_bottom, use_fused_prolong=True):
+def _options(use_fused_relax, use_bottom, use_fused_prolong=True, use_graphs=False):
     return ((MG_FUSED_RELAX if use_fused_relax else 0) | (MG_LDS_BOTTOM if use_bottom else 0) |
-            (MG_FUSED_PROLONG if use_fused_prolong else 0))
+            (MG_FUSED_PROLONG if use_fused_prolong else 0) | (MG_GRAPHS if use_graphs else 0))
 
 
 def _history(fn, h, extra=()):
@@ -37,11 +37,12 @@ class MultigridPressure3D:
     """Sol_MultigridPressure3DDevice<T> (ocuequation/sol_mgpressure3d.h:130-181)."""
 
     def __init__(self, rhs, hx, hy, hz, bc, nu1=2, nu2=2, convergence=CONVERGENCE_L2, sym=False,
-                 disable_relax_on_host=False, use_fused_relax=True, use_bottom=True, use_fused_prolong=True):
+                 disable_relax_on_host=False, use_fused_relax=True, use_bottom=True, use_fused_prolong=True,
+                 use_graphs=False):
         self._rhs = rhs  # keep alive
         self.cfg = MGConfig.make(bc, nu1, nu2, convergence, sym, disable_relax_on_host)
         self.h = _lib.host().ocu_mg_create(rhs.ptr(), hx, hy, hz, C.byref(self.cfg),
-                                           _options(use_fused_relax, use_bottom, use_fused_prolong))
+                                           _options(use_fused_relax, use_bottom, use_fused_prolong, use_graphs))
         if not self.h:
             raise _lib.OcuError(f"Sol_MultigridPressure3DDevice::initialize_storage failed: {_lib.last_error()}")
 
@@ -83,6 +84,12 @@ class MultigridPressure3D:
         """level-0 fused sweeps so far: (plain, from the zero grid, adding a prolongation)"""
         c = (C.c_longlong * 3)()
         _lib.check(_lib.host().ocu_mg_sweep_counts(self.h, c), "sweep_counts")
+        return tuple(c)
+
+    def graph_counts(self):
+        """V-cycle segments recorded as HIP graphs and replayed so far: (recorded, replayed)"""
+        c = (C.c_longlong * 2)()
+        _lib.check(_lib.host().ocu_mg_graph_counts(self.h, c), "graph_counts")
         return tuple(c)
 
     def __del__(self):

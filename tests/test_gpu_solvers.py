@@ -122,6 +122,39 @@ def test_mg_cycles_with_fused_prolongation_match_oracle(gpu, dtype, case, n, opt
         assert np.array_equal(got.buf, want.buf), f"{fn}: max diff {np.abs(got.buf - want.buf).max()}"
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-4)])
+@pytest.mark.parametrize("case", ["neumann", "dirichlet_z321", "periodic"])
+def test_mg_graph_segments_match_direct_launches(gpu, dtype, tol, case):
+    """The launches between two residual read-backs of the finest V-cycles,
+    recorded once as a HIP graph and replayed, give the pressure, V-cycle
+    count and residual history of the same solver launching every kernel
+    itself, bit for bit; three solves in a row replay the recorded graph
+    with the host's own launches and synchronisations between replays."""
+    n = (64, 64, 64)
+    h = (1.0 / 64,) * 3
+    bc, b, rand_init = rhs_for(case, n, h, dtype)
+    p0 = init_guess(b, rand_init)
+    gb = D().DeviceGrid.from_host(b)
+    runs = {}
+    for graphs in (False, True):
+        mg = S().MultigridPressure3D(gb, *h, bc, use_graphs=graphs)
+        out = []
+        for _ in range(3):
+            mg.set_pressure(p0)
+            ok, res = mg.solve(tol, 12)
+            st = mg.stats()
+            out.append((ok, res, mg.pressure().buf.copy(), st["vcycles"], st["history"]))
+        runs[graphs] = (out, mg.graph_counts())
+    assert runs[False][1] == (0, 0)
+    recorded, replayed = runs[True][1]
+    assert recorded >= 1 and replayed >= 3, runs[True][1]
+    for (oka, ra, pa, va, ha), (okb, rb, pb, vb, hb) in zip(runs[False][0], runs[True][0]):
+        assert oka == okb and ra == rb
+        assert va == vb and va >= 3
+        assert ha == hb
+        assert np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("n,h", [((64, 32, 48), (1 / 64, 1 / 32, 1 / 48)), ((32, 32, 32), (.25, .25, .5))])
 def test_mg_anisotropic_and_sym(gpu, n, h):
     """anisotropic spacing (non power-of-two coefficients) and
