@@ -88,6 +88,28 @@ __device__ __forceinline__ T diag_dev(const C &c, int i, int j, int k, int nx, i
   return d;
 }
 
+// diag_dev in two steps: the row part (i and j faces) and the z ends on top,
+// the same additions in the same order
+template <typename T, typename C>
+__device__ __forceinline__ T diag_row(const C &c, int i, int j, int nx, int ny) {
+  T d = c.ld;
+  if (i == nx - 1) d += c.m[0];
+  if (i == 0) d += c.m[1];
+  if (j == ny - 1) d += c.m[2];
+  if (j == 0) d += c.m[3];
+  return d;
+}
+
+// a uniform value held in scalar registers from here on (a kernel-argument
+// load would otherwise be repeated inside every branch that uses it)
+__device__ __forceinline__ float held(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double held(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 // relax_DIAG point update (devd.cu:572-591)
 __device__ __forceinline__ double relax_pt(const DevC64 &c, double uc, double zm, double zp, double ym,
                                            double yp, double xm, double xp, double b, double diag) {
@@ -328,32 +350,38 @@ __device__ __forceinline__ void write_ghost_row(const DGrid<T> &o, const GhostL1
 // returned by value: stores through a reference from the two bodies would be
 // merged into one with a computed element index, which sends the caller's
 // row to scratch memory.
+//
+// drow: the row's diagonal without the z ends (diag_row of the row's i and
+// j); the z ends go on top per cell, k = 0 only for e = 0 (k0 = V * lane).
 template <int P, typename T, int V, typename C>
-__device__ __forceinline__ Vec<T, V> relax_row_elems(const C &c, int i, int j, int nx, int ny, int kpar, int k0,
-                                                     int kupd, int nz, const Vec<T, V> &cen, T lo, T hi,
-                                                     const Vec<T, V> &ym, const Vec<T, V> &yp, const Vec<T, V> &xm,
-                                                     const Vec<T, V> &xp, const Vec<T, V> &b, Vec<T, V> out) {
+__device__ __forceinline__ Vec<T, V> relax_row_elems(const C &c, T drow, int kpar, int k0, int kupd, int nz,
+                                                     const Vec<T, V> &cen, T lo, T hi, const Vec<T, V> &ym,
+                                                     const Vec<T, V> &yp, const Vec<T, V> &xm, const Vec<T, V> &xp,
+                                                     const Vec<T, V> &b, Vec<T, V> out) {
   constexpr int S = V == 1 ? 1 : 2;  // element stride
   constexpr int NE = V == 1 ? 1 : V / 2;
 #pragma unroll
   for (int m = 0; m < NE; m++) {
     const int e = P + S * m, k = k0 + e;
+    T d = drow;
+    if (k == nz - 1) d += c.m[4];
+    if (e == 0 && k == 0) d += c.m[5];
     const T nv = relax_pt(c, cen.v[e], e > 0 ? cen.v[e - 1] : lo, e < V - 1 ? cen.v[e + 1] : hi, ym.v[e], yp.v[e],
-                          xm.v[e], xp.v[e], b.v[e], diag_dev<T>(c, i, j, k, nx, ny, nz));
+                          xm.v[e], xp.v[e], b.v[e], d);
     const bool keep = V == 1 ? ((k & 1) == kpar && k < kupd) : k < kupd;
     out.v[e] = keep ? nv : out.v[e];
   }
   return out;
 }
 template <typename T, int V, typename C>
-__device__ __forceinline__ Vec<T, V> relax_row(const C &c, int i, int j, int nx, int ny, int kpar, int k0, int kupd,
-                                               int nz, const Vec<T, V> &cen, T lo, T hi, const Vec<T, V> &ym,
+__device__ __forceinline__ Vec<T, V> relax_row(const C &c, T drow, int kpar, int k0, int kupd, int nz,
+                                               const Vec<T, V> &cen, T lo, T hi, const Vec<T, V> &ym,
                                                const Vec<T, V> &yp, const Vec<T, V> &xm, const Vec<T, V> &xp,
                                                const Vec<T, V> &b, const Vec<T, V> &out) {
   static_assert(V == 1 || V % 2 == 0, "rows hold 1 or an even number of cells per lane");
   if (V == 1 || kpar == 0)
-    return relax_row_elems<0, T, V>(c, i, j, nx, ny, kpar, k0, kupd, nz, cen, lo, hi, ym, yp, xm, xp, b, out);
-  return relax_row_elems<1, T, V>(c, i, j, nx, ny, kpar, k0, kupd, nz, cen, lo, hi, ym, yp, xm, xp, b, out);
+    return relax_row_elems<0, T, V>(c, drow, kpar, k0, kupd, nz, cen, lo, hi, ym, yp, xm, xp, b, out);
+  return relax_row_elems<1, T, V>(c, drow, kpar, k0, kupd, nz, cen, lo, hi, ym, yp, xm, xp, b, out);
 }
 
 // Rows of a block: NW waves of R rows each; the outer two rows on each side
@@ -442,6 +470,16 @@ __global__ __launch_bounds__(64 * NW, (fused_rows_min_waves<T, PRO>())) void k_r
     brow[r] = b.p + (int64_t)(py ? jw[r] : min(max(jb + r, 0), ny - 1)) * b.ys + kc;
   }
   auto inner_j = [&](int r) { return jb + r >= 0 && jb + r < ny; };
+  // the rows' diagonals without the z ends on the first plane, the last
+  // plane and the planes between (uniform: held in scalar registers)
+  T dlo[R], dhi[R], dmid[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    dlo[r] = held(diag_row<T>(c, 0, jw[r], nx, ny));
+    dhi[r] = held(diag_row<T>(c, nx - 1, jw[r], nx, ny));
+    dmid[r] = held(diag_row<T>(c, -1, jw[r], nx, ny));
+  }
+  auto drow = [&](int r, int i) { return i == 0 ? dlo[r] : i == nx - 1 ? dhi[r] : dmid[r]; };
   auto plane_u = [&](int p) { return px ? wrap1(p, nx) : min(max(p, -1), nx); };
   auto plane_b = [&](int p) { return px ? wrap1(p, nx) : min(max(p, 0), nx - 1); };
   auto ring1 = [&](int r) { return t0 + r >= 1 && t0 + r <= NR - 2; };
@@ -671,7 +709,7 @@ __global__ __launch_bounds__(64 * NW, (fused_rows_min_waves<T, PRO>())) void k_r
         const int j = jw[r], pw = px ? wrap1(p, nx) : p;
         const VT ym = r > 0 ? cc[r > 0 ? r - 1 : 0] : sU0[u0slot(p)][t0 - 1][lane];
         const VT yp = r < R - 1 ? cc[r < R - 1 ? r + 1 : 0] : sU0[u0slot(p)][t0 + R][lane];
-        u1[r] = relax_row<T, V>(c, pw, j, nx, ny, (first_color ^ pw ^ j) & 1, k0, kupd, nz, cc[r], lo, hi, ym, yp,
+        u1[r] = relax_row<T, V>(c, drow(r, pw), (first_color ^ pw ^ j) & 1, k0, kupd, nz, cc[r], lo, hi, ym, yp,
                                 u1m1[r], n[r], bcur[r], u1[r]);
       }
       sU1[p & 1][t0 + r][lane] = u1[r];
@@ -690,8 +728,8 @@ __global__ __launch_bounds__(64 * NW, (fused_rows_min_waves<T, PRO>())) void k_r
         const int j = jb + r;
         const VT ym = r > 0 ? u1m1[r > 0 ? r - 1 : 0] : sU1[q & 1][t0 - 1][lane];
         const VT yp = r < R - 1 ? u1m1[r < R - 1 ? r + 1 : 0] : sU1[q & 1][t0 + R][lane];
-        const VT v = relax_row<T, V>(c, q, j, nx, ny, (first_color ^ q ^ j ^ 1) & 1, k0, kupd, nz, u1m1[r], lo, hi,
-                                     ym, yp, u1m2[r], u1[r], bprev[r], u1m1[r]);
+        const VT v = relax_row<T, V>(c, drow(r, q), (first_color ^ q ^ j ^ 1) & 1, k0, kupd, nz, u1m1[r], lo, hi, ym,
+                                     yp, u1m2[r], u1[r], bprev[r], u1m1[r]);
         if (active) {
           *reinterpret_cast<VT *>(uout.p + (int64_t)q * uout.xs + (int64_t)j * uout.ys + k0) = v;
           if (wghost) {  // ghosts of this row: z ends, x and y face rows
@@ -1552,16 +1590,6 @@ __device__ __forceinline__ void bot_relax_cell(T *u, const T *b, const BotGeom &
 // level need no fill (edges and corners are left to the fill before
 // anything reads them).
 constexpr int BOT_SLOTS = 2;
-// a uniform value held in scalar registers from here on (a kernel-argument
-// load would otherwise be repeated inside every branch that uses it)
-__device__ __forceinline__ float held(float x) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
-}
-__device__ __forceinline__ double held(double x) {
-  const long long b = __builtin_bit_cast(long long, x);
-  const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
 template <bool HOST, typename T, typename C>
 __device__ __forceinline__ void bot_sweeps(T *u, const T *b, const BotGeom &g, const C &c, const GhostL1<T> &gw_arg,
                                            const BotPer &per, int iters) {
