@@ -88,6 +88,14 @@ int ocu_grid_add_scalar(const ocu_grid3d *g, double value, void *stream);
  * hold ocu_reduce_workspace_bytes() bytes of device memory. */
 size_t ocu_reduce_workspace_bytes(void);
 int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, void *stream);
+/* ocu_grid_reduce that also signals its completion (no counterpart in the
+ * reference; for hosts that poll instead of synchronising the stream): out
+ * is pinned host memory as the device sees it with room for two doubles;
+ * after the result out[0], with a system-scope fence between, the 8 bytes
+ * of out[1] receive `tag` (nonzero).  A host that reads tag from out[1]
+ * (an acquire read) then finds the result in out[0]. */
+int ocu_grid_reduce_tagged(const ocu_grid3d *g, int op, void *workspace, double *out, unsigned long long tag,
+                           void *stream);
 /* ocu_grid_reduce without the workspace argument and without any allocation:
  * no counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP-graph capture, tight serving loops).  The partials
@@ -234,6 +242,10 @@ int ocu_mg_residual_norm(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_gri
                          void *stream);
 /* out[0] = sum r^2, out[1] = max |r| (device doubles) from the partials */
 int ocu_norm_finalize(const double *norm_ws, double *out, void *stream);
+/* the same with a completion signal (no counterpart in the reference): out
+ * has room for three doubles and receives `tag` (nonzero) in the 8 bytes of
+ * out[2] after the two results, with a system-scope fence between */
+int ocu_norm_finalize_tagged(const double *norm_ws, double *out, unsigned long long tag, void *stream);
 /* u_fine += trilinear prolongation of u_coarse; prolong (devd.cu:118-223).
  * quirk = 1 reproduces the reference corner choice (devd.cu:181-185). */
 int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, void *stream);
@@ -293,6 +305,11 @@ int ocu_ir_residual(const ocu_grid3d *x, const ocu_grid3d *b, double bshift,
  * max, the others by sum) */
 int ocu_norm3_finalize(const double *ws, double *out, void *stream);
 int ocu_norm4_finalize(const double *ws, double *out, void *stream);
+/* the same with a completion signal (no counterpart in the reference): after
+ * the 3 (4) results, with a system-scope fence between, `tag` (nonzero) goes
+ * into the 8 bytes of out[3] (out[4]) */
+int ocu_norm3_finalize_tagged(const double *ws, double *out, unsigned long long tag, void *stream);
+int ocu_norm4_finalize_tagged(const double *ws, double *out, unsigned long long tag, void *stream);
 /* shift[0] = (xsum + inv_scale * sum of e over the interior) / (nx*ny*nz),
  * the interior mean of x + e*inv_scale when xsum is the interior sum of x
  * (e FP32; ws: OCU_NORM_PARTS doubles; shift: device memory). */

@@ -67,6 +67,7 @@ HIP_SIGS = {
     "ocu_grid_add_scalar": (I, [GP, D, P]),
     "ocu_reduce_workspace_bytes": (C.c_size_t, []),
     "ocu_grid_reduce": (I, [GP, I, P, DP, P]),
+    "ocu_grid_reduce_tagged": (I, [GP, I, P, DP, C.c_ulonglong, P]),
     "ocu_grid_reduce_noalloc": (I, [GP, I, DP, P]),
     "ocu_bc_fill_level1": (I, [GP, BP, D, D, D, P]),
     "ocu_bc_fill_nocorners": (I, [GP, BP, D, D, D, P]),
@@ -88,6 +89,7 @@ HIP_SIGS = {
     "ocu_mg_residual_restrict": (I, [GP, GP, GP, GP, D, D, D, D, DP, P]),
     "ocu_mg_residual_norm": (I, [GP, GP, GP, D, D, D, D, DP, P]),
     "ocu_norm_finalize": (I, [DP, DP, P]),
+    "ocu_norm_finalize_tagged": (I, [DP, DP, C.c_ulonglong, P]),
     "ocu_mg_prolong": (I, [GP, GP, I, P]),
     "ocu_mg_prolong_bc": (I, [GP, GP, I, BP, D, D, D, P]),
     "ocu_grid_clear_ghosts": (I, [GP, P]),
@@ -103,10 +105,12 @@ HIP_SIGS = {
     "ocu_mg_coarse_relax": (I, [GP, GP, I, I, MP, BP, D, D, D, I, P]),
     "ocu_ir_residual": (I, [GP, GP, D, GP, D, D, D, D, D, DP, P]),
     "ocu_norm3_finalize": (I, [DP, DP, P]),
+    "ocu_norm3_finalize_tagged": (I, [DP, DP, C.c_ulonglong, P]),
     "ocu_ir_correct": (I, [GP, GP, D, P]),
     "ocu_ir_correct_ex": (I, [GP, GP, D, DP, I, P]),
     "ocu_ir_shift": (I, [GP, D, D, DP, DP, P]),
     "ocu_norm4_finalize": (I, [DP, DP, P]),
+    "ocu_norm4_finalize_tagged": (I, [DP, DP, C.c_ulonglong, P]),
     "ocu_grid_sum": (I, [GP, DP, DP, P]),
     "ocu_norm_parts": (I, []),
     "ocu_divergence": (I, [GP, GP, GP, GP, I, I, I, D, D, D, P]),

@@ -159,12 +159,20 @@ __global__ __launch_bounds__(256) void k_reduce_max_rows(DGrid<T> g, double *par
   if (threadIdx.x == 0) parts[blockIdx.x] = r;
 }
 
-__global__ void k_reduce_stage2(int op, const double *parts, int n, double *out) {
+// tag != 0: after the result, with a system-scope fence between, the tag
+// goes into out[1] (host memory that the host polls)
+__global__ void k_reduce_stage2(int op, const double *parts, int n, double *out, unsigned long long tag) {
   __shared__ double sh[8];
   double acc = red_identity(op);
   for (int q = threadIdx.x; q < n; q += blockDim.x) acc = red_combine(op, acc, parts[q]);
   double r = block_reduce(op, acc, sh);
-  if (threadIdx.x == 0) *out = r;
+  if (threadIdx.x == 0) {
+    *out = r;
+    if (tag) {
+      __threadfence_system();
+      *reinterpret_cast<volatile unsigned long long *>(out + 1) = tag;
+    }
+  }
 }
 
 }  // namespace
@@ -214,7 +222,8 @@ int check_reduce(const ocu_grid3d *g, int op, const double *out, const char *wha
   return OCU_OK;
 }
 
-int launch_reduce(const ocu_grid3d *g, int op, double *parts, double *out, void *stream, const char *what) {
+int launch_reduce(const ocu_grid3d *g, int op, double *parts, double *out, void *stream, const char *what,
+                  unsigned long long tag = 0) {
 #define RMAX(TT, OP) k_reduce_max_rows<TT, OP><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<TT>(g), parts)
   if (op <= 2 && g->dtype == OCU_F64) {
     if (op == 0) RMAX(double, 0);
@@ -230,7 +239,7 @@ int launch_reduce(const ocu_grid3d *g, int op, double *parts, double *out, void 
     k_reduce_stage1<float><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<float>(g), op, parts);
   }
 #undef RMAX
-  k_reduce_stage2<<<1, 512, 0, S(stream)>>>(op, parts, NORM_PARTS, out);
+  k_reduce_stage2<<<1, 512, 0, S(stream)>>>(op, parts, NORM_PARTS, out, tag);
   OCU_HIP_LAUNCHED(what);
 }
 
@@ -332,6 +341,14 @@ int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, v
   if (int e = check_reduce(g, op, out, "ocu_grid_reduce")) return e;
   OCU_REQUIRE(workspace, "ocu_grid_reduce: null workspace");
   return launch_reduce(g, op, (double *)workspace, out, stream, "ocu_grid_reduce");
+}
+
+int ocu_grid_reduce_tagged(const ocu_grid3d *g, int op, void *workspace, double *out, unsigned long long tag,
+                           void *stream) {
+  if (int e = check_reduce(g, op, out, "ocu_grid_reduce_tagged")) return e;
+  OCU_REQUIRE(workspace, "ocu_grid_reduce_tagged: null workspace");
+  OCU_REQUIRE(tag, "ocu_grid_reduce_tagged: the tag must not be 0");
+  return launch_reduce(g, op, (double *)workspace, out, stream, "ocu_grid_reduce_tagged", tag);
 }
 
 int ocu_grid_reduce_noalloc(const ocu_grid3d *g, int op, double *out, void *stream) {

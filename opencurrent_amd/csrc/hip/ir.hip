@@ -177,9 +177,11 @@ __global__ __launch_bounds__(256) void k_stats3(DGrid<double> g, int tiles_k, in
 }
 
 // Reduces NW-tuples of per-block partials: slot 1 by max, the others by sum
-// (sum r^2, max |r|, sum r [, sum x]).  512 threads, fixed order.
+// (sum r^2, max |r|, sum r [, sum x]).  512 threads, fixed order.  tag != 0:
+// after the results, with a system-scope fence between, the tag goes into
+// out[NW] (host memory that the host polls).
 template <int NW>
-__global__ void k_norm_finalize(const double *parts, int n, double *out) {
+__global__ void k_norm_finalize(const double *parts, int n, double *out, unsigned long long tag) {
   __shared__ double sh[NW][8];
   double a[NW];
 #pragma unroll
@@ -198,6 +200,10 @@ __global__ void k_norm_finalize(const double *parts, int n, double *out) {
       double r = 0;
       for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = q == 1 ? fmax(r, sh[q][w]) : r + sh[q][w];
       out[q] = r;
+    }
+    if (tag) {
+      __threadfence_system();
+      *reinterpret_cast<volatile unsigned long long *>(out + NW) = tag;
     }
   }
 }
@@ -439,14 +445,26 @@ int ocu_grid_stats3(const ocu_grid3d *g, double *ws, void *stream) {
 
 int ocu_norm3_finalize(const double *ws, double *out, void *stream) {
   OCU_REQUIRE(ws && out, "ocu_norm3_finalize: null pointer");
-  k_norm_finalize<3><<<1, 512, 0, S(stream)>>>(ws, NORM_PARTS, out);
+  k_norm_finalize<3><<<1, 512, 0, S(stream)>>>(ws, NORM_PARTS, out, 0ull);
   OCU_HIP_LAUNCHED("ocu_norm3_finalize");
 }
 
 int ocu_norm4_finalize(const double *ws, double *out, void *stream) {
   OCU_REQUIRE(ws && out, "ocu_norm4_finalize: null pointer");
-  k_norm_finalize<4><<<1, 512, 0, S(stream)>>>(ws, NORM_PARTS, out);
+  k_norm_finalize<4><<<1, 512, 0, S(stream)>>>(ws, NORM_PARTS, out, 0ull);
   OCU_HIP_LAUNCHED("ocu_norm4_finalize");
+}
+
+int ocu_norm3_finalize_tagged(const double *ws, double *out, unsigned long long tag, void *stream) {
+  OCU_REQUIRE(ws && out && tag, "ocu_norm3_finalize_tagged: null pointer or zero tag");
+  k_norm_finalize<3><<<1, 512, 0, S(stream)>>>(ws, NORM_PARTS, out, tag);
+  OCU_HIP_LAUNCHED("ocu_norm3_finalize_tagged");
+}
+
+int ocu_norm4_finalize_tagged(const double *ws, double *out, unsigned long long tag, void *stream) {
+  OCU_REQUIRE(ws && out && tag, "ocu_norm4_finalize_tagged: null pointer or zero tag");
+  k_norm_finalize<4><<<1, 512, 0, S(stream)>>>(ws, NORM_PARTS, out, tag);
+  OCU_HIP_LAUNCHED("ocu_norm4_finalize_tagged");
 }
 
 int ocu_ir_shift(const ocu_grid3d *e, double inv_scale, double xsum, double *ws, double *shift, void *stream) {

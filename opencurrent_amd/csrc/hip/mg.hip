@@ -1056,7 +1056,9 @@ __global__ __launch_bounds__(256) void k_res_norm(DGrid<T> u, DGrid<T> b, DGrid<
   }
 }
 
-__global__ void k_norm_finalize(const double *parts, int n, double *out) {
+// tag != 0: after the two results, with a system-scope fence between, the
+// tag goes into out[2] (host memory that the host polls)
+__global__ void k_norm_finalize(const double *parts, int n, double *out, unsigned long long tag) {
   __shared__ double sh[2][8];
   double s = 0, m = 0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
@@ -1072,6 +1074,10 @@ __global__ void k_norm_finalize(const double *parts, int n, double *out) {
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) { a += sh[0][w]; b = fmax(b, sh[1][w]); }
     out[0] = a;
     out[1] = b;
+    if (tag) {
+      __threadfence_system();
+      *reinterpret_cast<volatile unsigned long long *>(out + 2) = tag;
+    }
   }
 }
 
@@ -2507,8 +2513,15 @@ int ocu_mg_residual_norm(const ocu_grid3d *u, const ocu_grid3d *b, const ocu_gri
 
 int ocu_norm_finalize(const double *norm_ws, double *out, void *stream) {
   OCU_REQUIRE(norm_ws && out, "ocu_norm_finalize: null pointer");
-  k_norm_finalize<<<1, 512, 0, S(stream)>>>(norm_ws, NORM_PARTS, out);
+  k_norm_finalize<<<1, 512, 0, S(stream)>>>(norm_ws, NORM_PARTS, out, 0ull);
   OCU_HIP_LAUNCHED("ocu_norm_finalize");
+}
+
+int ocu_norm_finalize_tagged(const double *norm_ws, double *out, unsigned long long tag, void *stream) {
+  OCU_REQUIRE(norm_ws && out, "ocu_norm_finalize_tagged: null pointer");
+  OCU_REQUIRE(tag, "ocu_norm_finalize_tagged: the tag must not be 0");
+  k_norm_finalize<<<1, 512, 0, S(stream)>>>(norm_ws, NORM_PARTS, out, tag);
+  OCU_HIP_LAUNCHED("ocu_norm_finalize_tagged");
 }
 
 int ocu_mg_prolong(const ocu_grid3d *u_coarse, const ocu_grid3d *u_fine, int quirk, void *stream) {

@@ -336,9 +336,15 @@ bool Grid3DDevice<T>::reduce_double(int op, double &result) const {
   }
   void *s = ThreadManager::get_compute_stream();
   ocu_grid3d d = this->desc();
-  // the result goes straight into pinned host memory
-  if (!ocu_status_ok(ocu_grid_reduce(&d, op, g_red.ws, g_red.host_dev, s), "Grid3DDevice::reduce")) return false;
-  if (!ocu_status_ok(ocu_stream_synchronize(s), "Grid3DDevice::reduce")) return false;
+  // the result goes straight into pinned host memory, with a completion tag
+  // the host polls for
+  const unsigned long long tag = next_completion_tag();
+  if (!ocu_status_ok(ocu_grid_reduce_tagged(&d, op, g_red.ws, g_red.host_dev, tag, s), "Grid3DDevice::reduce"))
+    return false;
+  if (!wait_completion_tag(g_red.host + 1, tag, s)) {
+    printf("[ERROR] Grid3DDevice::reduce - the result did not arrive\n");
+    return false;
+  }
   result = *g_red.host;
   return true;
 }

@@ -182,6 +182,8 @@ class Sol_MultigridPressure3DDevice : public Sol_MultigridPressure3DBase {
   virtual void norms_read(int fine_level, double *l2, double *linf);
   virtual void run_segment(const std::function<void()> &body);
   bool _norms_pending;  // norm kernels launched, values not read yet
+  unsigned long long _norm_tag;  // their completion tag (0: read after a stream synchronisation)
+  bool _capturing;  // launches are being recorded into a graph (no tags: a replay would repeat them)
   // recorded segments: the solver state they start from (options, stream,
   // deferred clears and prolongations per level), the flags they leave, the
   // level-0 sweep counts they add, and the executable graph

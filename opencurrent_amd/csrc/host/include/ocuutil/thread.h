@@ -23,6 +23,14 @@ class ThreadManager {
   static void barrier_and_fence() { compute_fence(); }
 };
 
+// Results that a kernel writes into pinned host memory followed by a
+// completion tag (the *_tagged entry points of ocu_hip.h): a fresh nonzero
+// tag per read, and the wait for it - polling the 8 bytes at `slot` (host
+// address), or after a bounded poll synchronising `stream`.  False when the
+// stream reports an error or the tag never arrived.
+unsigned long long next_completion_tag();
+bool wait_completion_tag(const void *slot, unsigned long long tag, void *stream);
+
 }  // namespace ocu
 
 #endif

@@ -1,5 +1,7 @@
 // Host runtime pieces of the drop-in API: the compute stream, status
 // reporting, Equation::advance.
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 
@@ -45,6 +47,29 @@ void ThreadManager::set_compute_stream(void *stream) {
 }
 
 void ThreadManager::compute_fence() { ocu_stream_synchronize(get_compute_stream()); }
+
+unsigned long long next_completion_tag() {
+  static std::atomic<unsigned long long> last{0};
+  return ++last;  // starts at 1: 0 means no tag
+}
+
+// Polling costs a host core while it waits but answers a few microseconds
+// sooner than a stream synchronisation; a wait past 200 ms (long work ahead
+// of the tag, or a failed kernel) falls back to the synchronisation.
+bool wait_completion_tag(const void *slot, unsigned long long tag, void *stream) {
+  const volatile unsigned long long *p = static_cast<const volatile unsigned long long *>(slot);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned n = 1;; n++) {
+    if (*p == tag) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      return true;
+    }
+    if ((n & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) break;
+  }
+  if (ocu_stream_synchronize(stream) != OCU_OK) return false;
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return *p == tag;
+}
 
 bool ocu_status_ok(int status, const char *what) {
   if (status == OCU_OK) return true;

@@ -351,6 +351,8 @@ Sol_MultigridPressure3DDevice<T>::Sol_MultigridPressure3DDevice() {
   graph_records = graph_replays = 0;
   _graphs_failed = false;
   _norms_pending = false;
+  _norm_tag = 0;
+  _capturing = false;
   level0_sweeps[0] = level0_sweeps[1] = level0_sweeps[2] = 0;
   for (int l = 0; l < kMaxLevels; l++) _u_zero[l] = _u_pro[l] = _fused_lvl[l] = false;
 }
@@ -670,8 +672,14 @@ void Sol_MultigridPressure3DDevice<T>::restrict_residuals_launch(int fine_level,
   }
   if (norms) {
     // the finalize kernel writes the two results straight into pinned host
-    // memory: no copy launch, one stream synchronisation when they are read
-    ok_status(ocu_norm_finalize(_norm_ws, _norm_dev, s), "Sol_MultigridPressure3DDevice::restrict_residuals");
+    // memory and a completion tag after them: no copy launch, and the host
+    // polls for the tag rather than synchronising the stream
+    _norm_tag = _capturing ? 0 : next_completion_tag();
+    if (_norm_tag)
+      ok_status(ocu_norm_finalize_tagged(_norm_ws, _norm_dev, _norm_tag, s),
+                "Sol_MultigridPressure3DDevice::restrict_residuals");
+    else
+      ok_status(ocu_norm_finalize(_norm_ws, _norm_dev, s), "Sol_MultigridPressure3DDevice::restrict_residuals");
     _norms_pending = true;
   }
 }
@@ -680,8 +688,15 @@ template <typename T>
 void Sol_MultigridPressure3DDevice<T>::norms_read(int fine_level, double *l2, double *linf) {
   if (!_norms_pending) return;
   _norms_pending = false;
-  ok_status(ocu_stream_synchronize(ThreadManager::get_compute_stream()),
-            "Sol_MultigridPressure3DDevice::restrict_residuals");
+  void *s = ThreadManager::get_compute_stream();
+  if (_norm_tag) {
+    if (!wait_completion_tag(_norm_host + 2, _norm_tag, s)) {
+      printf("[ERROR] Sol_MultigridPressure3DDevice::restrict_residuals - norms did not arrive\n");
+      add_error();
+    }
+  } else {
+    ok_status(ocu_stream_synchronize(s), "Sol_MultigridPressure3DDevice::restrict_residuals");
+  }
   double n = (double)nx(fine_level) * ny(fine_level) * nz(fine_level);
   if (l2) *l2 = sqrt(_norm_host[0] / n);
   if (linf) *linf = _norm_host[1];
@@ -735,6 +750,7 @@ void Sol_MultigridPressure3DDevice<T>::run_segment(const std::function<void()> &
     }
     for (int q = 0; q < 3; q++) level0_sweeps[q] += g.sweeps[q];
     _norms_pending = true;
+    _norm_tag = 0;
     graph_replays++;
     return;
   }
@@ -743,10 +759,13 @@ void Sol_MultigridPressure3DDevice<T>::run_segment(const std::function<void()> &
   for (int l = 0; l < _num_levels; l++) z[l] = _u_zero[l], p[l] = _u_pro[l];
   long long sw[3] = {level0_sweeps[0], level0_sweeps[1], level0_sweeps[2]};
   const bool pending = _norms_pending, err = any_error();
+  const unsigned long long tag = _norm_tag;
   void *exec = nullptr;
   bool ok = ocu_stream_begin_capture(s) == OCU_OK;
   if (ok) {
+    _capturing = true;
     body();
+    _capturing = false;
     ok = ocu_stream_end_capture(s, &exec) == OCU_OK && exec && (err || !any_error());
   }
   if (!ok) {
@@ -754,6 +773,7 @@ void Sol_MultigridPressure3DDevice<T>::run_segment(const std::function<void()> &
     for (int l = 0; l < _num_levels; l++) _u_zero[l] = z[l], _u_pro[l] = p[l];
     for (int q = 0; q < 3; q++) level0_sweeps[q] = sw[q];
     _norms_pending = pending;
+    _norm_tag = tag;
     if (!err) clear_error();
     _graphs_failed = true;
     body();

@@ -99,8 +99,13 @@ bool Sol_MultigridPressureMixed3DDeviceD::fp64_residual(double bshift, double sc
   void *s = ThreadManager::get_compute_stream();
   ocu_grid3d x = _x.desc(), b = _rhs->desc(), r = _r32.desc();
   bool ok = ok_status(ocu_ir_residual(&x, &b, bshift, &r, scale, get_h(0), _fx, _fy, _fz, _ws, s), "ir_residual");
-  ok = ok && ok_status(ocu_norm4_finalize(_ws, _host_dev, s), "ir_residual");  // into pinned host memory
-  ok = ok && ok_status(ocu_stream_synchronize(s), "ir_residual");
+  // into pinned host memory, with a completion tag the host polls for
+  const unsigned long long tag = next_completion_tag();
+  ok = ok && ok_status(ocu_norm4_finalize_tagged(_ws, _host_dev, tag, s), "ir_residual");
+  if (ok && !wait_completion_tag(_host + 4, tag, s)) {
+    printf("[ERROR] Sol_MultigridPressureMixed3DDeviceD::fp64_residual - norms did not arrive\n");
+    ok = false;
+  }
   sumsq = _host[0];
   maxabs = _host[1];
   sum = _host[2];
@@ -190,9 +195,10 @@ bool Sol_MultigridPressureMixed3DDeviceD::do_fmg(double tolerance, int max_iter,
 
   // sum, sum of squares and max|b| in one pass
   ocu_grid3d bd = _rhs->desc();
+  const unsigned long long tag = next_completion_tag();
   check_ok(ok_status(ocu_grid_stats3(&bd, _ws, s), "rhs statistics") &&
-           ok_status(ocu_norm3_finalize(_ws, _host_dev, s), "rhs statistics") &&
-           ok_status(ocu_stream_synchronize(s), "rhs statistics"));
+           ok_status(ocu_norm3_finalize_tagged(_ws, _host_dev, tag, s), "rhs statistics") &&
+           wait_completion_tag(_host + 3, tag, s));
   const double bsq = _host[0], bmax = _host[1], bsum = _host[2];
   double bm = _singular ? bsum / n : 0.0;
   double var = bsq / n - bm * bm;

@@ -1,0 +1,84 @@
+"""GPU check of the tagged reduction entry point (ocu_grid_reduce_tagged):
+the result goes into pinned host memory and a completion tag after it, so a
+host that polls the tag slot - no stream synchronisation - finds the same
+value as ocu_grid_reduce with a device result, bit for bit, for every op and
+both precisions, on several calls in a row with fresh tags."""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+from opencurrent_amd import _lib
+from opencurrent_amd.device import DeviceGrid
+
+OPS = {0: "maxabs", 1: "max", 2: "min", 3: "sum", 4: "sqrsum", 5: "checknan"}
+
+
+class Pinned:
+    """eight pinned host doubles and their device-side address"""
+
+    def __init__(self, hip):
+        self.hip = hip
+        self.h = C.c_void_p()
+        _lib.check(hip.ocu_host_malloc(C.byref(self.h), 64), "ocu_host_malloc")
+        d = C.c_void_p()
+        _lib.check(hip.ocu_host_device_pointer(C.byref(d), self.h), "ocu_host_device_pointer")
+        self.dev = C.cast(d, _lib.DP)
+        self.vals = (C.c_double * 8).from_address(self.h.value)
+        self.tags = (C.c_ulonglong * 8).from_address(self.h.value)
+
+    def wait(self, slot, tag, limit_s=10.0):
+        t0 = time.monotonic()
+        while self.tags[slot] != tag:
+            assert time.monotonic() - t0 < limit_s, f"tag {tag} did not arrive in slot {slot}"
+        return self.vals[slot - 1]
+
+    def free(self):
+        self.hip.ocu_host_free(self.h)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tagged_reduce_matches_device_reduce(gpu, dtype):
+    import torch
+    hip = _lib.hip()
+    pin = Pinned(hip)
+    try:
+        ws = torch.zeros(hip.ocu_reduce_workspace_bytes() // 8, dtype=torch.float64, device="cuda")
+        out = torch.full((1,), np.nan, dtype=torch.float64, device="cuda")
+        gen = torch.Generator(device="cuda").manual_seed(7101)
+        tag = 1000
+        for n in [(64, 48, 130), (9, 17, 33)]:
+            g = DeviceGrid(*n, dtype=dtype)
+            g.tensor.uniform_(-1, 1, generator=gen)
+            for op in sorted(OPS):
+                print(f"tagged reduce op={OPS[op]} dtype={np.dtype(dtype).name} grid={n}", flush=True)
+                _lib.check(hip.ocu_grid_reduce(g.ptr(), op, ws.data_ptr(), C.cast(out.data_ptr(), _lib.DP), None),
+                           "ocu_grid_reduce")
+                torch.cuda.synchronize()
+                want = out.item()
+                tag += 1
+                pin.vals[0] = np.nan
+                _lib.check(hip.ocu_grid_reduce_tagged(g.ptr(), op, ws.data_ptr(), pin.dev, tag, None),
+                           "ocu_grid_reduce_tagged")
+                got = pin.wait(1, tag)
+                assert np.array_equal(np.float64(got), np.float64(want)), (OPS[op], n, got, want)
+        torch.cuda.synchronize()
+    finally:
+        pin.free()
+
+
+@pytest.mark.gpu
+def test_tagged_entry_points_reject_a_zero_tag(gpu):
+    import torch
+    hip = _lib.hip()
+    g = DeviceGrid(8, 8, 8, dtype=np.float64)
+    ws = torch.zeros(hip.ocu_reduce_workspace_bytes() // 8, dtype=torch.float64, device="cuda")
+    out = torch.zeros((4,), dtype=torch.float64, device="cuda")
+    dp = C.cast(out.data_ptr(), _lib.DP)
+    assert hip.ocu_grid_reduce_tagged(g.ptr(), 0, ws.data_ptr(), dp, 0, None) != 0
+    wsp = C.cast(ws.data_ptr(), _lib.DP)
+    assert hip.ocu_norm_finalize_tagged(wsp, dp, 0, None) != 0
+    assert hip.ocu_norm3_finalize_tagged(wsp, dp, 0, None) != 0
+    assert hip.ocu_norm4_finalize_tagged(wsp, dp, 0, None) != 0
