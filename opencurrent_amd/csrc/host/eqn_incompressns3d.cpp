@@ -226,9 +226,7 @@ bool Eqn_IncompressibleNS3D<T>::set_parameters(const Eqn_IncompressibleNS3DParam
 template <typename T>
 double Eqn_IncompressibleNS3D<T>::get_max_stable_timestep() const {
   T max_u = 0, max_v = 0, max_w = 0;
-  _u.reduce_maxabs(max_u);
-  _v.reduce_maxabs(max_v);
-  _w.reduce_maxabs(max_w);
+  Grid3DDevice<T>::reduce_maxabs3(_u, _v, _w, max_u, max_v, max_w);
   double ut = this->hx() / max_u;
   double vt = this->hy() / max_v;
   double wt = this->hz() / max_w;
@@ -406,9 +404,7 @@ bool Eqn_ScalarAdvection3D<T>::set_parameters(const Eqn_ScalarAdvection3DParams<
 template <typename T>
 double Eqn_ScalarAdvection3D<T>::get_max_stable_timestep() const {
   T max_u = 0, max_v = 0, max_w = 0;
-  u.reduce_maxabs(max_u);
-  v.reduce_maxabs(max_v);
-  w.reduce_maxabs(max_w);
+  Grid3DDevice<T>::reduce_maxabs3(u, v, w, max_u, max_v, max_w);
   double ut = hx() / max_u, vt = hy() / max_v, wt = hz() / max_w;
   if (!check_float(ut)) ut = 1e10;
   if (!check_float(vt)) vt = 1e10;

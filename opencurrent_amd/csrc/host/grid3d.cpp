@@ -349,6 +349,37 @@ bool Grid3DDevice<T>::reduce_double(int op, double &result) const {
   return true;
 }
 
+// each result and its tag in its own pair of pinned slots; the kernels run
+// in stream order and each fences its result before its tag, so the last
+// tag arriving means all three results are there
+template <typename T>
+bool Grid3DDevice<T>::reduce_maxabs3(const Grid3DDevice<T> &a, const Grid3DDevice<T> &b, const Grid3DDevice<T> &c,
+                                     T &ra, T &rb, T &rc) {
+  if (!g_red.ready()) {
+    printf("[ERROR] Grid3DDevice::reduce - scratch allocation failed\n");
+    return false;
+  }
+  void *s = ThreadManager::get_compute_stream();
+  const Grid3DDevice<T> *g[3] = {&a, &b, &c};
+  unsigned long long tag[3];
+  for (int q = 0; q < 3; q++) {
+    ocu_grid3d d = g[q]->desc();
+    tag[q] = next_completion_tag();
+    if (!ocu_status_ok(ocu_grid_reduce_tagged(&d, 0, g_red.ws, g_red.host_dev + 2 * q, tag[q], s),
+                       "Grid3DDevice::reduce"))
+      return false;
+  }
+  const unsigned long long *tags = reinterpret_cast<const unsigned long long *>(g_red.host);
+  if (!wait_completion_tag(g_red.host + 5, tag[2], s) || tags[1] != tag[0] || tags[3] != tag[1]) {
+    printf("[ERROR] Grid3DDevice::reduce - the results did not arrive\n");
+    return false;
+  }
+  ra = (T)g_red.host[0];
+  rb = (T)g_red.host[2];
+  rc = (T)g_red.host[4];
+  return true;
+}
+
 template <typename T> bool Grid3DDevice<T>::reduce_maxabs(T &r) const { double x; bool ok = reduce_double(0, x); r = (T)x; return ok; }
 template <typename T> bool Grid3DDevice<T>::reduce_max(T &r) const { double x; bool ok = reduce_double(1, x); r = (T)x; return ok; }
 template <typename T> bool Grid3DDevice<T>::reduce_min(T &r) const { double x; bool ok = reduce_double(2, x); r = (T)x; return ok; }

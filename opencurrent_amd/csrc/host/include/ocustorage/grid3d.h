@@ -200,6 +200,10 @@ class Grid3DDevice : public Grid3DTypedBase<T> {
 
   // reduction in double precision (op as ocu_grid_reduce), blocking
   bool reduce_double(int op, double &result) const;
+  // max |.| of three grids (a MAC velocity) in one blocking call: the three
+  // reductions are launched back to back and the host waits once
+  static bool reduce_maxabs3(const Grid3DDevice<T> &a, const Grid3DDevice<T> &b, const Grid3DDevice<T> &c, T &ra,
+                             T &rb, T &rc);
 
   // exchange the storage of two grids of one layout (no reference
   // counterpart): a kernel that writes a new version of a grid out of
