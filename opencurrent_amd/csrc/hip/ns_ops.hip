@@ -109,9 +109,10 @@ __global__ __launch_bounds__(256) void k_passive_advect_up2(const T *__restrict_
 // passive advection is dphi_c = -sum_axes (F_{c+1} - F_c)/h with face fluxes
 // F_f = u_f * I(u_f, phi_{f-1}, phi_f), linear in phi for fixed u and in u for
 // a fixed upwind selection.  For lam (adjoint of dphi, zero ghost layer) and
-// g_f = scale * ((lam_f - lam_{f-1}) / h), one thread per cell of the ghost
-// box gathers  lam_out = lam + sum_faces (g_f * u_f) * dI/dphi_c  and, as the
-// owner of its low faces, accumulates  adj_u_f += g_f * I(u_f, ...).
+// g_f = scale * ((lam_f - lam_{f-1}) / h), each cell of the ghost box
+// gathers  lam_out = lam + sum_faces (g_f * u_f) * dI/dphi_c  (x faces low
+// then high, then y, then z) and, as the owner of its low faces, accumulates
+// adj_u_f += g_f * I(u_f, ...).
 template <typename T>
 __device__ __forceinline__ T interp_w0(int scheme, T u) {
   if (scheme == OCU_IT_SECOND_ORDER_CENTERED) return (T).5;
@@ -123,59 +124,78 @@ __device__ __forceinline__ T interp_w1(int scheme, T u) {
   return u < 0 ? (T)1 : (T)0;
 }
 
+// Marching along x: a thread takes the cells (i, j, k) of planes i0 ..
+// i1-1 of the ghost box (planes -1 .. nx) and keeps lam at i-1, i, i+1, u
+// at the faces i and i+1 and phi at i-1, i of its (j, k) in registers from
+// plane to plane, so every plane of lam, u and phi is read once.
 template <typename T>
-__global__ __launch_bounds__(256) void k_passive_advect_adjoint(
+__global__ __launch_bounds__(256) void k_passive_advect_adjoint_march(
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w, const T *__restrict__ phi,
     const T *__restrict__ lam, T *__restrict__ out, T *__restrict__ au, T *__restrict__ av, T *__restrict__ aw,
-    Inv<T> ih, T sc, int64_t xs, int64_t ys, int nx, int ny, int nz, int scheme) {
+    Inv<T> ih, T sc, int64_t xs, int64_t ys, int nx, int ny, int nz, int scheme, int ichunk) {
   const int k = (int)(blockIdx.x * TK + threadIdx.x) - 1;
   const int j = (int)(blockIdx.y * TJ + threadIdx.y) - 1;
-  const int i = (int)blockIdx.z - 1;
+  const int i0 = (int)blockIdx.z * ichunk - 1, i1 = min(nx + 1, i0 + ichunk);
   if (j > ny || k > nz) return;
-  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  const bool ii = i >= 0 && i < nx, ji = j >= 0 && j < ny, ki = k >= 0 && k < nz;
-  const T lc = lam[id];
-  T acc = 0;
-  if (ji && ki) {
-    if (i >= 0) {
-      const T g = sc * (ih.x * (lc - lam[id - xs]));
-      const T f = u[id];
-      acc += (g * f) * interp_w1(scheme, f);
-      au[id] += g * interp(scheme, f, phi[id - xs], phi[id]);
+  const bool ji = j >= 0 && j < ny, ki = k >= 0 && k < nz;
+  const int64_t row = (int64_t)j * ys + k;
+  // planes -1 .. nx of lam and phi, -1 .. nx+1 of u are allocated
+  T lm = i0 >= 0 ? lam[(int64_t)(i0 - 1) * xs + row] : (T)0;
+  T lc = lam[(int64_t)i0 * xs + row];
+  T uc = i0 >= 0 ? u[(int64_t)i0 * xs + row] : (T)0;
+  T pm = i0 >= 0 ? phi[(int64_t)(i0 - 1) * xs + row] : (T)0;
+  for (int i = i0; i < i1; i++) {
+    const int64_t id = (int64_t)i * xs + row;
+    const bool ii = i >= 0 && i < nx;
+    const T lp = i < nx ? lam[id + xs] : (T)0;
+    const T un = i < nx ? u[id + xs] : (T)0;
+    const T pc = phi[id];
+    T acc = 0;
+    if (ji && ki) {
+      if (i >= 0) {
+        const T g = sc * (ih.x * (lc - lm));
+        const T f = uc;
+        acc += (g * f) * interp_w1(scheme, f);
+        au[id] += g * interp(scheme, f, pm, pc);
+      }
+      if (i < nx) {
+        const T g = sc * (ih.x * (lp - lc));
+        const T f = un;
+        acc += (g * f) * interp_w0(scheme, f);
+      }
     }
-    if (i < nx) {
-      const T g = sc * (ih.x * (lam[id + xs] - lc));
-      const T f = u[id + xs];
-      acc += (g * f) * interp_w0(scheme, f);
+    if (ii && ki) {
+      if (j >= 0) {
+        const T g = sc * (ih.y * (lc - lam[id - ys]));
+        const T f = v[id];
+        acc += (g * f) * interp_w1(scheme, f);
+        av[id] += g * interp(scheme, f, phi[id - ys], pc);
+      }
+      if (j < ny) {
+        const T g = sc * (ih.y * (lam[id + ys] - lc));
+        const T f = v[id + ys];
+        acc += (g * f) * interp_w0(scheme, f);
+      }
     }
+    if (ii && ji) {
+      if (k >= 0) {
+        const T g = sc * (ih.z * (lc - lam[id - 1]));
+        const T f = w[id];
+        acc += (g * f) * interp_w1(scheme, f);
+        aw[id] += g * interp(scheme, f, phi[id - 1], pc);
+      }
+      if (k < nz) {
+        const T g = sc * (ih.z * (lam[id + 1] - lc));
+        const T f = w[id + 1];
+        acc += (g * f) * interp_w0(scheme, f);
+      }
+    }
+    out[id] = lc + acc;
+    lm = lc;
+    lc = lp;
+    uc = un;
+    pm = pc;
   }
-  if (ii && ki) {
-    if (j >= 0) {
-      const T g = sc * (ih.y * (lc - lam[id - ys]));
-      const T f = v[id];
-      acc += (g * f) * interp_w1(scheme, f);
-      av[id] += g * interp(scheme, f, phi[id - ys], phi[id]);
-    }
-    if (j < ny) {
-      const T g = sc * (ih.y * (lam[id + ys] - lc));
-      const T f = v[id + ys];
-      acc += (g * f) * interp_w0(scheme, f);
-    }
-  }
-  if (ii && ji) {
-    if (k >= 0) {
-      const T g = sc * (ih.z * (lc - lam[id - 1]));
-      const T f = w[id];
-      acc += (g * f) * interp_w1(scheme, f);
-      aw[id] += g * interp(scheme, f, phi[id - 1], phi[id]);
-    }
-    if (k < nz) {
-      const T g = sc * (ih.z * (lam[id + 1] - lc));
-      const T f = w[id + 1];
-      acc += (g * f) * interp_w0(scheme, f);
-    }
-  }
-  out[id] = lc + acc;
 }
 
 // adjoint of the device no-corner fill: each face ghost is c * (source) +
@@ -555,17 +575,21 @@ int ocu_passive_advect_adjoint(const ocu_grid3d *u, const ocu_grid3d *v, const o
               "%s - invalid interpolation type %d", what, scheme);
   if ((int64_t)nx * ny * nz == 0) return OCU_OK;
   const int64_t xs = (int64_t)u->pny * u->pnz, ys = u->pnz;
-  const dim3 g((nz + 2 + TK - 1) / TK, (ny + 2 + TJ - 1) / TJ, nx + 2);
+  // planes per thread: long marches, but enough blocks to fill the chip
+  const int64_t tiles = (int64_t)((nz + 2 + TK - 1) / TK) * ((ny + 2 + TJ - 1) / TJ);
+  int ichunk = 16;
+  while (ichunk > 1 && tiles * ((nx + 2 + ichunk - 1) / ichunk) < 4096) ichunk /= 2;
+  const dim3 g((nz + 2 + TK - 1) / TK, (ny + 2 + TJ - 1) / TJ, (nx + 2 + ichunk - 1) / ichunk);
   if (u->dtype == OCU_F64)
-    k_passive_advect_adjoint<double><<<g, stencil_block(), 0, S(stream)>>>(
+    k_passive_advect_adjoint_march<double><<<g, stencil_block(), 0, S(stream)>>>(
         at0<double>(u), at0<double>(v), at0<double>(w), at0<double>(phi), at0<double>(lam), at0<double>(lam_out),
         at0<double>(adj_u), at0<double>(adj_v), at0<double>(adj_w), inv<double>(hx, hy, hz), (double)scale, xs, ys,
-        nx, ny, nz, scheme);
+        nx, ny, nz, scheme, ichunk);
   else
-    k_passive_advect_adjoint<float><<<g, stencil_block(), 0, S(stream)>>>(
+    k_passive_advect_adjoint_march<float><<<g, stencil_block(), 0, S(stream)>>>(
         at0<float>(u), at0<float>(v), at0<float>(w), at0<float>(phi), at0<float>(lam), at0<float>(lam_out),
         at0<float>(adj_u), at0<float>(adj_v), at0<float>(adj_w), inv<float>(hx, hy, hz), (float)scale, xs, ys, nx,
-        ny, nz, scheme);
+        ny, nz, scheme, ichunk);
   OCU_HIP_LAUNCHED(what);
 }
 
