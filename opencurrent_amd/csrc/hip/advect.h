@@ -29,6 +29,18 @@ struct GAt {
   const T *__restrict__ p;  // the cell
   __device__ __forceinline__ T operator()(int64_t o) const { return p[o]; }
 };
+// GAt with the samples at x offsets -xs, 0, +xs held in registers (a
+// kernel marching along x); with xs > ys > 1 known to the compiler every
+// offset test folds away
+template <typename T>
+struct XAt {
+  const T *__restrict__ p;
+  T m, c, n;
+  int64_t xs;
+  __device__ __forceinline__ T operator()(int64_t o) const {
+    return o == xs ? n : o == -xs ? m : o == 0 ? c : p[o];
+  }
+};
 
 // Self-advection tendencies of the u, v and w faces at a cell (flux form,
 // Advection3DF/D_apply_upwind_TEX, sol_selfadvection3ddev.cu:136-337)
@@ -81,8 +93,8 @@ __device__ __forceinline__ void self_advect_cell(const T *__restrict__ U, const 
 
 // Passive advection tendency of phi at a cell
 // (Sol_PassiveAdvection3D_apply_interp, sol_passiveadvection3ddev.cu:31-78)
-template <typename T, typename A>
-__device__ __forceinline__ T passive_advect_at(const A &u, const A &v, const A &w, const A &phi, int64_t xs,
+template <typename T, typename AU, typename AV, typename AW, typename AP>
+__device__ __forceinline__ T passive_advect_at(const AU &u, const AV &v, const AW &w, const AP &phi, int64_t xs,
                                                int64_t ys, Inv<T> ih, int scheme) {
   T p = phi(0);
   T u_pi = u(xs), u0 = u(0);

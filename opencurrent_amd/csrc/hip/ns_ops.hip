@@ -58,18 +58,33 @@ __global__ __launch_bounds__(256) void k_self_advect(const T *__restrict__ U, co
   DW[id] = dw;
 }
 
-// Sol_PassiveAdvection3D_apply_interp (sol_passiveadvection3ddev.cu:31-78)
+// Sol_PassiveAdvection3D_apply_interp (sol_passiveadvection3ddev.cu:31-78),
+// marching along x: a thread takes the cells (i, j, k) of planes i0 .. i1-1
+// and keeps phi at i-1, i, i+1 and u at the faces i, i+1 of its (j, k) in
+// registers, so every plane of u and phi is read once.
 template <typename T>
-__global__ __launch_bounds__(256) void k_passive_advect(const T *__restrict__ u, const T *__restrict__ v,
-                                                        const T *__restrict__ w, const T *__restrict__ phi,
-                                                        T *__restrict__ d, Inv<T> ih, int64_t xs, int64_t ys, int nx,
-                                                        int ny, int nz, int scheme) {
-  int k = blockIdx.x * TK + threadIdx.x;
-  int j = blockIdx.y * TJ + threadIdx.y;
-  int i = blockIdx.z;
+__global__ __launch_bounds__(256) void k_passive_advect_march(const T *__restrict__ u, const T *__restrict__ v,
+                                                              const T *__restrict__ w, const T *__restrict__ phi,
+                                                              T *__restrict__ d, Inv<T> ih, int64_t xs, int64_t ys,
+                                                              int nx, int ny, int nz, int scheme, int ichunk) {
+  __builtin_assume(xs > ys && ys > 1);
+  const int k = blockIdx.x * TK + threadIdx.x;
+  const int j = blockIdx.y * TJ + threadIdx.y;
+  const int i0 = blockIdx.z * ichunk, i1 = min(nx, i0 + ichunk);
   if (j >= ny || k >= nz) return;
-  const int64_t id = (int64_t)i * xs + (int64_t)j * ys + k;
-  d[id] = passive_advect_cell(u, v, w, phi, id, xs, ys, ih, scheme);
+  const int64_t row = (int64_t)j * ys + k;
+  // planes -1 .. nx of phi and -1 .. nx+1 of u are allocated
+  T pm = phi[(int64_t)(i0 - 1) * xs + row], pc = phi[(int64_t)i0 * xs + row];
+  T uc = u[(int64_t)i0 * xs + row];
+  for (int i = i0; i < i1; i++) {
+    const int64_t id = (int64_t)i * xs + row;
+    const T pn = phi[id + xs], un = u[id + xs];
+    d[id] = passive_advect_at<T>(XAt<T>{u + id, (T)0, uc, un, xs}, GAt<T>{v + id}, GAt<T>{w + id},
+                                 XAt<T>{phi + id, pm, pc, pn, xs}, xs, ys, ih, scheme);
+    pm = pc;
+    pc = pn;
+    uc = un;
+  }
 }
 
 // ---- second-order (linear) upwind (no reference counterpart; advect.h) ----
@@ -548,14 +563,19 @@ int ocu_passive_advect_ex(const ocu_grid3d *u, const ocu_grid3d *v, const ocu_gr
           inv<float>(hx, hy, hz), xs, ys, nx, ny, nz, periodic_axes);
     OCU_HIP_LAUNCHED("ocu_passive_advect");
   }
+  // planes per thread: long marches, but enough blocks to fill the chip
+  const int64_t tiles = (int64_t)g.x * g.y;
+  int ichunk = 16;
+  while (ichunk > 1 && tiles * ((nx + ichunk - 1) / ichunk) < 4096) ichunk /= 2;
+  const dim3 gm(g.x, g.y, (nx + ichunk - 1) / ichunk);
   if (u->dtype == OCU_F64)
-    k_passive_advect<double><<<g, stencil_block(), 0, S(stream)>>>(at0<double>(u), at0<double>(v), at0<double>(w),
-                                                                   at0<double>(phi), at0<double>(dphi),
-                                                                   inv<double>(hx, hy, hz), xs, ys, nx, ny, nz, scheme);
+    k_passive_advect_march<double><<<gm, stencil_block(), 0, S(stream)>>>(
+        at0<double>(u), at0<double>(v), at0<double>(w), at0<double>(phi), at0<double>(dphi), inv<double>(hx, hy, hz),
+        xs, ys, nx, ny, nz, scheme, ichunk);
   else
-    k_passive_advect<float><<<g, stencil_block(), 0, S(stream)>>>(at0<float>(u), at0<float>(v), at0<float>(w),
-                                                                  at0<float>(phi), at0<float>(dphi),
-                                                                  inv<float>(hx, hy, hz), xs, ys, nx, ny, nz, scheme);
+    k_passive_advect_march<float><<<gm, stencil_block(), 0, S(stream)>>>(
+        at0<float>(u), at0<float>(v), at0<float>(w), at0<float>(phi), at0<float>(dphi), inv<float>(hx, hy, hz), xs,
+        ys, nx, ny, nz, scheme, ichunk);
   OCU_HIP_LAUNCHED("ocu_passive_advect");
 }
 
