@@ -553,7 +553,7 @@ def main():
                        "kernel": names[dom], "avg_launch_us": round(kms * 1e3, 2), "alg_bytes_per_launch": kbytes,
                        "launches_per_solve": round(counts[kinds[dom]], 2),
                        "us_per_solve": round(per_solve_us[dom], 1)}
-    # north_star's smoother figure: all level-0 sweeps of a solve, algorithmic
+    # the smoother figure: all level-0 sweeps of a solve, algorithmic
     # bytes over time, weighted by how often the solver runs each kind
     tot_b = sum(counts[kinds[k]] * v[1] for k, v in sweeps.items())
     tot_s = sum(counts[kinds[k]] * v[0] / 1e3 for k, v in sweeps.items())

@@ -3,7 +3,7 @@
  *
  * CPU restatement of OpenCurrent's Grid3D layout, ghost-cell fills and grid
  * operations.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
- * baseline use it, as the checker; the product never links it.
+ * baseline use it, to compare results against; the product never links it.
  *
  * Every function names the reference lines it restates.  Floating-point
  * expressions keep the reference's operand order and the precision each
