@@ -1,7 +1,7 @@
 """Loader of the CPU oracle (oracle/_build/libocu_oracle.so) for the tests.
 
 The oracle is test infrastructure: it restates the reference algorithm on
-the CPU and is only ever used as the checker."""
+the CPU and is only ever used to compare results against."""
 import ctypes as C
 import os
 import subprocess
