@@ -1,8 +1,9 @@
-"""GPU check of the tagged reduction entry point (ocu_grid_reduce_tagged):
-the result goes into pinned host memory and a completion tag after it, so a
-host that polls the tag slot - no stream synchronisation - finds the same
-value as ocu_grid_reduce with a device result, bit for bit, for every op and
-both precisions, on several calls in a row with fresh tags."""
+"""GPU checks of the tagged entry points (ocu_grid_reduce_tagged and the
+ocu_norm*_finalize_tagged): the results go into pinned host memory and a
+completion tag after them, so a host that polls the tag slot - no stream
+synchronisation - finds the same values as the untagged entry points with
+device results, bit for bit, for every reduction op and both precisions, on
+several calls in a row with fresh tags."""
 import ctypes as C
 import time
 
@@ -82,3 +83,65 @@ def test_tagged_entry_points_reject_a_zero_tag(gpu):
     assert hip.ocu_norm_finalize_tagged(wsp, dp, 0, None) != 0
     assert hip.ocu_norm3_finalize_tagged(wsp, dp, 0, None) != 0
     assert hip.ocu_norm4_finalize_tagged(wsp, dp, 0, None) != 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tagged_norm_finalizes_match_untagged(gpu, dtype):
+    """ocu_norm_finalize_tagged / ocu_norm3_finalize_tagged /
+    ocu_norm4_finalize_tagged: the results polled from pinned host memory
+    equal the untagged finalizes' (device memory), bit for bit, and the tag
+    lands in the slot after them"""
+    import torch
+    hip = _lib.hip()
+    pin = Pinned(hip)
+    try:
+        parts = hip.ocu_norm_parts()
+        ws = torch.zeros(4 * parts + 64, dtype=torch.float64, device="cuda")
+        wsp = C.cast(ws.data_ptr(), _lib.DP)
+        out = torch.full((8,), np.nan, dtype=torch.float64, device="cuda")
+        outp = C.cast(out.data_ptr(), _lib.DP)
+        gen = torch.Generator(device="cuda").manual_seed(7201)
+        n = (40, 36, 70)
+        u = DeviceGrid(*n, dtype=dtype)
+        b = DeviceGrid(*n, dtype=dtype)
+        r = DeviceGrid(*n, dtype=dtype)
+        for g in (u, b):
+            g.tensor.uniform_(-1, 1, generator=gen)
+        h = 1.0 / 64
+        tag = 5000
+
+        def untagged(fn, k):
+            _lib.check(fn(wsp, outp, None), "finalize")
+            torch.cuda.synchronize()
+            return out[:k].cpu().numpy().copy()
+
+        def tagged(fn, k):
+            nonlocal tag
+            tag += 1
+            for q in range(8):
+                pin.vals[q] = np.nan
+            _lib.check(fn(wsp, pin.dev, tag, None), "finalize_tagged")
+            pin.wait(k, tag)
+            return np.array([pin.vals[q] for q in range(k)])
+
+        print(f"tagged norm finalize dtype={np.dtype(dtype).name}", flush=True)
+        _lib.check(hip.ocu_mg_residual_norm(u.ptr(), b.ptr(), r.ptr(), h, 1.0, 1.0, 1.0, wsp, None), "norm")
+        want = untagged(hip.ocu_norm_finalize, 2)
+        got = tagged(hip.ocu_norm_finalize_tagged, 2)
+        assert np.array_equal(got, want) and np.isfinite(want).all(), (got, want)
+        if dtype == np.float64:
+            print("tagged norm3/norm4 finalize", flush=True)
+            _lib.check(hip.ocu_grid_stats3(u.ptr(), wsp, None), "stats3")
+            want = untagged(hip.ocu_norm3_finalize, 3)
+            got = tagged(hip.ocu_norm3_finalize_tagged, 3)
+            assert np.array_equal(got, want) and np.isfinite(want).all(), (got, want)
+            r32 = DeviceGrid(*n, dtype=np.float32)
+            _lib.check(hip.ocu_ir_residual(u.ptr(), b.ptr(), 0.25, r32.ptr(), 0.5, h, 1.0, 1.0, 1.0, wsp, None),
+                       "ir_residual")
+            want = untagged(hip.ocu_norm4_finalize, 4)
+            got = tagged(hip.ocu_norm4_finalize_tagged, 4)
+            assert np.array_equal(got, want) and np.isfinite(want).all(), (got, want)
+        torch.cuda.synchronize()
+    finally:
+        pin.free()
