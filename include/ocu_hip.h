@@ -96,6 +96,17 @@ int ocu_grid_reduce(const ocu_grid3d *g, int op, void *workspace, double *out, v
  * (an acquire read) then finds the result in out[0]. */
 int ocu_grid_reduce_tagged(const ocu_grid3d *g, int op, void *workspace, double *out, unsigned long long tag,
                            void *stream);
+/* max |a|, max |b| and max |c| over the interiors of three grids of one
+ * type in one pass, with the completion signal of ocu_grid_reduce_tagged
+ * (no counterpart in the reference; it takes the place of the three
+ * reduce_maxabs calls of get_max_stable_timestep, eqn_incompressns3d.cu:
+ * 302-307): out is pinned host memory as the device sees it with room for
+ * four doubles; after the maxima out[0..2], with a system-scope fence
+ * between, out[3] receives `tag` (nonzero).  `workspace` must hold
+ * 3 * ocu_reduce_workspace_bytes() bytes of device memory.  The maxima are
+ * those of three ocu_grid_reduce calls with op 0, bit for bit. */
+int ocu_grid_maxabs3_tagged(const ocu_grid3d *a, const ocu_grid3d *b, const ocu_grid3d *c, void *workspace,
+                            double *out, unsigned long long tag, void *stream);
 /* ocu_grid_reduce without the workspace argument and without any allocation:
  * no counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP-graph capture, tight serving loops).  The partials

@@ -68,6 +68,7 @@ HIP_SIGS = {
     "ocu_reduce_workspace_bytes": (C.c_size_t, []),
     "ocu_grid_reduce": (I, [GP, I, P, DP, P]),
     "ocu_grid_reduce_tagged": (I, [GP, I, P, DP, C.c_ulonglong, P]),
+    "ocu_grid_maxabs3_tagged": (I, [GP, GP, GP, P, DP, C.c_ulonglong, P]),
     "ocu_grid_reduce_noalloc": (I, [GP, I, DP, P]),
     "ocu_bc_fill_level1": (I, [GP, BP, D, D, D, P]),
     "ocu_bc_fill_nocorners": (I, [GP, BP, D, D, D, P]),

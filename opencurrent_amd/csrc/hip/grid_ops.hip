@@ -138,9 +138,7 @@ __global__ void k_reduce_stage1(DGrid<T> g, int op, double *parts) {
 // round-robin over the waves), every lane keeps four loads of a row in
 // flight; one partial per block as in k_reduce_stage1
 template <typename T, int OP>
-__global__ __launch_bounds__(256) void k_reduce_max_rows(DGrid<T> g, double *parts) {
-  __shared__ double sh[8];
-  double acc = red_identity(OP);
+__device__ __forceinline__ double max_rows_acc(const DGrid<T> &g, double acc) {
   const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * 4, w0 = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int rows = g.nx * g.ny, nz = g.nz;
@@ -155,8 +153,49 @@ __global__ __launch_bounds__(256) void k_reduce_max_rows(DGrid<T> g, double *par
     }
     for (; k < nz; k += 64) acc = red_combine(OP, acc, red_process(OP, (double)p[k]));
   }
-  double r = block_reduce(OP, acc, sh);
+  return acc;
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_reduce_max_rows(DGrid<T> g, double *parts) {
+  __shared__ double sh[8];
+  double r = block_reduce(OP, max_rows_acc<T, OP>(g, red_identity(OP)), sh);
   if (threadIdx.x == 0) parts[blockIdx.x] = r;
+}
+
+// max |a|, max |b|, max |c| of three grids in one pass: block b's partials
+// go to parts[b], parts[NORM_PARTS + b], parts[2 * NORM_PARTS + b]
+template <typename T>
+__global__ __launch_bounds__(256) void k_maxabs3_rows(DGrid<T> a, DGrid<T> b, DGrid<T> c, double *parts) {
+  __shared__ double sh[8];
+  const double ra = block_reduce(0, max_rows_acc<T, 0>(a, 0.0), sh);
+  const double rb = block_reduce(0, max_rows_acc<T, 0>(b, 0.0), sh);
+  const double rc = block_reduce(0, max_rows_acc<T, 0>(c, 0.0), sh);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = ra;
+    parts[NORM_PARTS + blockIdx.x] = rb;
+    parts[2 * NORM_PARTS + blockIdx.x] = rc;
+  }
+}
+
+// the three maxima into out[0..2]; then, with a system-scope fence
+// between, the tag into out[3] (host memory that the host polls)
+__global__ void k_maxabs3_stage2(const double *parts, double *out, unsigned long long tag) {
+  __shared__ double sh[8];
+  double r[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < NORM_PARTS; t += blockDim.x) acc = red_combine(0, acc, parts[q * NORM_PARTS + t]);
+    r[q] = block_reduce(0, acc, sh);
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
+    __threadfence_system();
+    *reinterpret_cast<volatile unsigned long long *>(out + 3) = tag;
+  }
 }
 
 // tag != 0: after the result, with a system-scope fence between, the tag
@@ -349,6 +388,25 @@ int ocu_grid_reduce_tagged(const ocu_grid3d *g, int op, void *workspace, double 
   OCU_REQUIRE(workspace, "ocu_grid_reduce_tagged: null workspace");
   OCU_REQUIRE(tag, "ocu_grid_reduce_tagged: the tag must not be 0");
   return launch_reduce(g, op, (double *)workspace, out, stream, "ocu_grid_reduce_tagged", tag);
+}
+
+int ocu_grid_maxabs3_tagged(const ocu_grid3d *a, const ocu_grid3d *b, const ocu_grid3d *c, void *workspace,
+                            double *out, unsigned long long tag, void *stream) {
+  const char *what = "ocu_grid_maxabs3_tagged";
+  for (const ocu_grid3d *g : {a, b, c})
+    if (int e = check_reduce(g, 0, out, what)) return e;
+  OCU_REQUIRE(a->dtype == b->dtype && a->dtype == c->dtype, "%s: the grids' types differ", what);
+  OCU_REQUIRE(workspace, "%s: null workspace", what);
+  OCU_REQUIRE(tag, "%s: the tag must not be 0", what);
+  double *parts = (double *)workspace;
+  if (a->dtype == OCU_F64)
+    k_maxabs3_rows<double><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<double>(a), dgrid<double>(b), dgrid<double>(c),
+                                                              parts);
+  else
+    k_maxabs3_rows<float><<<NORM_PARTS, 256, 0, S(stream)>>>(dgrid<float>(a), dgrid<float>(b), dgrid<float>(c),
+                                                             parts);
+  k_maxabs3_stage2<<<1, 512, 0, S(stream)>>>(parts, out, tag);
+  OCU_HIP_LAUNCHED(what);
 }
 
 int ocu_grid_reduce_noalloc(const ocu_grid3d *g, int op, double *out, void *stream) {

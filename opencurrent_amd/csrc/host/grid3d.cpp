@@ -317,7 +317,7 @@ struct ReduceScratch {
   double *host_dev = nullptr;  // host as the device sees it
   bool ready() {
     if (ws && host_dev) return true;
-    if (!ws && ocu_malloc(&ws, ocu_reduce_workspace_bytes() + 64) != OCU_OK) return false;
+    if (!ws && ocu_malloc(&ws, 3 * ocu_reduce_workspace_bytes() + 64) != OCU_OK) return false;
     void *h = nullptr, *hd = nullptr;
     if (ocu_host_malloc(&h, 64) != OCU_OK || ocu_host_device_pointer(&hd, h) != OCU_OK) return false;
     host = (double *)h;
@@ -349,9 +349,8 @@ bool Grid3DDevice<T>::reduce_double(int op, double &result) const {
   return true;
 }
 
-// each result and its tag in its own pair of pinned slots; the kernels run
-// in stream order and each fences its result before its tag, so the last
-// tag arriving means all three results are there
+// the three maxima from one pass over the grids (one partials launch and
+// one fold), polled from the pinned slots 0..2 once the tag is in slot 3
 template <typename T>
 bool Grid3DDevice<T>::reduce_maxabs3(const Grid3DDevice<T> &a, const Grid3DDevice<T> &b, const Grid3DDevice<T> &c,
                                      T &ra, T &rb, T &rc) {
@@ -360,23 +359,18 @@ bool Grid3DDevice<T>::reduce_maxabs3(const Grid3DDevice<T> &a, const Grid3DDevic
     return false;
   }
   void *s = ThreadManager::get_compute_stream();
-  const Grid3DDevice<T> *g[3] = {&a, &b, &c};
-  unsigned long long tag[3];
-  for (int q = 0; q < 3; q++) {
-    ocu_grid3d d = g[q]->desc();
-    tag[q] = next_completion_tag();
-    if (!ocu_status_ok(ocu_grid_reduce_tagged(&d, 0, g_red.ws, g_red.host_dev + 2 * q, tag[q], s),
-                       "Grid3DDevice::reduce"))
-      return false;
-  }
-  const unsigned long long *tags = reinterpret_cast<const unsigned long long *>(g_red.host);
-  if (!wait_completion_tag(g_red.host + 5, tag[2], s) || tags[1] != tag[0] || tags[3] != tag[1]) {
+  ocu_grid3d da = a.desc(), db = b.desc(), dc = c.desc();
+  const unsigned long long tag = next_completion_tag();
+  if (!ocu_status_ok(ocu_grid_maxabs3_tagged(&da, &db, &dc, g_red.ws, g_red.host_dev, tag, s),
+                     "Grid3DDevice::reduce"))
+    return false;
+  if (!wait_completion_tag(g_red.host + 3, tag, s)) {
     printf("[ERROR] Grid3DDevice::reduce - the results did not arrive\n");
     return false;
   }
   ra = (T)g_red.host[0];
-  rb = (T)g_red.host[2];
-  rc = (T)g_red.host[4];
+  rb = (T)g_red.host[1];
+  rc = (T)g_red.host[2];
   return true;
 }
 

@@ -1,5 +1,5 @@
-"""GPU checks of the tagged entry points (ocu_grid_reduce_tagged and the
-ocu_norm*_finalize_tagged): the results go into pinned host memory and a
+"""GPU checks of the tagged entry points (ocu_grid_reduce_tagged,
+ocu_grid_maxabs3_tagged and the ocu_norm*_finalize_tagged): the results go into pinned host memory and a
 completion tag after them, so a host that polls the tag slot - no stream
 synchronisation - finds the same values as the untagged entry points with
 device results, bit for bit, for every reduction op and both precisions, on
@@ -71,6 +71,53 @@ def test_tagged_reduce_matches_device_reduce(gpu, dtype):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_maxabs3_matches_three_device_reduces(gpu, dtype):
+    """ocu_grid_maxabs3_tagged: the three maxima of one pass equal three
+    ocu_grid_reduce(op 0) calls bit for bit, on MAC-shaped grids of
+    different extents (u: nx+1, v: ny+1, w: nz+1), where the largest
+    interior magnitude sits in a different grid each call and a larger one
+    in a ghost is left out, and on a second, smaller call in a row"""
+    import torch
+    hip = _lib.hip()
+    pin = Pinned(hip)
+    try:
+        ws = torch.zeros(3 * hip.ocu_reduce_workspace_bytes() // 8, dtype=torch.float64, device="cuda")
+        out = torch.full((1,), np.nan, dtype=torch.float64, device="cuda")
+        gen = torch.Generator(device="cuda").manual_seed(7301)
+        tag = 9000
+        for n in [(64, 48, 130), (9, 17, 33)]:
+            nx, ny, nz = n
+            gs = [DeviceGrid(nx + 1, ny, nz, dtype=dtype), DeviceGrid(nx, ny + 1, nz, dtype=dtype),
+                  DeviceGrid(nx, ny, nz + 1, dtype=dtype)]
+            for big in range(3):
+                for q, g in enumerate(gs):
+                    g.tensor.uniform_(-1, 1, generator=gen)
+                    g.full()[0, 0, 0] = 7.0  # a ghost corner: outside the interior
+                    if q == big:
+                        g.interior()[nx // 2, ny // 3, nz - 1] = -3.5
+                print(f"maxabs3 dtype={np.dtype(dtype).name} grid={n} largest in {big}", flush=True)
+                want = []
+                for g in gs:
+                    _lib.check(hip.ocu_grid_reduce(g.ptr(), 0, ws.data_ptr(), C.cast(out.data_ptr(), _lib.DP), None),
+                               "ocu_grid_reduce")
+                    torch.cuda.synchronize()
+                    want.append(out.item())
+                tag += 1
+                for q in range(4):
+                    pin.vals[q] = np.nan
+                _lib.check(hip.ocu_grid_maxabs3_tagged(gs[0].ptr(), gs[1].ptr(), gs[2].ptr(), ws.data_ptr(),
+                                                       pin.dev, tag, None), "ocu_grid_maxabs3_tagged")
+                pin.wait(3, tag)
+                got = [pin.vals[q] for q in range(3)]
+                assert np.array_equal(np.array(got), np.array(want)), (n, big, got, want)
+                assert max(want) == 3.5 and want[big] == 3.5
+        torch.cuda.synchronize()
+    finally:
+        pin.free()
+
+
+@pytest.mark.gpu
 def test_tagged_entry_points_reject_a_zero_tag(gpu):
     import torch
     hip = _lib.hip()
@@ -79,6 +126,7 @@ def test_tagged_entry_points_reject_a_zero_tag(gpu):
     out = torch.zeros((4,), dtype=torch.float64, device="cuda")
     dp = C.cast(out.data_ptr(), _lib.DP)
     assert hip.ocu_grid_reduce_tagged(g.ptr(), 0, ws.data_ptr(), dp, 0, None) != 0
+    assert hip.ocu_grid_maxabs3_tagged(g.ptr(), g.ptr(), g.ptr(), ws.data_ptr(), dp, 0, None) != 0
     wsp = C.cast(ws.data_ptr(), _lib.DP)
     assert hip.ocu_norm_finalize_tagged(wsp, dp, 0, None) != 0
     assert hip.ocu_norm3_finalize_tagged(wsp, dp, 0, None) != 0
