@@ -830,10 +830,12 @@ def test_ir_residual(gpu, size):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("size", [(1, 1, 1), (3, 5, 7), (17, 9, 64), (4, 3, 255), (2, 6, 300), (2, 2, 513)])
+@pytest.mark.parametrize("size", [(1, 1, 1), (3, 5, 7), (17, 9, 64), (4, 3, 255), (2, 6, 300), (2, 2, 513),
+                                  (100, 100, 40), (90, 100, 257)])
 def test_grid_reduce_max_ops(gpu, dtype, size):
-    """maxabs, max and min (rows per wave, four loads per lane in flight)
-    over row lengths below, at and past one pass of 256 cells, with the
+    """maxabs, max and min (rows per wave, four loads per lane in flight;
+    the last two sizes have more rows than the walk has waves) over
+    row lengths below, at and past one pass of 256 cells, with the
     extremes planted in the ghost layer (never read) and, in a second grid,
     in the last cell of the last row"""
     g = random_grid(*size, dtype, 901)

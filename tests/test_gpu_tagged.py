@@ -77,7 +77,8 @@ def test_maxabs3_matches_three_device_reduces(gpu, dtype):
     ocu_grid_reduce(op 0) calls bit for bit, on MAC-shaped grids of
     different extents (u: nx+1, v: ny+1, w: nz+1), where the largest
     interior magnitude sits in a different grid each call and a larger one
-    in a ghost is left out, and on a second, smaller call in a row"""
+    in a ghost is left out (the first size has more rows than the walk has
+    waves), and on smaller calls after it"""
     import torch
     hip = _lib.hip()
     pin = Pinned(hip)
@@ -86,7 +87,7 @@ def test_maxabs3_matches_three_device_reduces(gpu, dtype):
         out = torch.full((1,), np.nan, dtype=torch.float64, device="cuda")
         gen = torch.Generator(device="cuda").manual_seed(7301)
         tag = 9000
-        for n in [(64, 48, 130), (9, 17, 33)]:
+        for n in [(100, 96, 40), (64, 48, 130), (9, 17, 33)]:
             nx, ny, nz = n
             gs = [DeviceGrid(nx + 1, ny, nz, dtype=dtype), DeviceGrid(nx, ny + 1, nz, dtype=dtype),
                   DeviceGrid(nx, ny, nz + 1, dtype=dtype)]
